@@ -1,0 +1,937 @@
+/* swe_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline; see swe_oracle.h).
+ *
+ * A plain-C restatement of the reference explicit step, following the reference line by line:
+ *   set-up   SFEMaNS/GAUSS_POINTS_OBJET/gauss_points_2d_p1.f90:106-132,347-393 (P1 geometry)
+ *            SFEMaNS/FEMSUB/st_sparsekit.f90:788-863,1019-1084 (st_csr pattern)
+ *            SFEMaNS/FEMSUB_OBJET/fem_sparsekit.f90:6-53 (qs_00_M mass)
+ *            SOURCES/update.f90:18-132 (construct_matrices: lumped, diag, c_ij)
+ *   step     SOURCES/update.f90:134-935, SOURCES/fct.f90:5-42,
+ *            SOURCES/condlim.f90:21-90,425-486,879-981, SOURCES/main.f90:105-136,210-232
+ *
+ * Fortran's MAX(a,b) is evaluated as (a > b ? a : b) and MIN(a,b) as (a < b ? a : b), which is
+ * how flang lowers them; SUM over an array section is a sequential left-to-right loop from 0.
+ * The O(np^2) re-assignment of `rescale` inside `DO i` (SOURCES/update.f90:865-867) assigns the
+ * same array np times; it is evaluated once here, which gives the same bits.
+ */
+#include "swe_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static inline double fmaxF(double a, double b) { return (a > b) ? a : b; }
+static inline double fminF(double a, double b) { return (a < b) ? a : b; }
+
+/* ============================================================================================ */
+/* Set-up                                                                                         */
+/* ============================================================================================ */
+
+/* st_csr (st_sparsekit.f90:788-863): every node's row holds itself and every node sharing an
+ * element with it, sorted ascending and unique (tri_jlg, 1019-1084). */
+static int build_rows(int np, int me, const int *jj, int **rowptr_out, int **cols_out)
+{
+    int *cnt = (int *)calloc((size_t)np + 1, sizeof(int));
+    if (!cnt) return -1;
+    for (int m = 0; m < me; m++)
+        for (int a = 0; a < 3; a++) cnt[jj[3 * m + a] + 1] += 3;
+    for (int i = 0; i < np; i++) cnt[i + 1] += cnt[i];
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)cnt[np]);
+    int *fill = (int *)calloc((size_t)np, sizeof(int));
+    if (!tmp || !fill) { free(cnt); free(tmp); free(fill); return -1; }
+    for (int m = 0; m < me; m++)
+        for (int a = 0; a < 3; a++) {
+            int i = jj[3 * m + a];
+            for (int b = 0; b < 3; b++) tmp[cnt[i] + fill[i]++] = jj[3 * m + b];
+        }
+    int *rowptr = (int *)malloc(sizeof(int) * ((size_t)np + 1));
+    rowptr[0] = 0;
+    for (int i = 0; i < np; i++) {
+        int *r = tmp + cnt[i];
+        int n = fill[i];
+        for (int x = 1; x < n; x++) {           /* insertion sort, rows are short */
+            int v = r[x], y = x - 1;
+            while (y >= 0 && r[y] > v) { r[y + 1] = r[y]; y--; }
+            r[y + 1] = v;
+        }
+        int u = 0;
+        for (int x = 0; x < n; x++)
+            if (u == 0 || r[x] != r[u - 1]) r[u++] = r[x];
+        fill[i] = u;
+        rowptr[i + 1] = rowptr[i] + u;
+    }
+    int *cols = (int *)malloc(sizeof(int) * (size_t)(rowptr[np] > 0 ? rowptr[np] : 1));
+    for (int i = 0; i < np; i++) memcpy(cols + rowptr[i], tmp + cnt[i], sizeof(int) * (size_t)fill[i]);
+    free(tmp); free(fill); free(cnt);
+    *rowptr_out = rowptr;
+    *cols_out = cols;
+    return 0;
+}
+
+long orc_csr_nnz(int np, int me, const int *jj)
+{
+    int *rp, *cl;
+    if (build_rows(np, me, jj, &rp, &cl)) return -1;
+    long n = rp[np];
+    free(rp); free(cl);
+    return n;
+}
+
+static inline int find_entry(const int *ia, const int *ja, int i, int j)
+{
+    for (int p = ia[i]; p < ia[i + 1]; p++)
+        if (ja[p] == j) return p;
+    return -1;
+}
+
+int orc_assemble(int np, int me, const int *jj, const double *rr,
+                 int *ia, int *ja, int *diag, double *mass, double *lumped, double *cij)
+{
+    int *rp, *cl;
+    if (build_rows(np, me, jj, &rp, &cl)) return -1;
+    memcpy(ia, rp, sizeof(int) * ((size_t)np + 1));
+    memcpy(ja, cl, sizeof(int) * (size_t)rp[np]);
+    int nnz = rp[np];
+    free(rp); free(cl);
+
+    /* element_2d (gauss_points_2d_p1.f90:347-393): 3-point rule at (1/6,1/6),(2/3,1/6),(1/6,2/3) */
+    const double one = 1, two = 2, three = 3, six = 6;
+    double xx[3] = {one / six, two / three, one / six};
+    double yy[3] = {one / six, one / six, two / three};
+    double ww[3][3];   /* ww[n][l] */
+    double dd[2][3];   /* reference-element gradients, the same at every Gauss point */
+    double pp = one / six;
+    for (int l = 0; l < 3; l++) {
+        ww[0][l] = one - xx[l] - yy[l];
+        ww[1][l] = xx[l];
+        ww[2][l] = yy[l];
+    }
+    dd[0][0] = -one; dd[1][0] = -one;
+    dd[0][1] = one;  dd[1][1] = 0.0;
+    dd[0][2] = 0.0;  dd[1][2] = one;
+
+    memset(mass, 0, sizeof(double) * (size_t)nnz);
+    memset(cij, 0, sizeof(double) * 2 * (size_t)nnz);
+    for (int m = 0; m < me; m++) {
+        const int *nd = jj + 3 * m;
+        double dw[2][3][3];   /* dw[k][n][l] */
+        double rj[3];
+        for (int l = 0; l < 3; l++) {
+            /* dr(k,k1) = SUM(r * dd(k1,:,l))  (gauss_points_2d_p1.f90:112-117) */
+            double dr[2][2];
+            for (int k = 0; k < 2; k++)
+                for (int k1 = 0; k1 < 2; k1++) {
+                    double s = 0.0;
+                    for (int n = 0; n < 3; n++) s = s + rr[2 * nd[n] + k] * dd[k1][n];
+                    dr[k][k1] = s;
+                }
+            double rjac = dr[0][0] * dr[1][1] - dr[0][1] * dr[1][0];
+            for (int n = 0; n < 3; n++) {
+                dw[0][n][l] = (+dd[0][n] * dr[1][1] - dd[1][n] * dr[1][0]) / rjac;
+                dw[1][n][l] = (-(dd[0][n] * dr[0][1]) + dd[1][n] * dr[0][0]) / rjac;
+            }
+            rj[l] = rjac * pp;
+        }
+        /* qs_00_M (fem_sparsekit.f90:31-51): l, then nj, then ni */
+        for (int l = 0; l < 3; l++) {
+            double al = 1.0 * rj[l];
+            for (int nj = 0; nj < 3; nj++) {
+                int j = nd[nj];
+                for (int ni = 0; ni < 3; ni++) {
+                    int i = nd[ni];
+                    double x = ww[nj][l] * al * ww[ni][l];
+                    int p = find_entry(ia, ja, i, j);
+                    mass[p] = mass[p] + x;
+                }
+            }
+        }
+        /* c_ij = int_K grad(phi_j) phi_i  (update.f90:106-124) */
+        for (int ni = 0; ni < 3; ni++) {
+            int i = nd[ni];
+            for (int nj = 0; nj < 3; nj++) {
+                int j = nd[nj];
+                double x[2];
+                for (int d = 0; d < 2; d++) {
+                    double s = 0.0;
+                    for (int l = 0; l < 3; l++) s = s + dw[d][nj][l] * ww[ni][l] * rj[l];
+                    x[d] = s;
+                }
+                int p = find_entry(ia, ja, i, j);
+                cij[2 * p] = cij[2 * p] + x[0];
+                cij[2 * p + 1] = cij[2 * p + 1] + x[1];
+            }
+        }
+    }
+    /* lumped and diag (update.f90:35-49) */
+    for (int i = 0; i < np; i++) {
+        double s = 0.0;
+        for (int p = ia[i]; p < ia[i + 1]; p++) s = s + mass[p];
+        lumped[i] = s;
+        diag[i] = find_entry(ia, ja, i, i);
+    }
+    return nnz;
+}
+
+/* ============================================================================================ */
+/* Context                                                                                        */
+/* ============================================================================================ */
+
+struct orc_ctx {
+    int np, nnz;
+    int *ia, *ja, *diag;
+    double *cij, *lumped, *bath;
+    orc_params p;
+    double dt;
+    /* module-global arrays of the reference (velocity, one_over_h, fix_roundoff, dij, ...) */
+    double *velocity, *one_over_h, *fix, *vv;
+    double *dij, *muij, *dijL, *muijL, *resij, *fctmat, *lij;
+    double *rk, *rk2, *unext2, *alpha, *res, *ent, *scaled;
+    int err;
+};
+
+static void *xmalloc(size_t n) { return malloc(n ? n : 1); }
+
+orc_ctx *orc_create(int np, int nnz, const int *ia, const int *ja, const double *cij,
+                    const double *lumped, const double *bath, const orc_params *prm)
+{
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->np = np;
+    c->nnz = nnz;
+    c->ia = (int *)xmalloc(sizeof(int) * ((size_t)np + 1));
+    c->ja = (int *)xmalloc(sizeof(int) * (size_t)nnz);
+    c->diag = (int *)xmalloc(sizeof(int) * (size_t)np);
+    c->cij = (double *)xmalloc(sizeof(double) * 2 * (size_t)nnz);
+    c->lumped = (double *)xmalloc(sizeof(double) * (size_t)np);
+    c->bath = (double *)xmalloc(sizeof(double) * (size_t)np);
+    memcpy(c->ia, ia, sizeof(int) * ((size_t)np + 1));
+    memcpy(c->ja, ja, sizeof(int) * (size_t)nnz);
+    memcpy(c->cij, cij, sizeof(double) * 2 * (size_t)nnz);
+    memcpy(c->lumped, lumped, sizeof(double) * (size_t)np);
+    memcpy(c->bath, bath, sizeof(double) * (size_t)np);
+    for (int i = 0; i < np; i++) c->diag[i] = find_entry(c->ia, c->ja, i, i);
+    c->p = *prm;
+    size_t N = (size_t)np, E = (size_t)nnz;
+    c->velocity = (double *)xmalloc(sizeof(double) * 2 * N);
+    c->one_over_h = (double *)xmalloc(sizeof(double) * N);
+    c->fix = (double *)xmalloc(sizeof(double) * N);
+    c->vv = (double *)xmalloc(sizeof(double) * 6 * N);
+    c->dij = (double *)xmalloc(sizeof(double) * E);
+    c->muij = (double *)xmalloc(sizeof(double) * E);
+    c->dijL = (double *)xmalloc(sizeof(double) * E);
+    c->muijL = (double *)xmalloc(sizeof(double) * E);
+    c->resij = (double *)xmalloc(sizeof(double) * E);
+    c->fctmat = (double *)xmalloc(sizeof(double) * E);
+    c->lij = (double *)xmalloc(sizeof(double) * E);
+    c->rk = (double *)xmalloc(sizeof(double) * 3 * N);
+    c->rk2 = (double *)xmalloc(sizeof(double) * 3 * N);
+    c->unext2 = (double *)xmalloc(sizeof(double) * 3 * N);
+    c->alpha = (double *)xmalloc(sizeof(double) * N);
+    c->res = (double *)xmalloc(sizeof(double) * N);
+    c->ent = (double *)xmalloc(sizeof(double) * 3 * N);
+    c->scaled = (double *)xmalloc(sizeof(double) * N);
+    memset(c->dij, 0, sizeof(double) * E);
+    memset(c->muij, 0, sizeof(double) * E);
+    return c;
+}
+
+void orc_set_params(orc_ctx *c, const orc_params *prm) { c->p = *prm; }
+
+void orc_destroy(orc_ctx *c)
+{
+    if (!c) return;
+    free(c->ia); free(c->ja); free(c->diag); free(c->cij); free(c->lumped); free(c->bath);
+    free(c->velocity); free(c->one_over_h); free(c->fix); free(c->vv);
+    free(c->dij); free(c->muij); free(c->dijL); free(c->muijL); free(c->resij); free(c->fctmat);
+    free(c->lij); free(c->rk); free(c->rk2); free(c->unext2); free(c->alpha); free(c->res);
+    free(c->ent); free(c->scaled);
+    free(c);
+}
+
+/* ============================================================================================ */
+/* Pointwise physics (SOURCES/condlim.f90)                                                        */
+/* ============================================================================================ */
+
+/* compute_one_over_h (condlim.f90:79-90): 2h/(h^2 + max(h,htiny)^2) */
+static inline double ovh_of(double h, double htiny)
+{
+    double m = fmaxF(h, htiny);
+    return 2 * h / (h * h + m * m);
+}
+
+/* phi (condlim.f90:960-981) */
+static inline double phi_f(double h, double ul, double hl, double ur, double hr, double g, double htiny)
+{
+    double fl, fr;
+    if (h > hl) {
+        double ovh = ovh_of(h, htiny);
+        double ovhl = ovh_of(hl, htiny);
+        fl = (h - hl) * sqrt((g / 2) * (ovh + ovhl));
+    } else {
+        fl = 2 * (sqrt(g * h) - sqrt(g * hl));
+    }
+    if (h > hr) {
+        double ovh = ovh_of(h, htiny);
+        double ovhr = ovh_of(hr, htiny);
+        fr = (h - hr) * sqrt((g / 2) * (ovh + ovhr));
+    } else {
+        fr = 2 * (sqrt(g * h) - sqrt(g * hr));
+    }
+    return fl + fr + ur - ul;
+}
+
+/* compute_lambda_vacc (condlim.f90:879-957) */
+double orc_lambda_vacc(const double *ul, const double *ur, const double *nij,
+                       double g, double htiny, int *err)
+{
+    double hl = ul[0], hr = ur[0];
+    double ovhl = 2 * hl / (hl * hl + fmaxF(hl, htiny) * fmaxF(hl, htiny));
+    double ovhr = 2 * hr / (hr * hr + fmaxF(hr, htiny) * fmaxF(hr, htiny));
+    double vl = (0.0 + ul[1] * nij[0] + ul[2] * nij[1]) * ovhl;
+    double vr = (0.0 + ur[1] * nij[0] + ur[2] * nij[1]) * ovhr;
+    double sql = sqrt(g * fabs(hl));
+    double sqr = sqrt(g * fabs(hr));
+    double x0 = (2.0 * sqrt(2.0) - 1.0);
+    x0 = x0 * x0;
+    double hmin, hmax;
+    if (hl <= hr) { hmin = hl; hmax = hr; }
+    else { hmin = hr; hmax = hl; }
+    double ht, lbdl, lbdr;
+    /* Case 1 */
+    double fh = phi_f(x0 * hmin, vl, hl, vr, hr, g, htiny);
+    if (0.0 <= fh) {
+        double t = fmaxF(vl - vr + 2 * sql + 2 * sqr, 0.0);
+        ht = fminF(x0 * hmin, t * t / (16 * g));
+        goto speeds;
+    }
+    /* Case 2 */
+    fh = phi_f(x0 * hmax, vl, hl, vr, hr, g, htiny);
+    if (0.0 <= fh) {
+        double sqrmin = sqrt(hmin);
+        double sqrmax = sqrt(hmax);
+        double a = 1 / (2.0 * sqrt(2.0));
+        double cc = -hmin * a - sqrmin * sqrmax + sqrmin * (vr - vl) / (2 * sqrt(g));
+        double Delta = hmin - 4 * a * cc;
+        if (Delta < 0.0) {
+            if (err) *err = ORC_ELAMBDA;
+            return 0.0;
+        }
+        double t = (-sqrmin + sqrt(Delta)) / (2 * a);
+        ht = fminF(x0 * hmax, t * t);
+        goto speeds;
+    }
+    /* Case 3 */
+    {
+        double sqrmin = sqrt(hmin);
+        double sqrmax = sqrt(hmax);
+        ht = sqrmin * sqrmax * (1 + sqrt(2 / g) * (vl - vr) / (sqrmin + sqrmax));
+    }
+speeds:
+    lbdl = vl - sql * sqrt((1 + fmaxF((ht - hl) * ovhl / 2, 0.0)) * (1 + fmaxF((ht - hl) * ovhl, 0.0)));
+    lbdr = vr + sqr * sqrt((1 + fmaxF((ht - hr) * ovhr / 2, 0.0)) * (1 + fmaxF((ht - hr) * ovhr, 0.0)));
+    return fmaxF(fabs(lbdl), fabs(lbdr));
+}
+
+void orc_lambda_batch(int n, const double *rec8, double gravity, double htiny, double *out, int *err)
+{
+    for (int i = 0; i < n; i++) {
+        const double *r = rec8 + 8 * (size_t)i;
+        out[i] = orc_lambda_vacc(r, r + 3, r + 6, gravity, htiny, err);
+    }
+}
+
+/* flux (condlim.f90:21-77) for the 3-component system: velocity, one_over_h, vv(k,d,n) */
+static void flux(orc_ctx *c, const double *un)
+{
+    double ht = c->p.htiny;
+    for (int n = 0; n < c->np; n++) {
+        double h = un[3 * n];
+        double o = ovh_of(h, ht);
+        c->one_over_h[n] = o;
+        double v1 = un[3 * n + 1] * o;
+        double v2 = un[3 * n + 2] * o;
+        c->velocity[2 * n] = v1;
+        c->velocity[2 * n + 1] = v2;
+        double *vv = c->vv + 6 * (size_t)n;   /* vv[2*k+d] = vv(k+1,d+1,n) */
+        vv[0] = v1 * h;
+        vv[1] = v2 * h;
+        vv[2] = v1 * un[3 * n + 1];
+        vv[3] = v2 * un[3 * n + 1];
+        vv[4] = v1 * un[3 * n + 2];
+        vv[5] = v2 * un[3 * n + 2];
+    }
+}
+
+/* ============================================================================================ */
+/* Viscosities (SOURCES/update.f90)                                                               */
+/* ============================================================================================ */
+
+/* compute_dij (update.f90:291-326) */
+static int compute_dij(orc_ctx *c, const double *un)
+{
+    int err = 0;
+    const double g = c->p.gravity, ht = c->p.htiny;
+    for (int i = 0; i < c->np; i++) {
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            if (i != j) {
+                double nij[2] = {c->cij[2 * p], c->cij[2 * p + 1]};
+                double norm = sqrt(0.0 + nij[0] * nij[0] + nij[1] * nij[1]);
+                nij[0] = nij[0] / norm;
+                nij[1] = nij[1] / norm;
+                double lambda = orc_lambda_vacc(un + 3 * i, un + 3 * j, nij, g, ht, &err);
+                c->dij[p] = norm * lambda;
+            } else {
+                c->dij[p] = 0.0;
+            }
+        }
+    }
+    for (int i = 0; i < c->np; i++) {
+        double s = 0.0;
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) s = s + c->dij[p];
+        c->dij[c->diag[i]] = -s;
+    }
+    return err;
+}
+
+/* compute_muij (update.f90:328-364) */
+static void compute_muij(orc_ctx *c, const double *un)
+{
+    const double ht = c->p.htiny;
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i];
+        double oi = 2 * hi / (hi * hi + fmaxF(hi, ht) * fmaxF(hi, ht));
+        double ul[2] = {un[3 * i + 1] * oi, un[3 * i + 2] * oi};
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            if (i != j) {
+                double n1 = c->cij[2 * p], n2 = c->cij[2 * p + 1];
+                double hj = un[3 * j];
+                double oj = 2 * hj / (hj * hj + fmaxF(hj, ht) * fmaxF(hj, ht));
+                double ur[2] = {un[3 * j + 1] * oj, un[3 * j + 2] * oj};
+                double sl = 0.0 + n1 * ul[0] + n2 * ul[1];
+                double sr = 0.0 + n1 * ur[0] + n2 * ur[1];
+                c->muij[p] = fmaxF(fmaxF(-sl, 0.0), fmaxF(sr, 0.0));
+            } else {
+                c->muij[p] = 0.0;
+            }
+        }
+    }
+    for (int i = 0; i < c->np; i++) {
+        double s = 0.0;
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) s = s + c->muij[p];
+        c->muij[c->diag[i]] = -s;
+    }
+}
+
+/* alpha_limit (update.f90:750-799); alpha**8 with an integer exponent is three squarings */
+static void alpha_limit(orc_ctx *c, const double *un)
+{
+    const double ht = c->p.htiny;
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i];
+        if (hi <= ht) { c->alpha[i] = 1.0; continue; }
+        double num = 0.0, denom = 0.0, a;
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            double hj = un[3 * c->ja[p]];
+            num = num + hj - hi;
+            denom = denom + fabs(hj - hi);
+        }
+        if (denom <= fabs(num)) a = 1.0;
+        else a = fabs(num) / (denom);
+        a = a * a; a = a * a; a = a * a;
+        c->alpha[i] = a;
+    }
+    for (int i = 0; i < c->np; i++)
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            if (i != j) {
+                double s = fmaxF(c->alpha[i], c->alpha[j]);
+                c->dij[p] = c->dij[p] * s;
+                c->muij[p] = c->muij[p] * s;
+            } else {
+                c->dij[p] = 0.0;
+            }
+        }
+    for (int i = 0; i < c->np; i++) {
+        double s = 0.0, t = 0.0;
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) s = s + c->dij[p];
+        c->dij[c->diag[i]] = -s;
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) t = t + c->muij[p];
+        c->muij[c->diag[i]] = -t;
+    }
+}
+
+/* ============================================================================================ */
+/* Right-hand sides                                                                               */
+/* ============================================================================================ */
+
+/* friction / my_friction / coriolis dispatch on type_test (update.f90:412-421,614-623,672-681) */
+static void sources(orc_ctx *c, const double *un, double *rk)
+{
+    int tt = c->p.type_test;
+    if (tt == 9 || tt == 10) {
+        /* friction (update.f90:425-452) */
+        const double g = c->p.gravity;
+        double fric = g * (c->p.mannings * c->p.mannings);
+        const double gamma = 4.0 / 3.0;
+        for (int i = 0; i < c->np; i++) {
+            double hs = pow(fmaxF(un[3 * i], c->p.htiny), gamma);
+            double v1 = c->velocity[2 * i], v2 = c->velocity[2 * i + 1];
+            double vel = fric * sqrt(v1 * v1 + v2 * v2);
+            double hloc = 1.0 * vel * c->dt;
+            for (int k = 1; k < 3; k++)
+                rk[3 * i + k] = rk[3 * i + k] - c->lumped[i] * un[3 * i + k] * vel * 2 / (hs + fmaxF(hs, hloc));
+        }
+    } else if (tt == 11) {
+        /* my_friction (update.f90:454-469) */
+        const double kappa = 0.2;
+        for (int i = 0; i < c->np; i++)
+            for (int k = 1; k < 3; k++)
+                rk[3 * i + k] = rk[3 * i + k] - c->lumped[i] * kappa * un[3 * i + k];
+    } else if (tt == 12) {
+        /* coriolis (update.f90:471-486) with syst_size = 3 */
+        const double f = 2.0;
+        for (int i = 0; i < c->np; i++) {
+            rk[3 * i + 1] = rk[3 * i + 1] + c->lumped[i] * f * un[3 * i + 2];
+            rk[3 * i + 2] = rk[3 * i + 2] - c->lumped[i] * f * un[3 * i + 1];
+        }
+    }
+}
+
+/* smb_2_roundoff (update.f90:554-624) */
+static void smb_2_roundoff(orc_ctx *c, const double *un, double *rk)
+{
+    const double g = c->p.gravity, ht = c->p.htiny;
+    const double *z = c->bath;
+    flux(c, un);
+    memset(rk, 0, sizeof(double) * 3 * (size_t)c->np);
+    for (int i = 0; i < c->np; i++) c->fix[i] = 0.0;
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i], zi = z[i];
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            double hj = un[3 * j], zj = z[j];
+            double zmax = fmaxF(zi, zj);
+            double Hij = fmaxF(0.0, hi + zi - zmax);
+            double Hji = fmaxF(0.0, hj + zj - zmax);
+            double ratij = (hi <= ht) ? 0.0 : Hij / hi;
+            double ratji = (hj <= ht) ? 0.0 : Hji / hj;
+            double c1 = c->cij[2 * p], c2 = c->cij[2 * p + 1];
+            double dd = c->dij[p], mu = c->muij[p];
+            /* mass */
+            double xx = 0.0;
+            xx = xx - c1 * (c->velocity[2 * j]);
+            xx = xx - c2 * (c->velocity[2 * j + 1]);
+            if (i == j) {
+                c->fix[i] = c->fix[i] + xx;
+            } else {
+                c->fix[i] = c->fix[i] - mu + (dd - mu) * (-ratij);
+                rk[3 * i] = rk[3 * i] + (xx + mu) * hj + fmaxF((dd - mu) * Hji, 0.0);
+            }
+            /* momentum */
+            for (int k = 1; k < 3; k++) {
+                const double *vvj = c->vv + 6 * (size_t)j + 2 * k;
+                xx = 0.0;
+                xx = xx - c1 * (vvj[0]);
+                xx = xx - c2 * (vvj[1]);
+                double ukj = un[3 * j + k], uki = un[3 * i + k];
+                rk[3 * i + k] = rk[3 * i + k] + xx + dd * (ukj * ratji - uki * ratij)
+                              + mu * (ukj * (1.0 - ratji) - uki * (1.0 - ratij));
+            }
+            /* topography */
+            double topo = g * hi * (hj + zj - hi - zi);
+            rk[3 * i + 1] = rk[3 * i + 1] - topo * c1;
+            rk[3 * i + 2] = rk[3 * i + 2] - topo * c2;
+        }
+    }
+    sources(c, un, rk);
+}
+
+/* smb_2 (update.f90:626-682) */
+static void smb_2(orc_ctx *c, const double *un, double *rk)
+{
+    const double g = c->p.gravity;
+    const double *z = c->bath;
+    flux(c, un);
+    memset(rk, 0, sizeof(double) * 3 * (size_t)c->np);
+    for (int i = 0; i < c->np; i++) c->fix[i] = 0.0;
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i], zi = z[i];
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            double hj = un[3 * j], zj = z[j];
+            double zmax = fmaxF(zi, zj);
+            double Hij = fmaxF(0.0, hi + zi - zmax);
+            double Hji = fmaxF(0.0, hj + zj - zmax);
+            double ratij = (hi <= 0) ? 0.0 : Hij / hi;
+            double ratji = (hj <= 0) ? 0.0 : Hji / hj;
+            double c1 = c->cij[2 * p], c2 = c->cij[2 * p + 1];
+            double dd = c->dij[p], mu = c->muij[p];
+            for (int k = 0; k < 3; k++) {
+                const double *vvj = c->vv + 6 * (size_t)j + 2 * k;
+                double xx = 0.0;
+                xx = xx - c1 * (vvj[0]);
+                xx = xx - c2 * (vvj[1]);
+                double ukj = un[3 * j + k], uki = un[3 * i + k];
+                rk[3 * i + k] = rk[3 * i + k] + xx + dd * (ukj * ratji - uki * ratij)
+                              + mu * (ukj * (1.0 - ratji) - uki * (1.0 - ratij));
+            }
+            double topo = g * hi * (hj + zj - hi - zi);
+            rk[3 * i + 1] = rk[3 * i + 1] - topo * c1;
+            rk[3 * i + 2] = rk[3 * i + 2] - topo * c2;
+        }
+    }
+    sources(c, un, rk);
+}
+
+/* smb_1 (update.f90:366-423) */
+static void smb_1(orc_ctx *c, const double *un, double *rk)
+{
+    const double g = c->p.gravity, ht = c->p.htiny;
+    const double *z = c->bath;
+    flux(c, un);
+    memset(rk, 0, sizeof(double) * 3 * (size_t)c->np);
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i], zi = z[i];
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            double hj = un[3 * j], zj = z[j];
+            double zmax = fmaxF(zi, zj);
+            double Hij = fmaxF(0.0, hi + zi - zmax);
+            double Hji = fmaxF(0.0, hj + zj - zmax);
+            double ratij = (hi <= ht) ? 0.0 : Hij / hi;
+            double ratji = (hj <= ht) ? 0.0 : Hji / hj;
+            double c1 = c->cij[2 * p], c2 = c->cij[2 * p + 1];
+            double dd = c->dij[p];
+            for (int k = 0; k < 3; k++) {
+                const double *vvj = c->vv + 6 * (size_t)j + 2 * k;
+                const double *vvi = c->vv + 6 * (size_t)i + 2 * k;
+                double xx = 0.0;
+                xx = xx - c1 * (vvj[0] * ratji + vvi[0] * ratij);
+                xx = xx - c2 * (vvj[1] * ratji + vvi[1] * ratij);
+                rk[3 * i + k] = rk[3 * i + k] + xx + dd * (un[3 * j + k] * ratji - un[3 * i + k] * ratij);
+            }
+            double t = 0.5 * g * (Hji * Hji - Hij * Hij);
+            rk[3 * i + 1] = rk[3 * i + 1] - t * c1;
+            rk[3 * i + 2] = rk[3 * i + 2] - t * c2;
+        }
+    }
+    sources(c, un, rk);
+}
+
+static void divide_by_lumped(orc_ctx *c, double *rk)
+{
+    for (int i = 0; i < c->np; i++)
+        for (int k = 0; k < 3; k++) rk[3 * i + k] = rk[3 * i + k] / c->lumped[i];
+}
+
+/* check_hmin (update.f90:810-822) */
+static int check_hmin(orc_ctx *c, const double *u)
+{
+    int tt = c->p.type_test;
+    if (tt < 1 || tt > 13) return 0;
+    for (int i = 0; i < c->np; i++)
+        if (u[3 * i] < 0.0) return ORC_ENEG_H;
+    return 0;
+}
+
+/* entropy_residual (update.f90:825-878) with maxmin (924-935) */
+static void entropy_residual(orc_ctx *c, const double *un)
+{
+    const double g = c->p.gravity, ht = c->p.htiny, dt = c->dt;
+    double *rk = c->rk2;
+    double *un2 = c->unext2;
+    smb_2(c, un, rk);
+    divide_by_lumped(c, rk);
+    for (size_t q = 0; q < 3 * (size_t)c->np; q++) un2[q] = un[q] + dt * rk[q];
+    double *Ent = c->ent;
+    for (int n = 0; n < c->np; n++) {
+        double h = un[3 * n], hn = un2[3 * n];
+        double on = ovh_of(hn, ht);
+        double vn1 = un2[3 * n + 1] * on, vn2 = un2[3 * n + 2] * on;
+        double v1 = c->velocity[2 * n], v2 = c->velocity[2 * n + 1];
+        double E = g * (h * h);
+        double En = g * (hn * hn);
+        E = E + v1 * un[3 * n + 1];
+        En = En + vn1 * un2[3 * n + 1];
+        E = E + v2 * un[3 * n + 2];
+        En = En + vn2 * un2[3 * n + 2];
+        E = 0.5 * E;
+        En = 0.5 * En;
+        double Ep0 = g * h;
+        Ep0 = Ep0 - 0.5 * (v1 * v1);
+        Ep0 = Ep0 - 0.5 * (v2 * v2);
+        double m = c->lumped[n];
+        double r = m * (En - E) / dt;
+        r = r - m * rk[3 * n] * Ep0;
+        r = r - m * rk[3 * n + 1] * v1;
+        r = r - m * rk[3 * n + 2] * v2;
+        Ent[n] = E;
+        c->res[n] = r;
+    }
+    const double cst = 1.0e-10 * g * (c->p.max_water_h * c->p.max_water_h);
+    for (int i = 0; i < c->np; i++) {
+        int p0 = c->ia[i];
+        double mx = Ent[c->ja[p0]], mn = Ent[c->ja[p0]];
+        for (int p = p0 + 1; p < c->ia[i + 1]; p++) {
+            double e = Ent[c->ja[p]];
+            if (e > mx) mx = e;
+            if (e < mn) mn = e;
+        }
+        double rescale = fabs(mx - mn) / 2 + cst;
+        c->res[i] = fabs(c->res[i]) / rescale;
+    }
+    for (int i = 0; i < c->np; i++)
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++)
+            c->resij[p] = fmaxF(c->res[i], c->res[c->ja[p]]);
+}
+
+/* compute_fct_matrix (update.f90:721-748) */
+static void compute_fct_matrix(orc_ctx *c, const double *un)
+{
+    const double *z = c->bath;
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i], zi = z[i];
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            double hj = un[3 * j], zj = z[j];
+            double zmax = fmaxF(zi, zj);
+            double Hij = fmaxF(0.0, hi + zi - zmax);
+            double Hji = fmaxF(0.0, hj + zj - zmax);
+            double ratij = (hi <= 0.0) ? 0.0 : Hij / hi;
+            double ratji = (hj <= 0.0) ? 0.0 : Hji / hj;
+            c->fctmat[p] = (c->dij[p] - c->dijL[p]) * (hj * ratji - hi * ratij)
+                         + (c->muij[p] - c->muijL[p]) * (hj * (1.0 - ratji) - hi * (1.0 - ratij));
+        }
+    }
+}
+
+/* FCT_positivity (fct.f90:5-42) */
+void orc_fct_positivity(int np, const int *ia, const int *ja, const double *f,
+                        const double *unext, double htiny, double *lij)
+{
+    const double smallminus = -1.0e-15;
+    double *R = (double *)xmalloc(sizeof(double) * (size_t)np);
+    for (int i = 0; i < np; i++) {
+        double Q = 0.0 - unext[i];
+        double P = smallminus;
+        for (int p = ia[i]; p < ia[i + 1]; p++) {
+            double fij = f[p];
+            if (fij < 0.0) P = P + fij;
+        }
+        R[i] = fminF(Q / P, 1.0);
+        if (unext[i] <= htiny) R[i] = 0.0;
+    }
+    for (int i = 0; i < np; i++)
+        for (int p = ia[i]; p < ia[i + 1]; p++)
+            lij[p] = (f[p] >= 0.0) ? R[ja[p]] : R[i];
+    free(R);
+}
+
+/* apply_viscosity (update.f90:684-718) */
+static void apply_viscosity(orc_ctx *c, const double *un, double *rk)
+{
+    const double *z = c->bath;
+    for (int i = 0; i < c->np; i++) {
+        double hi = un[3 * i], zi = z[i];
+        double xx[3] = {0.0, 0.0, 0.0};
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            double hj = un[3 * j], zj = z[j];
+            double zmax = fmaxF(zi, zj);
+            double Hij = fmaxF(0.0, hi + zi - zmax);
+            double Hji = fmaxF(0.0, hj + zj - zmax);
+            double ratij = (hi <= 0.0) ? 0.0 : Hij / hi;
+            double ratji = (hj <= 0.0) ? 0.0 : Hji / hj;
+            for (int k = 0; k < 3; k++) {
+                double ukj = un[3 * j + k], uki = un[3 * i + k];
+                xx[k] = xx[k] + c->dij[p] * (ukj * ratji - uki * ratij)
+                      + c->muij[p] * (ukj * (1.0 - ratji) - uki * (1.0 - ratij));
+            }
+        }
+        rk[3 * i] = xx[0];
+        rk[3 * i + 1] = xx[1];
+        rk[3 * i + 2] = xx[2];
+    }
+}
+
+/* ============================================================================================ */
+/* euler (update.f90:134-289)                                                                     */
+/* ============================================================================================ */
+
+int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *dij_out)
+{
+    const int N = c->np;
+    const double *L = c->lumped;
+    double *rk = c->rk;
+    int vt = c->p.viscosity_type, ty = c->p.viscous_type;
+    int err = 0;
+    c->dt = dt;
+
+    if (vt == ORC_GALERKIN) {
+        memset(c->dij, 0, sizeof(double) * (size_t)c->nnz);
+        memset(c->muij, 0, sizeof(double) * (size_t)c->nnz);
+        smb_2(c, un, rk);
+        divide_by_lumped(c, rk);
+        for (int i = 0; i < N; i++) {
+            unext[3 * i] = fmaxF(un[3 * i] + dt * rk[3 * i], 0.0);
+            unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
+            unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+        }
+        goto done;
+    }
+
+    err = compute_dij(c, un);
+    if (err) return err;
+    compute_muij(c, un);
+    for (int p = 0; p < c->nnz; p++) c->dij[p] = fmaxF(c->dij[p], c->muij[p]);
+    if (c->p.alpha_limit) alpha_limit(c, un);
+
+    if (ty == 1) smb_1(c, un, rk);
+    else if (ty == 2) {
+        if (c->p.roundoff_fix) smb_2_roundoff(c, un, rk);
+        else smb_2(c, un, rk);
+    } else return ORC_EBADOPT;
+    divide_by_lumped(c, rk);
+
+    if (ty == 1) {
+        for (size_t q = 0; q < 3 * (size_t)N; q++) unext[q] = un[q] + dt * rk[q];
+    } else if (c->p.roundoff_fix) {
+        for (int i = 0; i < N; i++) {
+            unext[3 * i] = un[3 * i] * (1 + dt * c->fix[i] / L[i]) + dt * rk[3 * i];
+            unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
+            unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+        }
+    } else {
+        for (int i = 0; i < N; i++) {
+            unext[3 * i] = fmaxF(un[3 * i] + dt * rk[3 * i], 0.0);
+            unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
+            unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+        }
+    }
+    if (vt == ORC_VISCOUS) goto done;
+
+    if (vt == ORC_FCT) {
+        memcpy(c->dijL, c->dij, sizeof(double) * (size_t)c->nnz);
+        memcpy(c->muijL, c->muij, sizeof(double) * (size_t)c->nnz);
+    }
+    entropy_residual(c, un);
+    for (int p = 0; p < c->nnz; p++) {
+        c->dij[p] = fminF(c->dij[p], 1.0 * c->resij[p]);
+        c->muij[p] = fminF(c->muij[p], 1.0 * c->resij[p]);
+    }
+    if (vt == ORC_ENTROPY_VISC) {
+        if (ty != 2) return ORC_EBADOPT;
+        if (c->p.roundoff_fix) smb_2_roundoff(c, un, rk);
+        else smb_2(c, un, rk);
+        divide_by_lumped(c, rk);
+        if (c->p.roundoff_fix) {
+            for (int i = 0; i < N; i++) {
+                unext[3 * i] = un[3 * i] * (1 + dt * c->fix[i] / L[i]) + dt * rk[3 * i];
+                unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
+                unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+            }
+        } else {
+            for (int i = 0; i < N; i++) {
+                unext[3 * i] = fmaxF(un[3 * i] + dt * rk[3 * i], 0.0);
+                unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
+                unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+            }
+        }
+    } else if (vt == ORC_FCT) {
+        err = check_hmin(c, unext);
+        if (err) return err;
+        if (ty != 2) return ORC_EBADOPT;
+        compute_fct_matrix(c, un);
+        for (int i = 0; i < N; i++) c->scaled[i] = (L[i] / dt) * unext[3 * i];
+        orc_fct_positivity(N, c->ia, c->ja, c->fctmat, c->scaled, c->p.htiny, c->lij);
+        for (int p = 0; p < c->nnz; p++) {
+            c->dij[p] = (c->dij[p] - c->dijL[p]) * c->lij[p];
+            c->muij[p] = (c->muij[p] - c->muijL[p]) * c->lij[p];
+        }
+        apply_viscosity(c, un, rk);
+        divide_by_lumped(c, rk);
+        for (size_t q = 0; q < 3 * (size_t)N; q++) unext[q] = unext[q] + dt * rk[q];
+    } else {
+        return ORC_EBADOPT;
+    }
+    err = check_hmin(c, unext);
+done:
+    if (dij_out) memcpy(dij_out, c->dij, sizeof(double) * (size_t)c->nnz);
+    return err;
+}
+
+int orc_compute_dij(orc_ctx *c, const double *un, double *dij_out)
+{
+    int err = compute_dij(c, un);
+    if (dij_out) memcpy(dij_out, c->dij, sizeof(double) * (size_t)c->nnz);
+    return err;
+}
+
+/* COMPUTE_DT (main.f90:210-218): dt = CFL*1/MAXVAL(ABS(dij%aa(diag))/lumped) */
+double orc_compute_dt(orc_ctx *c, const double *un, int *err)
+{
+    int e = compute_dij(c, un);
+    if (err) *err = e;
+    double mx = 0.0;
+    int first = 1;
+    for (int i = 0; i < c->np; i++) {
+        double v = fabs(c->dij[c->diag[i]]) / c->lumped[i];
+        if (first || v > mx) { mx = v; first = 0; }
+    }
+    return c->p.cfl * 1 / mx;
+}
+
+/* Dirichlet overwrite of bdy (main.f90:220-232) for the problems driven by this checker */
+static void bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double period, double level,
+                int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
+{
+    if (bc_kind == 2) {
+        const double pi = acos(-1.0);
+        for (int q = 0; q < nh; q++) {
+            int n = nodes_h[q];
+            u[3 * n] = fmaxF(level - c->bath[n] + amp * sin(2.0 * pi * t / period), 0.0);
+        }
+    }
+    if (bc_kind >= 1) {
+        for (int q = 0; q < nux; q++) u[3 * nodes_ux[q] + 1] = 0.0;
+        for (int q = 0; q < nuy; q++) u[3 * nodes_uy[q] + 2] = 0.0;
+    }
+}
+
+/* SSP-RK3 time loop body of main.f90:111-135, repeated nsteps times */
+int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
+            int bc_kind, double amp, double period, double level,
+            int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
+{
+    if (bc_kind == 0 && (nh || nux || nuy)) return ORC_EBADOPT;
+    size_t n3 = 3 * (size_t)c->np;
+    double *uo = (double *)xmalloc(sizeof(double) * n3);
+    double *ui = (double *)xmalloc(sizeof(double) * n3);
+    double t = *time_io;
+    int err = 0;
+    for (int it = 0; it < nsteps && !err; it++) {
+        double dt = orc_compute_dt(c, un, &err);
+        if (err) break;
+        if (dts) dts[it] = dt;
+        double to = t;
+        memcpy(uo, un, sizeof(double) * n3);
+        err = orc_euler(c, uo, un, dt, NULL);
+        if (err) break;
+        bdy(c, un, t + dt, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        t = to + dt;
+        err = orc_euler(c, un, ui, dt, NULL);
+        if (err) break;
+        for (size_t q = 0; q < n3; q++) un[q] = (3 * uo[q] + ui[q]) / 4;
+        bdy(c, un, t + dt / 2, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        t = to + dt / 2;
+        err = orc_euler(c, un, ui, dt, NULL);
+        if (err) break;
+        for (size_t q = 0; q < n3; q++) un[q] = (uo[q] + 2 * ui[q]) / 3;
+        bdy(c, un, t + dt, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        t = to + dt;
+    }
+    *time_io = t;
+    free(uo);
+    free(ui);
+    return err;
+}

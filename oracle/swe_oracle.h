@@ -1,0 +1,69 @@
+/* swe_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the reference explicit shallow-water step (SOURCES/update.f90, fct.f90,
+ * condlim.f90, main.f90 of t-bltg/shallow-water-eric) and of its P1 set-up (SFEMaNS gauss points,
+ * st_csr, qs_00_M).  It is the checker for the HIP library and the CPU baseline of the benchmark;
+ * the product never links, loads or calls it.
+ *
+ * Conventions: 0-based CSR (row i holds entries ia[i]..ia[i+1]-1, columns ascending, diagonal
+ * included, as st_csr builds them); state u is the reference's un(3,np) in Fortran order, i.e.
+ * u[3*i+k] for node i and component k (h, qx, qy); c_ij is interleaved c[2*p+d].
+ * Every floating-point expression keeps the reference's left-to-right evaluation order and is
+ * compiled without FMA contraction.
+ */
+#ifndef SWE_ORACLE_H
+#define SWE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_GALERKIN = 0, ORC_VISCOUS = 1, ORC_ENTROPY_VISC = 2, ORC_FCT = 3 };
+enum { ORC_OK = 0, ORC_ENEG_H = 1, ORC_ELAMBDA = 2, ORC_EBADOPT = 3 };
+
+typedef struct orc_params {
+    double gravity, htiny, cfl, max_water_h, mannings;
+    int type_test;        /* reference test number: selects the source term (9/10 Manning,
+                             11 linear friction, 12 Coriolis; anything else none) */
+    int viscosity_type;   /* ORC_GALERKIN .. ORC_FCT */
+    int viscous_type;     /* 1 = 'type1', 2 = 'type2' */
+    int alpha_limit;      /* inputs%if_alpha_limit */
+    int roundoff_fix;     /* if_roundoff_fix (SOURCES/update.f90:15 hard-codes .TRUE.) */
+} orc_params;
+
+typedef struct orc_ctx orc_ctx;
+
+/* set-up */
+long orc_csr_nnz(int np, int me, const int *jj);
+int orc_assemble(int np, int me, const int *jj, const double *rr,
+                 int *ia, int *ja, int *diag, double *mass, double *lumped, double *cij);
+
+/* solver context: copies the arrays it is given */
+orc_ctx *orc_create(int np, int nnz, const int *ia, const int *ja, const double *cij,
+                    const double *lumped, const double *bath, const orc_params *prm);
+void orc_destroy(orc_ctx *c);
+void orc_set_params(orc_ctx *c, const orc_params *prm);
+
+/* pointwise pieces */
+double orc_lambda_vacc(const double *ul, const double *ur, const double *nij,
+                       double gravity, double htiny, int *err);
+void orc_lambda_batch(int n, const double *rec8, double gravity, double htiny, double *out, int *err);
+void orc_fct_positivity(int np, const int *ia, const int *ja, const double *f,
+                        const double *unext_scaled, double htiny, double *lij);
+
+/* step pieces on a context */
+int orc_compute_dij(orc_ctx *c, const double *un, double *dij_out);
+double orc_compute_dt(orc_ctx *c, const double *un, int *err);
+int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *dij_out);
+
+/* Fixed-step SSP-RK3 loop of SOURCES/main.f90:105-136.  bc_kind 0: no Dirichlet lists allowed;
+ * 1: walls (qx = 0 on nodes_ux, qy = 0 on nodes_uy); 2: walls plus the inflow
+ * h = max(level - z + amp*sin(2*pi*t/period), 0) on nodes_h.  dts receives dt per step. */
+int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
+            int bc_kind, double amp, double period, double level,
+            int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

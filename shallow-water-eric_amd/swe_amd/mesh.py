@@ -1,0 +1,281 @@
+"""P1 triangle meshes: the reference's ``.FEM`` file format, its loader's node renumbering, its
+Dirichlet node lists, and the mesh generators for the benchmark problems.
+
+Reference behaviour restated here (integer work, exact):
+
+* ``.FEM`` layout, read by ``load_mesh_free_format`` (SFEMaNS/FEMSUB_OBJET/prep_mesh.f90:759-1033)
+  with ``mesh_formatted=.FALSE.`` (SOURCES/mesh.f90:16-17): Fortran unformatted sequential
+  records with 4-byte markers,
+  ``np,nw,me,nws,mes`` / ``jj(nw,me),neigh(3,me),i_d(me)`` / ``jjs(nws,mes),neighs(mes),sides(mes)``
+  / ``rr(2,np)``.
+* Node renumbering in first-touch element order (prep_mesh.f90:884-916).
+* Dirichlet node lists: sorted unique nodes of the boundary edges whose side tag is listed
+  (SFEMaNS/FEMSUB/dir_nodes.f90:243-357, called from SOURCES/mesh.f90:25-34).
+
+Side tags used by the generators: 1 = south (y = y0), 2 = east (x = x1), 3 = north (y = y1),
+4 = west (x = x0).  All node and element indices in this module are 0-based; the ``.FEM`` file
+and the C-ABI use the reference's 1-based numbering.
+"""
+from __future__ import annotations
+
+import dataclasses
+import struct
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Mesh:
+    """A P1 triangulation.  ``jj`` (me,3) and ``jjs`` (mes,2) are 0-based node indices."""
+
+    rr: np.ndarray        # (np, 2) float64 coordinates
+    jj: np.ndarray        # (me, 3) int32, counter-clockwise
+    neigh: np.ndarray     # (me, 3) int32, element across the edge opposite vertex n, -1 on the boundary
+    i_d: np.ndarray       # (me,) int32 sub-domain tag
+    jjs: np.ndarray       # (mes, 2) int32 boundary edges
+    neighs: np.ndarray    # (mes,) int32 element owning each boundary edge
+    sides: np.ndarray     # (mes,) int32 side tag of each boundary edge
+
+    @property
+    def np(self) -> int:
+        return int(self.rr.shape[0])
+
+    @property
+    def me(self) -> int:
+        return int(self.jj.shape[0])
+
+
+# ----------------------------------------------------------------------------------------------
+# .FEM read / write
+# ----------------------------------------------------------------------------------------------
+
+def _write_record(f, *arrays):
+    payload = b"".join(np.ascontiguousarray(a).tobytes() for a in arrays)
+    n = len(payload)
+    if n >= 2**31:
+        raise ValueError("record too large for a 4-byte marker (%d bytes)" % n)
+    f.write(struct.pack("<i", n))
+    f.write(payload)
+    f.write(struct.pack("<i", n))
+
+
+def _read_record(f) -> bytes:
+    head = f.read(4)
+    if len(head) != 4:
+        raise EOFError("truncated .FEM file")
+    (n,) = struct.unpack("<i", head)
+    if n < 0:
+        raise ValueError("sub-records (markers < 0) are not supported")
+    payload = f.read(n)
+    (tail,) = struct.unpack("<i", f.read(4))
+    if tail != n or len(payload) != n:
+        raise ValueError("corrupt record marker in .FEM file")
+    return payload
+
+
+def write_fem(path: str, mesh: Mesh) -> None:
+    """Write ``mesh`` as an unformatted P1 ``.FEM`` file (1-based indices, Fortran column order)."""
+    me, mes = mesh.me, int(mesh.jjs.shape[0])
+    with open(path, "wb") as f:
+        _write_record(f, np.array([mesh.np, 3, me, 2, mes], dtype="<i4"))
+        _write_record(f,
+                      (mesh.jj.astype("<i4") + 1),                 # jj(3,me): element-major rows
+                      (mesh.neigh.astype("<i4") + 1),              # -1 -> 0 on the boundary
+                      mesh.i_d.astype("<i4"))
+        _write_record(f,
+                      (mesh.jjs.astype("<i4") + 1),
+                      (mesh.neighs.astype("<i4") + 1),
+                      mesh.sides.astype("<i4"))
+        _write_record(f, mesh.rr.astype("<f8"))
+
+
+def read_fem(path: str) -> Mesh:
+    """Read an unformatted P1 ``.FEM`` file (the layout the reference loader reads)."""
+    with open(path, "rb") as f:
+        np_, nw, me, nws, mes = np.frombuffer(_read_record(f), dtype="<i4")
+        if (nw, nws) != (3, 2):
+            raise ValueError("only P1 triangle meshes (nw=3, nws=2) are supported, got nw=%d nws=%d" % (nw, nws))
+        r2 = np.frombuffer(_read_record(f), dtype="<i4")
+        jj = r2[: 3 * me].reshape(me, 3) - 1
+        neigh = r2[3 * me: 6 * me].reshape(me, 3) - 1
+        i_d = r2[6 * me: 7 * me].copy()
+        r3 = np.frombuffer(_read_record(f), dtype="<i4")
+        jjs = r3[: 2 * mes].reshape(mes, 2) - 1
+        neighs = r3[2 * mes: 3 * mes] - 1
+        sides = r3[3 * mes: 4 * mes].copy()
+        rr = np.frombuffer(_read_record(f), dtype="<f8").reshape(np_, 2).copy()
+    return Mesh(rr=rr, jj=jj.astype(np.int32), neigh=neigh.astype(np.int32), i_d=i_d.astype(np.int32),
+                jjs=jjs.astype(np.int32), neighs=neighs.astype(np.int32), sides=sides.astype(np.int32))
+
+
+# ----------------------------------------------------------------------------------------------
+# The reference loader's renumbering and Dirichlet lists
+# ----------------------------------------------------------------------------------------------
+
+def renumber_first_touch(mesh: Mesh, list_dom=(1,)) -> Mesh:
+    """Keep the elements whose tag is in ``list_dom`` and renumber nodes in first-touch order.
+
+    Restates prep_mesh.f90:884-916 (elements kept in file order, a node gets the next number the
+    first time an element visits it) and 933-966 (a boundary edge is kept when its element, or
+    the element across it, is kept; it then refers to the kept element).
+    """
+    keep = np.isin(mesh.i_d, np.asarray(list_dom))
+    old_el = np.nonzero(keep)[0]
+    new_el = np.full(mesh.me, -1, dtype=np.int64)
+    new_el[old_el] = np.arange(old_el.size)
+    flat = mesh.jj[old_el].reshape(-1)
+    _, first = np.unique(flat, return_index=True)
+    order = flat[np.sort(first)]                     # ancien_nd: old index of each new node
+    new_nd = np.full(mesh.np, -1, dtype=np.int64)
+    new_nd[order] = np.arange(order.size)
+    jj = new_nd[mesh.jj[old_el]].astype(np.int32)
+    nb = mesh.neigh[old_el]
+    neigh = np.where(nb >= 0, new_el[np.maximum(nb, 0)], -1).astype(np.int32)
+    # boundary edges (prep_mesh.f90:933-955)
+    e1 = mesh.neighs
+    keep_s = keep[e1]
+    other = np.full(e1.shape, -1, dtype=np.int64)
+    if not keep_s.all():
+        for s in np.nonzero(~keep_s)[0]:
+            el = e1[s]
+            verts = mesh.jj[el]
+            opp = [n for n in range(3) if verts[n] not in mesh.jjs[s]]
+            n2 = mesh.neigh[el, opp[0]] if opp else -1
+            if n2 >= 0 and keep[n2]:
+                keep_s[s] = True
+                other[s] = n2
+    owner = np.where(other >= 0, other, e1)
+    sel = np.nonzero(keep_s)[0]
+    return Mesh(rr=mesh.rr[order].copy(), jj=jj, neigh=neigh, i_d=mesh.i_d[old_el].copy(),
+                jjs=new_nd[mesh.jjs[sel]].astype(np.int32),
+                neighs=new_el[owner[sel]].astype(np.int32), sides=mesh.sides[sel].copy())
+
+
+def dirichlet_nodes(mesh: Mesh, side_list) -> np.ndarray:
+    """Sorted unique 0-based nodes on the boundary edges whose side tag is in ``side_list``
+    (dir_nodes.f90:243-357)."""
+    side_list = np.asarray(list(side_list), dtype=np.int64)
+    if side_list.size == 0:
+        return np.zeros(0, dtype=np.int32)
+    sel = np.isin(mesh.sides, side_list)
+    return np.unique(mesh.jjs[sel].reshape(-1)).astype(np.int32)
+
+
+# ----------------------------------------------------------------------------------------------
+# Generators
+# ----------------------------------------------------------------------------------------------
+
+def _grid_topology(nx: int, ny: int, diag_ne: np.ndarray | None = None):
+    """Triangles and boundary edges of an nx-by-ny lattice (node (i,j) -> j*nx+i).
+
+    Each cell is split along its SW-NE diagonal unless ``diag_ne`` (shape (ny-1, nx-1)) is False,
+    in which case the NW-SE diagonal is used.  Triangles are counter-clockwise.  Returns jj (me,3),
+    neigh (me,3), jjs (mes,2), neighs (mes,), sides (mes,).
+    """
+    ci, cj = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1))
+    ci = ci.reshape(-1).astype(np.int64)
+    cj = cj.reshape(-1).astype(np.int64)
+    ncell = ci.size
+    sw = cj * nx + ci
+    se = sw + 1
+    nw = sw + nx
+    ne = nw + 1
+    if diag_ne is None:
+        dne = np.ones(ncell, dtype=bool)
+    else:
+        dne = np.asarray(diag_ne, dtype=bool).reshape(-1)
+    # two triangles per cell, element 2c is the "lower/right" one, 2c+1 the "upper/left" one
+    jj = np.empty((2 * ncell, 3), dtype=np.int64)
+    jj[0::2] = np.where(dne[:, None], np.stack([sw, se, ne], 1), np.stack([sw, se, nw], 1))
+    jj[1::2] = np.where(dne[:, None], np.stack([sw, ne, nw], 1), np.stack([se, ne, nw], 1))
+    me = jj.shape[0]
+    # neighbours through a hash of sorted edges
+    a = np.concatenate([jj[:, 1], jj[:, 2], jj[:, 0]])
+    b = np.concatenate([jj[:, 2], jj[:, 0], jj[:, 1]])
+    owner = np.tile(np.arange(me), 3)
+    slot = np.repeat(np.arange(3), me)           # edge opposite vertex `slot`
+    lo = np.minimum(a, b)
+    hi = np.maximum(a, b)
+    key = lo * (nx * ny) + hi
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    neigh = np.full((me, 3), -1, dtype=np.int64)
+    same = np.nonzero(ks[1:] == ks[:-1])[0]
+    e0 = order[same]
+    e1 = order[same + 1]
+    neigh[owner[e0], slot[e0]] = owner[e1]
+    neigh[owner[e1], slot[e1]] = owner[e0]
+    bmask = neigh[owner, slot] < 0
+    bidx = np.nonzero(bmask)[0]
+    jjs = np.stack([a[bidx], b[bidx]], 1)
+    neighs = owner[bidx]
+    i0, j0 = jjs[:, 0] % nx, jjs[:, 0] // nx
+    i1, j1 = jjs[:, 1] % nx, jjs[:, 1] // nx
+    sides = np.zeros(bidx.size, dtype=np.int64)
+    sides[(j0 == 0) & (j1 == 0)] = 1
+    sides[(i0 == nx - 1) & (i1 == nx - 1)] = 2
+    sides[(j0 == ny - 1) & (j1 == ny - 1)] = 3
+    sides[(i0 == 0) & (i1 == 0)] = 4
+    assert (sides > 0).all()
+    # boundary edges in a stable geometric order: by side, then along the side
+    pos = np.where((sides == 1) | (sides == 3), np.minimum(i0, i1), np.minimum(j0, j1))
+    o = np.lexsort((pos, sides))
+    return jj, neigh, jjs[o], neighs[o], sides[o]
+
+
+def structured_mesh(nx: int, ny: int, x0: float, x1: float, y0: float, y1: float,
+                    jitter: float = 0.0, seed: int = 0) -> Mesh:
+    """Uniform lattice, each cell split along its (x,y)->(x+h,y+h) diagonal.
+
+    ``jitter`` > 0 moves every interior node by U(-jitter*h, +jitter*h) in each direction, from a
+    fixed seed; boundary nodes stay where they are.
+    """
+    xs = np.linspace(x0, x1, nx)
+    ys = np.linspace(y0, y1, ny)
+    X, Y = np.meshgrid(xs, ys)
+    rr = np.stack([X.reshape(-1), Y.reshape(-1)], 1)
+    if jitter > 0.0:
+        hx = (x1 - x0) / (nx - 1)
+        hy = (y1 - y0) / (ny - 1)
+        rng = np.random.default_rng(seed)
+        d = rng.uniform(-jitter, jitter, size=rr.shape)
+        I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+        interior = ((I > 0) & (I < nx - 1) & (J > 0) & (J < ny - 1)).reshape(-1)
+        rr[interior, 0] += d[interior, 0] * hx
+        rr[interior, 1] += d[interior, 1] * hy
+    jj, neigh, jjs, neighs, sides = _grid_topology(nx, ny)
+    return Mesh(rr=rr, jj=jj.astype(np.int32), neigh=neigh.astype(np.int32),
+                i_d=np.ones(jj.shape[0], dtype=np.int32), jjs=jjs.astype(np.int32),
+                neighs=neighs.astype(np.int32), sides=sides.astype(np.int32))
+
+
+def jittered_delaunay_mesh(n_side: int, L: float, jitter: float = 0.25, seed: int = 0) -> Mesh:
+    """Delaunay triangulation of a jittered lattice of ``n_side``^2 points on [0,L]^2.
+
+    Interior points are moved by U(-jitter*h, +jitter*h) (a blue-noise sample with minimum spacing
+    (1-2*jitter)*h).  For jitter <= 0.25 the Delaunay triangulation of such a point set splits each
+    lattice cell along the diagonal chosen by the in-circle test, so it is built cell by cell in
+    O(N) time and memory.
+    """
+    if not (0.0 <= jitter <= 0.25):
+        raise ValueError("jitter must lie in [0, 0.25] for the cell-wise Delaunay construction")
+    m = structured_mesh(n_side, n_side, 0.0, L, 0.0, L, jitter=jitter, seed=seed)
+    nx = ny = n_side
+    rr = m.rr
+    ci, cj = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1))
+    sw = (cj * nx + ci).reshape(-1)
+    se, nw = sw + 1, sw + nx
+    ne = nw + 1
+    # NE is inside the circumcircle of (SW, SE, NW) <=> the SW-NE diagonal is the Delaunay edge
+    def incircle(a, b, c, d):
+        ax, ay = rr[a, 0] - rr[d, 0], rr[a, 1] - rr[d, 1]
+        bx, by = rr[b, 0] - rr[d, 0], rr[b, 1] - rr[d, 1]
+        cx, cy = rr[c, 0] - rr[d, 0], rr[c, 1] - rr[d, 1]
+        return ((ax * ax + ay * ay) * (bx * cy - cx * by)
+                - (bx * bx + by * by) * (ax * cy - cx * ay)
+                + (cx * cx + cy * cy) * (ax * by - bx * ay))
+    diag_ne = incircle(sw, se, nw, ne) > 0.0
+    jj, neigh, jjs, neighs, sides = _grid_topology(nx, ny, diag_ne.reshape(ny - 1, nx - 1))
+    return Mesh(rr=rr, jj=jj.astype(np.int32), neigh=neigh.astype(np.int32),
+                i_d=np.ones(jj.shape[0], dtype=np.int32), jjs=jjs.astype(np.int32),
+                neighs=neighs.astype(np.int32), sides=sides.astype(np.int32))
