@@ -1,0 +1,160 @@
+"""Benchmark of the MI355X explicit shallow-water step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bumps|dam_break|paraboloid]
+
+A step is one SSP-RK3 step (COMPUTE_DT + three Euler stages + bdy) over the whole mesh, with the
+state resident in HBM.  The default workload is the 16.8M-node FCT + alpha problem (still water
+over 64 Gaussian bumps with a sinusoidal inflow, 4097^2 jittered-lattice Delaunay nodes).
+Prints one JSON line: node-stage updates/s, the HBM roofline of the dominant kernel (measured
+with HIP events on the solver's stream), and the CPU checker's rate on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "shallow-water-eric_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes_per_node(kernel, k, alpha=True):
+    """Algorithmic HBM bytes per node of one launch (one graph sweep, each array read once).
+    G = 4 + 20k: row pointer, column index and c_ij (16 B) per entry."""
+    G = 4 + 20 * k
+    return {
+        "k_fct_lo": G + 80,        # U,z,m,alpha in (48); U^L, res out (32)
+        "k_fct_rminus": G + 72,    # U,z (32), res (8), U^L.h... (m, hL, res in; R out)
+        "k_fct_apply": G + 112,    # U,z, res, R, U^L, U^n in; U out
+        "k_stage_lo": 96 + 26.67 * k if not alpha else 124 + 30.67 * k,
+        "k_compute_dt": G + 32,
+        "k_alpha": 4 + 4 * k + 16,
+    }[kernel]
+
+
+def step_bytes_per_node_stage(k, fct=True, alpha=True):
+    """Compulsory bytes per node-stage of the whole step ('Algorithmic bytes')."""
+    if fct:
+        return 324 + 70.67 * k
+    return (124 + 30.67 * k) if alpha else (96 + 26.67 * k)
+
+
+def build_problem(name, size=None):
+    from swe_amd import problems
+    if name == "bumps":
+        return problems.bumps(n_side=size or 4097)
+    if name == "dam_break":
+        nx = size or 2241
+        return problems.dam_break(nx=nx, ny=(nx - 1) // 5 + 1)
+    if name == "paraboloid":
+        return problems.paraboloid(n=size or 81)
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(name, seconds=15.0):
+    """The CPU checker (a restatement of the reference step, 1 thread) on a bounded sample of
+    the same workload family: a smaller mesh, as many steps as fit in about `seconds`."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as O
+    size = {"bumps": 257, "dam_break": 561, "paraboloid": 81}[name]
+    p = build_problem(name, size)
+    hd, uxd, uyd = p.dirichlet()
+    pr = p.params
+    visc = {"galerkin": O.GALERKIN, "viscous": O.VISCOUS, "entropy_visc": O.ENTROPY_VISC, "fct": O.FCT}
+    orc = O.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=pr["gravity"], htiny=pr["htiny"], cfl=pr["cfl"],
+                   max_water_h=pr["max_water_h"], viscosity_type=visc[pr["viscosity"]],
+                   viscous_type=pr["viscous_type"], alpha_limit=pr["alpha_limit"], type_test=pr["type_test"],
+                   mannings=pr["mannings"])
+    kind, amp, period, level = p.bc
+    kw = dict(bc_kind=kind, amp=amp, period=period, level=level, nodes_h=hd, nodes_ux=uxd, nodes_uy=uyd)
+    u = p.u0
+    t0 = time.perf_counter()
+    u1, _, t, err = orc.run(u, 1, **kw)
+    one = time.perf_counter() - t0
+    n = max(1, int(seconds / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    u2, _, t, err = orc.run(u1, n, t0=t, **kw)
+    el = time.perf_counter() - t0
+    rate = p.np * 3 * n / el
+    return {"value": rate, "unit": "node-stage updates/s", "cores": 1, "kind": "port",
+            "sample": "%s, %d nodes, %d steps (%.1f s), CPU restatement of the reference step, 1 thread"
+                      % (name, p.np, n, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="bumps", choices=["bumps", "dam_break", "paraboloid"])
+    ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding of a single mesh is a design note "
+              "only (see DESIGN.md), so --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from swe_amd.solver import load_library
+    load_library()
+    t0 = time.perf_counter()
+    p = build_problem(args.workload, args.size)
+    t_setup = time.perf_counter() - t0
+    k = p.nnz / p.np
+    fct = p.params["viscosity"] == "fct"
+    alpha = p.params["alpha_limit"]
+    dominant = "k_fct_lo" if fct else "k_stage_lo"
+    print("# workload %s: %d nodes, %d entries (k=%.3f), set-up %.1f s" % (args.workload, p.np, p.nnz, k, t_setup),
+          file=sys.stderr, flush=True)
+    s = p.solver()
+    s.set_state(p.u0)
+    s.time = 0.0
+    s.run(args.warmup)
+    s.sync()
+    s.enable_kernel_timing(dominant)
+    t0 = time.perf_counter()
+    s.run_async(args.steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    nlaunch, kms = s.kernel_timing(dominant)
+    s.enable_kernel_timing(None)
+    hmin = float(s.get_state()[:, 0].min())
+    rate = p.np * 3 * args.steps / el
+    bpn = kernel_bytes_per_node(dominant, k, alpha)
+    avg_s = kms / 1e3 / max(nlaunch, 1)
+    achieved = bpn * p.np / avg_s / 1e9
+    res = {
+        "metric": "node-stage updates/s (FP64, SSP-RK3, FCT + alpha)" if fct else "node-stage updates/s (FP64, SSP-RK3)",
+        "value": rate,
+        "unit": "node-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": p.name, "nodes": p.np, "entries": p.nnz, "viscosity": p.params["viscosity"],
+                   "alpha_limit": bool(alpha), "steps": args.steps, "hmin_end": hmin},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": avg_s * 1e3,
+                     "bytes_per_node": bpn},
+        "step_effective_GBs": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9,
+    }
+    s.close()
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/* swe_amd.h -- C-ABI of the MI355X (gfx950) explicit shallow-water step.
+ *
+ * The library replaces the explicit time step of t-bltg/shallow-water-eric: MODULE update
+ * (SOURCES/update.f90) and MODULE fct (SOURCES/fct.f90), with the pointwise physics they call in
+ * SOURCES/condlim.f90, and the per-step work of the driver's time loop (SOURCES/main.f90:105-136:
+ * COMPUTE_DT, the SSP-RK3 combinations and bdy).  Every entry point takes plain pointers and
+ * sizes; arrays keep the reference's layout: 1-based CSR (ia(np+1), ja(nnz), columns ascending,
+ * diagonal included, as st_csr builds it), state un(3,np) in Fortran order (h, qx, qy per node),
+ * c_ij as two arrays cij(1)%aa and cij(2)%aa.
+ *
+ * All arithmetic is IEEE binary64 in the reference's evaluation order; results agree bit for bit
+ * with the reference CPU step except where a source term calls pow (Manning friction), where
+ * they agree to rounding.
+ *
+ * Threading: a handle owns its device buffers and one HIP stream; use a handle from one host
+ * thread at a time.  No host pointer is kept after a call returns.
+ * Errors: every call returns a swe_status; swe_last_error() gives a message for the calling
+ * thread.
+ */
+#ifndef SWE_AMD_H
+#define SWE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum swe_status {
+    SWE_OK = 0,
+    SWE_ENEG_H = 1,     /* 'Min h<0, STOP' of check_hmin (SOURCES/update.f90:816-820) */
+    SWE_ELAMBDA = 2,    /* 'BUG in compute_lambda_vacc' (SOURCES/condlim.f90:934-937) */
+    SWE_EBADOPT = 3,    /* option strings the reference rejects (SOURCES/update.f90:199,217,243,273) */
+    SWE_EHIP = 4,       /* a HIP runtime call failed */
+    SWE_EINVAL = 5,     /* malformed arguments (sizes, indices, null pointers) */
+    SWE_ENOMEM = 6      /* device allocation failed */
+} swe_status;
+
+typedef enum swe_viscosity {      /* inputs%viscosity_type (SOURCES/read_data.f90:57-58) */
+    SWE_GALERKIN = 0,
+    SWE_VISCOUS = 1,
+    SWE_ENTROPY_VISC = 2,
+    SWE_FCT = 3
+} swe_viscosity;
+
+typedef enum swe_bc_kind {        /* Dirichlet values written by bdy after each stage */
+    SWE_BC_NONE = 0,              /* no Dirichlet nodes */
+    SWE_BC_WALLS = 1,             /* qx = 0 on the ux list, qy = 0 on the uy list */
+    SWE_BC_INFLOW = 2,            /* walls, plus h = max(level - z + amp*sin(2*pi*t/period), 0) on the h list */
+    SWE_BC_VALUES = 3             /* fixed values given per listed node by swe_set_bc_values */
+} swe_bc_kind;
+
+typedef struct swe_params {
+    double gravity;       /* inputs%gravity */
+    double htiny;         /* inputs%htiny = epsilon*max_water_h (SOURCES/condlim.f90:233) */
+    double cfl;           /* inputs%CFL */
+    double max_water_h;   /* max_water_h (entropy-residual scaling, SOURCES/update.f90:866) */
+    double mannings;      /* inputs%mannings (tests 9 and 10) */
+    int type_test;        /* inputs%type_test: 9/10 Manning friction, 11 linear friction, 12 Coriolis */
+    int viscosity_type;   /* swe_viscosity */
+    int viscous_type;     /* 1 = 'type1' (smb_1), 2 = 'type2' (smb_2 family) */
+    int alpha_limit;      /* inputs%if_alpha_limit */
+    int roundoff_fix;     /* if_roundoff_fix (SOURCES/update.f90:15; the reference fixes .TRUE.) */
+} swe_params;
+
+typedef struct swe_mesh_desc {
+    int np;               /* mesh%np */
+    int nnz;              /* SIZE(ja) */
+    const int *ia;        /* 1-based row pointers, np+1 */
+    const int *ja;        /* 1-based column indices, nnz */
+    const double *cx;     /* cij(1)%aa, nnz */
+    const double *cy;     /* cij(2)%aa, nnz */
+    const double *lumped; /* lumped mass, np */
+    const double *bath;   /* bathymetry z, np */
+    int n_h_dir, n_ux_dir, n_uy_dir;     /* sizes of h_js_D, ux_js_D, uy_js_D */
+    const int *h_dir, *ux_dir, *uy_dir;  /* 1-based node lists (SOURCES/mesh.f90:25-34) */
+} swe_mesh_desc;
+
+typedef struct swe_handle swe_handle;
+
+/* Message describing the last error of the calling thread ("" if none). */
+const char *swe_last_error(void);
+
+/* P1 set-up of construct_matrices (SOURCES/update.f90:18-132) with the geometry of
+ * gauss_points_2d_p1 (SFEMaNS/GAUSS_POINTS_OBJET/gauss_points_2d_p1.f90:106-132) and the pattern
+ * of st_csr (SFEMaNS/FEMSUB/st_sparsekit.f90:788-863).  jj is 1-based jj(3,me); rr is rr(2,np).
+ * swe_assemble_nnz returns SIZE(ja); swe_assemble fills ia, ja (1-based), diag (1-based),
+ * mass%aa, lumped, cij(1)%aa, cij(2)%aa. */
+long swe_assemble_nnz(int np, int me, const int *jj);
+int swe_assemble(int np, int me, const int *jj, const double *rr, int *ia, int *ja, int *diag,
+                 double *mass, double *lumped, double *cx, double *cy);
+
+/* Create a solver over one mesh: copies the mesh arrays to the device.  Replaces the module
+ * state built by construct_matrices (SOURCES/update.f90:18-132) and construct_mesh. */
+int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **out);
+int swe_destroy(swe_handle *h);
+int swe_set_params(swe_handle *h, const swe_params *prm);
+
+/* Dirichlet program applied by the device-resident steps (bdy, SOURCES/main.f90:220-232). */
+int swe_set_bc(swe_handle *h, int kind, double amp, double period, double level);
+int swe_set_bc_values(swe_handle *h, const double *h_vals, const double *ux_vals, const double *uy_vals);
+
+/* compute_dij (SOURCES/update.f90:291-326): dij%aa at state un(3,np) (host arrays). */
+int swe_compute_dij(swe_handle *h, const double *un, double *dij);
+
+/* COMPUTE_DT (SOURCES/main.f90:210-218): dt = CFL/MAXVAL(ABS(dij%aa(diag))/lumped).
+ * un == NULL uses the device-resident state. */
+int swe_compute_dt(swe_handle *h, const double *un, double *dt);
+
+/* euler(un, unext) (SOURCES/update.f90:134-289): one forward-Euler stage with time step dt on
+ * host arrays un(3,np) -> unext(3,np).  Per-stage drop-in form; copies over PCIe each call. */
+int swe_euler(swe_handle *h, const double *un, double *unext, double dt);
+
+/* FCT_positivity (SOURCES/fct.f90:5-42) on the handle's CSR pattern: unext_scaled(np) is
+ * (lumped/dt)*h of the low-order solution, f(nnz) the antidiffusive fluxes, lij(nnz) out. */
+int swe_fct_positivity(swe_handle *h, const double *unext_scaled, const double *f, double *lij);
+
+/* Device-resident state. */
+int swe_set_state(swe_handle *h, const double *un);
+int swe_get_state(swe_handle *h, double *un);
+
+/* Simulation time of the handle (inputs%time); it starts at 0 and each step adds its dt. */
+int swe_set_time(swe_handle *h, double t);
+int swe_get_time(swe_handle *h, double *t);
+
+/* nsteps SSP-RK3 steps on the device-resident state, as in SOURCES/main.f90:111-135 with a
+ * fixed step count: COMPUTE_DT, three euler stages, the combinations (3*uo+ui)/4 and
+ * (uo+2*ui)/3, and bdy after each stage.  If time is not NULL the run starts at *time and
+ * *time receives the end time; dts (may be NULL) receives dt of each step.  Returns
+ * SWE_ENEG_H / SWE_ELAMBDA if any stage tripped those checks. */
+int swe_run(swe_handle *h, int nsteps, double *time, double *dts);
+
+/* The same steps queued without waiting for completion (a time-dependent inflow still reads
+ * each step's dt back while the step's first stage runs).  swe_sync waits for the queued work,
+ * folds the queued dts into the handle's time and reports the checks. */
+int swe_run_async(swe_handle *h, int nsteps);
+int swe_sync(swe_handle *h);
+
+/* The handle's HIP stream (hipStream_t), for callers that time or order their own work. */
+void *swe_stream(swe_handle *h);
+
+/* Per-launch timing of one kernel (by name, e.g. "k_fct_lo"; NULL stops timing) with HIP events
+ * recorded around each of its launches on the handle's stream.  swe_kernel_timing returns the
+ * number of launches since timing was enabled and their summed duration in milliseconds. */
+int swe_enable_kernel_timing(swe_handle *h, const char *kernel);
+int swe_kernel_timing(swe_handle *h, const char *kernel, int *launches, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWE_AMD_H */

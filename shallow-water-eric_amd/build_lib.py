@@ -1,0 +1,49 @@
+"""Build libswe_amd.so in place with hipcc for gfx950.
+
+    python shallow-water-eric_amd/build_lib.py
+
+-ffp-contract=off keeps every multiply and add separately rounded, as in the reference build.
+Packed FP32 instructions are disabled for the whole library.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+OUT = os.path.join(PKG, "libswe_amd.so")
+SOURCES = [os.path.join(PKG, "csrc", "swe_amd.hip"), os.path.join(PKG, "csrc", "swe_setup.cpp")]
+HEADERS = [os.path.join(REPO, "include", "swe_amd.h"), os.path.join(PKG, "csrc", "swe_kernels.h")]
+
+
+def hipcc():
+    for c in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def command(out=OUT, extra=()):
+    return [hipcc(), "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I", os.path.join(REPO, "include"), "-o", out, *extra, *SOURCES]
+
+
+def build(verbose: bool = False) -> str:
+    tmp = OUT + ".tmp"
+    cmd = command(tmp)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed (%d):\n%s\n%s" % (r.returncode, " ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, OUT)
+    if verbose:
+        print(" ".join(cmd))
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

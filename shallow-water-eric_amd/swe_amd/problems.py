@@ -1,0 +1,152 @@
+"""Problem set-ups for the benchmark workloads and the parity cases.
+
+* :func:`paraboloid`   planar free surface oscillating in a paraboloid: the reference's test 5
+  (init: SOURCES/condlim.f90:151-161, sol_anal: 429-456), first-order graph viscosity.
+* :func:`dam_break`    dam break on a flat [0,100]x[0,20] rectangle with a jittered structured
+  mesh, Manning friction (the reference's friction source, dispatched for type_test 10,
+  SOURCES/update.f90:614-616), FCT + alpha limiter, reflecting walls.
+* :func:`bumps`        still water over seeded Gaussian bumps on a jittered-lattice Delaunay mesh,
+  with a sinusoidal free-surface inflow on the west side; FCT + alpha limiter.
+
+Node numbering is the reference loader's (first-touch renumbering of the generated element list),
+so a problem built here and the reference fed with the same ``.FEM`` file see the same arrays.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import mesh as M
+from .solver import BC_INFLOW, BC_NONE, BC_WALLS, Solver, assemble
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    mesh: M.Mesh                 # renumbered as the reference loader does
+    ia: np.ndarray
+    ja: np.ndarray
+    diag: np.ndarray
+    lumped: np.ndarray
+    cij: np.ndarray
+    bath: np.ndarray
+    u0: np.ndarray               # (np, 3)
+    params: dict
+    h_sides: tuple = ()
+    ux_sides: tuple = ()
+    uy_sides: tuple = ()
+    bc: tuple = (BC_NONE, 0.0, 1.0, 0.0)   # (kind, amp, period, level)
+    nsteps: int = 1
+    reference_init: bool = False           # True when the reference's own init covers the case
+
+    @property
+    def np(self):
+        return self.mesh.np
+
+    @property
+    def nnz(self):
+        return int(self.ja.size)
+
+    def dirichlet(self):
+        return (M.dirichlet_nodes(self.mesh, self.h_sides), M.dirichlet_nodes(self.mesh, self.ux_sides),
+                M.dirichlet_nodes(self.mesh, self.uy_sides))
+
+    def solver(self, **overrides) -> Solver:
+        prm = dict(self.params)
+        prm.update(overrides)
+        hd, uxd, uyd = self.dirichlet()
+        s = Solver(self.ia, self.ja, self.cij, self.lumped, self.bath, h_dir=hd, ux_dir=uxd, uy_dir=uyd, **prm)
+        s.set_bc(*self.bc)
+        return s
+
+
+def _finish(name, raw_mesh, bath_fn, u0_fn, params, **kw):
+    m = M.renumber_first_touch(raw_mesh)
+    ia, ja, diag, _mass, lumped, cij = assemble(m.jj, m.rr)
+    bath = bath_fn(m.rr)
+    u0 = u0_fn(m.rr, bath)
+    return Problem(name=name, mesh=m, ia=ia, ja=ja, diag=diag, lumped=lumped, cij=cij, bath=bath,
+                   u0=np.ascontiguousarray(u0), params=params, **kw)
+
+
+# ----------------------------------------------------------------------------------------------
+def paraboloid_state(rr, t, htiny):
+    """sol_anal of test 5 (SOURCES/condlim.f90:429-456), evaluated in the reference's order."""
+    g, a, h0, hL, eta = 9.81, 1.0, 0.1, 4.0, 0.5
+    omega = math.sqrt(2 * g * h0) / a
+    x, y = rr[:, 0], rr[:, 1]
+    bathi = -h0 * (1.0 - ((x - hL / 2.0) ** 2 + (y - hL / 2.0) ** 2) / a ** 2)
+    c, s = math.cos(omega * t), math.sin(omega * t)
+    base = np.maximum((eta * h0 / a ** 2) * (2.0 * (x - hL / 2.0) * c + 2.0 * (y - hL / 2.0) * s) - bathi, 0.0)
+    u = np.empty((rr.shape[0], 3))
+    u[:, 0] = base + htiny
+    u[:, 1] = -eta * omega * s * base
+    u[:, 2] = eta * omega * c * base
+    return u
+
+
+def paraboloid(n=81, cfl=0.25, viscosity="viscous", alpha=False, viscous_type=2) -> Problem:
+    """Test 5 on an n-by-n P1 grid of [0,4]^2 split along (x,y)->(x+h,y+h): the first-order
+    graph-viscosity configuration (81x81 = 6,561 nodes, CFL 0.25, 2000 steps)."""
+    g, h0, a, hL = 9.81, 0.1, 1.0, 4.0
+    max_water_h = hL
+    htiny = 1.0e-4 * max_water_h
+
+    def bath(rr):
+        return -h0 * (1.0 - ((rr[:, 0] - hL / 2.0) ** 2 + (rr[:, 1] - hL / 2.0) ** 2) / a ** 2)
+
+    params = dict(gravity=g, htiny=htiny, cfl=cfl, max_water_h=max_water_h, viscosity=viscosity,
+                  viscous_type=viscous_type, alpha_limit=alpha, type_test=5, mannings=0.0)
+    return _finish("paraboloid", M.structured_mesh(n, n, 0.0, 4.0, 0.0, 4.0), bath,
+                   lambda rr, z: paraboloid_state(rr, 0.0, htiny), params, nsteps=2000, reference_init=True)
+
+
+# ----------------------------------------------------------------------------------------------
+def dam_break(nx=2241, ny=449, jitter=0.2, seed=20261202, cfl=0.25, mannings=0.02,
+              viscosity="fct", alpha=True) -> Problem:
+    """Dam break on [0,100]x[0,20]: h=1 for x<50, dry elsewhere, q=0, flat bottom, Manning n=0.02,
+    walls (qx=0 on x=0 and x=100, qy=0 on y=0 and y=20).  2241x449 = 1,006,209 nodes, 500 steps."""
+    params = dict(gravity=9.81, htiny=1.0e-4 * 1.0, cfl=cfl, max_water_h=1.0, viscosity=viscosity,
+                  viscous_type=2, alpha_limit=alpha, type_test=10, mannings=mannings)
+
+    def u0(rr, z):
+        u = np.zeros((rr.shape[0], 3))
+        u[:, 0] = np.where(rr[:, 0] < 50.0, 1.0, 0.0)
+        return u
+    return _finish("dam_break", M.structured_mesh(nx, ny, 0.0, 100.0, 0.0, 20.0, jitter=jitter, seed=seed),
+                   lambda rr: np.zeros(rr.shape[0]), u0, params, ux_sides=(2, 4), uy_sides=(1, 3),
+                   bc=(BC_WALLS, 0.0, 1.0, 0.0), nsteps=500)
+
+
+# ----------------------------------------------------------------------------------------------
+def bump_field(rr, L, nbumps=64, seed=20261202):
+    """z = sum_k A_k exp(-|x-c_k|^2/(2 w_k^2)), A ~ U[0,0.8], w ~ U[0.02L, 0.1L], c ~ U[0,L]^2."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.0, 0.8, nbumps)
+    w = rng.uniform(0.02 * L, 0.1 * L, nbumps)
+    c = rng.uniform(0.0, L, (nbumps, 2))
+    z = np.zeros(rr.shape[0])
+    for k in range(nbumps):
+        d2 = (rr[:, 0] - c[k, 0]) ** 2 + (rr[:, 1] - c[k, 1]) ** 2
+        z += A[k] * np.exp(-d2 / (2.0 * w[k] ** 2))
+    return z
+
+
+def bumps(n_side=4097, L=1000.0, nbumps=64, seed=20261202, cfl=0.25, amp=0.05, period=20.0,
+          viscosity="fct", alpha=True) -> Problem:
+    """Still water of level 1 over 64 Gaussian bumps (dry islands where z > 1) on a Delaunay mesh
+    of n_side^2 jittered lattice points of [0,L]^2; free-surface inflow
+    h = max(1 - z + amp*sin(2*pi*t/period), 0) on the west side, walls elsewhere.
+    n_side = 4097 gives 16,785,409 nodes (the 16.8M-node workload), 200 steps."""
+    params = dict(gravity=9.81, htiny=1.0e-4 * 1.0, cfl=cfl, max_water_h=1.0, viscosity=viscosity,
+                  viscous_type=2, alpha_limit=alpha, type_test=4, mannings=0.0)
+
+    def u0(rr, z):
+        u = np.zeros((rr.shape[0], 3))
+        u[:, 0] = np.maximum(1.0 - z, 0.0)
+        return u
+    return _finish("bumps", M.jittered_delaunay_mesh(n_side, L, jitter=0.25, seed=seed),
+                   lambda rr: bump_field(rr, L, nbumps, seed), u0, params,
+                   h_sides=(4,), ux_sides=(2,), uy_sides=(1, 3), bc=(BC_INFLOW, amp, period, 1.0), nsteps=200)

@@ -1,0 +1,266 @@
+"""ctypes binding of libswe_amd.so (include/swe_amd.h): the host-side mirror of the reference's
+MODULE update / MODULE fct interface (SOURCES/update.f90:5-10, SOURCES/fct.f90:5).
+
+States are numpy arrays of shape (np, 3), which is the memory layout of the reference's
+un(3,np) (h, qx, qy per node).  CSR arrays given to :class:`Solver` are 0-based; the C-ABI gets
+the reference's 1-based numbering.  Every failure raises :class:`SweError`; the library has no
+CPU fallback, so a missing or unloadable library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libswe_amd.so")
+
+SWE_OK, SWE_ENEG_H, SWE_ELAMBDA, SWE_EBADOPT, SWE_EHIP, SWE_EINVAL, SWE_ENOMEM = range(7)
+GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
+BC_NONE, BC_WALLS, BC_INFLOW, BC_VALUES = 0, 1, 2, 3
+VISCOSITY = {"galerkin": GALERKIN, "viscous": VISCOUS, "entropy_visc": ENTROPY_VISC, "fct": FCT}
+
+
+class SweError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("swe status %d: %s" % (status, message))
+        self.status = status
+
+
+class Params(C.Structure):
+    _fields_ = [("gravity", C.c_double), ("htiny", C.c_double), ("cfl", C.c_double),
+                ("max_water_h", C.c_double), ("mannings", C.c_double),
+                ("type_test", C.c_int), ("viscosity_type", C.c_int), ("viscous_type", C.c_int),
+                ("alpha_limit", C.c_int), ("roundoff_fix", C.c_int)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("np", C.c_int), ("nnz", C.c_int), ("ia", C.c_void_p), ("ja", C.c_void_p),
+                ("cx", C.c_void_p), ("cy", C.c_void_p), ("lumped", C.c_void_p), ("bath", C.c_void_p),
+                ("n_h_dir", C.c_int), ("n_ux_dir", C.c_int), ("n_uy_dir", C.c_int),
+                ("h_dir", C.c_void_p), ("ux_dir", C.c_void_p), ("uy_dir", C.c_void_p)]
+
+
+# every symbol the header declares: name -> (restype, argtypes)
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+SIGNATURES = {
+    "swe_last_error": (C.c_char_p, []),
+    "swe_assemble_nnz": (C.c_long, [_I, _I, _P]),
+    "swe_assemble": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "swe_create": (_I, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(_P)]),
+    "swe_destroy": (_I, [_P]),
+    "swe_set_params": (_I, [_P, C.POINTER(Params)]),
+    "swe_set_bc": (_I, [_P, _I, _D, _D, _D]),
+    "swe_set_bc_values": (_I, [_P, _P, _P, _P]),
+    "swe_compute_dij": (_I, [_P, _P, _P]),
+    "swe_compute_dt": (_I, [_P, _P, C.POINTER(_D)]),
+    "swe_euler": (_I, [_P, _P, _P, _D]),
+    "swe_fct_positivity": (_I, [_P, _P, _P, _P]),
+    "swe_set_state": (_I, [_P, _P]),
+    "swe_get_state": (_I, [_P, _P]),
+    "swe_set_time": (_I, [_P, _D]),
+    "swe_get_time": (_I, [_P, C.POINTER(_D)]),
+    "swe_run": (_I, [_P, _I, C.POINTER(_D), _P]),
+    "swe_run_async": (_I, [_P, _I]),
+    "swe_sync": (_I, [_P]),
+    "swe_stream": (_P, [_P]),
+    "swe_enable_kernel_timing": (_I, [_P, C.c_char_p]),
+    "swe_kernel_timing": (_I, [_P, C.c_char_p, C.POINTER(_I), C.POINTER(_D)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libswe_amd.so and bind every declared symbol (raises if the library is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SweError(SWE_EHIP, "libswe_amd.so is not built at %s (run __graft_entry__.build())" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != SWE_OK:
+        msg = load_library().swe_last_error()
+        raise SweError(rc, msg.decode() if msg else "")
+
+
+def assemble(jj0: np.ndarray, rr: np.ndarray):
+    """P1 set-up (construct_matrices): returns 0-based (ia, ja, diag), mass, lumped, cij (nnz,2)."""
+    lib = load_library()
+    jj1 = np.ascontiguousarray(jj0, dtype=np.int32) + 1
+    rr = np.ascontiguousarray(rr, dtype=np.float64)
+    npn, me = rr.shape[0], jj1.shape[0]
+    nnz = lib.swe_assemble_nnz(npn, me, _ptr(jj1))
+    if nnz < 0:
+        raise SweError(SWE_EINVAL, "invalid element list")
+    ia = np.zeros(npn + 1, np.int32)
+    ja = np.zeros(nnz, np.int32)
+    diag = np.zeros(npn, np.int32)
+    mass = np.zeros(nnz)
+    lumped = np.zeros(npn)
+    cx = np.zeros(nnz)
+    cy = np.zeros(nnz)
+    _check(lib.swe_assemble(npn, me, _ptr(jj1), _ptr(rr), _ptr(ia), _ptr(ja), _ptr(diag), _ptr(mass),
+                            _ptr(lumped), _ptr(cx), _ptr(cy)))
+    return ia - 1, ja - 1, diag - 1, mass, lumped, np.stack([cx, cy], 1)
+
+
+class Solver:
+    """One mesh, one parameter set, one device-resident state.
+
+    Methods follow the reference: :meth:`compute_dij` (update.f90:291), :meth:`compute_dt`
+    (COMPUTE_DT, main.f90:210), :meth:`euler` (update.f90:134), :meth:`fct_positivity`
+    (fct.f90:5) and the SSP-RK3 time loop :meth:`run` (main.f90:105-136).
+    """
+
+    def __init__(self, ia, ja, cij, lumped, bath, *, gravity, htiny, cfl, max_water_h,
+                 viscosity="viscous", viscous_type=2, alpha_limit=False, type_test=5, mannings=0.0,
+                 roundoff_fix=True, h_dir=(), ux_dir=(), uy_dir=()):
+        lib = load_library()
+        self.ia1 = np.ascontiguousarray(ia, np.int32) + 1
+        self.ja1 = np.ascontiguousarray(ja, np.int32) + 1
+        self.np = self.ia1.size - 1
+        self.nnz = self.ja1.size
+        cij = np.asarray(cij, np.float64).reshape(self.nnz, 2)
+        self._cx = np.ascontiguousarray(cij[:, 0])
+        self._cy = np.ascontiguousarray(cij[:, 1])
+        self._lumped = np.ascontiguousarray(lumped, np.float64)
+        self._bath = np.ascontiguousarray(bath, np.float64)
+        self._lists = [np.ascontiguousarray(x, np.int32) + 1 for x in (h_dir, ux_dir, uy_dir)]
+        vt = VISCOSITY[viscosity] if isinstance(viscosity, str) else int(viscosity)
+        self.params = Params(gravity, htiny, cfl, max_water_h, mannings, type_test, vt, viscous_type,
+                             int(bool(alpha_limit)), int(bool(roundoff_fix)))
+        desc = MeshDesc(self.np, self.nnz, _ptr(self.ia1), _ptr(self.ja1), _ptr(self._cx), _ptr(self._cy),
+                        _ptr(self._lumped), _ptr(self._bath),
+                        self._lists[0].size, self._lists[1].size, self._lists[2].size,
+                        _ptr(self._lists[0]), _ptr(self._lists[1]), _ptr(self._lists[2]))
+        h = C.c_void_p()
+        _check(lib.swe_create(C.byref(desc), C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    # -- lifetime -------------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().swe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- configuration ----------------------------------------------------------------------------
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k == "viscosity":
+                k, v = "viscosity_type", VISCOSITY[v] if isinstance(v, str) else v
+            setattr(self.params, k, v)
+        _check(load_library().swe_set_params(self.h, C.byref(self.params)))
+
+    def set_bc(self, kind, amp=0.0, period=1.0, level=0.0):
+        _check(load_library().swe_set_bc(self.h, kind, amp, period, level))
+
+    def set_bc_values(self, h_vals=None, ux_vals=None, uy_vals=None):
+        arrs = [None if v is None else np.ascontiguousarray(v, np.float64) for v in (h_vals, ux_vals, uy_vals)]
+        _check(load_library().swe_set_bc_values(self.h, *[_ptr(a) for a in arrs]))
+
+    # -- reference routines on host arrays ----------------------------------------------------------
+    def _state(self, un):
+        un = np.ascontiguousarray(un, np.float64)
+        if un.shape != (self.np, 3):
+            raise SweError(SWE_EINVAL, "state must have shape (%d, 3), got %s" % (self.np, un.shape))
+        return un
+
+    def compute_dij(self, un):
+        un = self._state(un)
+        d = np.zeros(self.nnz)
+        _check(load_library().swe_compute_dij(self.h, _ptr(un), _ptr(d)))
+        return d
+
+    def compute_dt(self, un=None):
+        dt = C.c_double()
+        _check(load_library().swe_compute_dt(self.h, _ptr(None if un is None else self._state(un)), C.byref(dt)))
+        return dt.value
+
+    def euler(self, un, dt):
+        un = self._state(un)
+        out = np.zeros_like(un)
+        _check(load_library().swe_euler(self.h, _ptr(un), _ptr(out), float(dt)))
+        return out
+
+    def fct_positivity(self, unext_scaled, f):
+        us = np.ascontiguousarray(unext_scaled, np.float64)
+        f = np.ascontiguousarray(f, np.float64)
+        if us.size != self.np or f.size != self.nnz:
+            raise SweError(SWE_EINVAL, "fct_positivity: sizes do not match the mesh")
+        lij = np.zeros(self.nnz)
+        _check(load_library().swe_fct_positivity(self.h, _ptr(us), _ptr(f), _ptr(lij)))
+        return lij
+
+    # -- device-resident time loop ----------------------------------------------------------------
+    def set_state(self, un):
+        _check(load_library().swe_set_state(self.h, _ptr(self._state(un))))
+
+    def get_state(self):
+        out = np.zeros((self.np, 3))
+        _check(load_library().swe_get_state(self.h, _ptr(out)))
+        return out
+
+    @property
+    def time(self):
+        t = C.c_double()
+        _check(load_library().swe_get_time(self.h, C.byref(t)))
+        return t.value
+
+    @time.setter
+    def time(self, t):
+        _check(load_library().swe_set_time(self.h, float(t)))
+
+    def run(self, nsteps, time=None):
+        """nsteps SSP-RK3 steps; returns (end time, dt of each step)."""
+        dts = np.zeros(max(int(nsteps), 1))
+        t = C.c_double(self.time if time is None else float(time))
+        _check(load_library().swe_run(self.h, int(nsteps), C.byref(t), _ptr(dts)))
+        return t.value, dts[:nsteps]
+
+    def run_async(self, nsteps):
+        _check(load_library().swe_run_async(self.h, int(nsteps)))
+
+    def sync(self):
+        _check(load_library().swe_sync(self.h))
+
+    @property
+    def stream(self):
+        return load_library().swe_stream(self.h)
+
+    def enable_kernel_timing(self, kernel):
+        _check(load_library().swe_enable_kernel_timing(self.h, None if kernel is None else kernel.encode()))
+
+    def kernel_timing(self, kernel):
+        n = C.c_int()
+        ms = C.c_double()
+        _check(load_library().swe_kernel_timing(self.h, kernel.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value

@@ -1,0 +1,183 @@
+"""The HIP library (through its C-ABI) against the reference fixtures and the CPU checker.
+
+Bar: bit-for-bit equality, except runs with Manning friction (pow), which must agree to
+relative L1 and Linf <= 1e-11 in (h, qx, qy).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_diff, same_bits
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+RUNS = {
+    "dambreak_run.npz": ("fct", True, 10, 0.02, 1),
+    "bumps_run.npz": ("fct", True, 4, 0.0, 2),
+    "cav025_run.npz": ("fct", True, 4, 0.0, 0),
+    "cdisk_coriolis.npz": ("viscous", True, 12, 0.0, 0),
+    "par_friction.npz": ("viscous", True, 11, 0.0, 0),
+}
+
+
+def solver_from(z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, bc=0):
+    from swe_amd.solver import Solver
+    g, htiny, hmax, cfl = z["scalars"]
+    s = Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+               max_water_h=hmax, viscosity=viscosity, viscous_type=viscous_type, alpha_limit=alpha,
+               type_test=type_test, mannings=mannings, h_dir=z["h_dir"], ux_dir=z["ux_dir"], uy_dir=z["uy_dir"])
+    if bc == 2:
+        amp, period, level = z["bc"]
+        s.set_bc(2, amp, period, level)
+    else:
+        s.set_bc(bc)
+    return s
+
+
+def test_compute_dij_bitwise():
+    z = golden("par41_modes.npz")
+    with solver_from(z, "viscous", False, 5) as s:
+        for k in range(2):
+            print("compute_dij state", k, flush=True)
+            assert same_bits(s.compute_dij(z["states"][k]), z["dij_at_state"][k])
+
+
+def test_lambda_bitwise_on_random_states(oracle):
+    """compute_dij over a mesh with random wet/dry states: every |c_ij| lambda_ij bit for bit."""
+    from swe_amd import problems
+    p = problems.dam_break(61, 15, seed=5)
+    rng = np.random.default_rng(1)
+    orc = oracle.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0,
+                        type_test=4)
+    with p.solver(type_test=4) as s:
+        for trial in range(5):
+            u = np.zeros((p.np, 3))
+            u[:, 0] = rng.uniform(0, 2, p.np) ** 3 * (rng.random(p.np) < 0.8)
+            u[:, 1:] = rng.normal(0, 2.0, (p.np, 2)) * u[:, :1]
+            d_ref, err = orc.compute_dij(u)
+            assert err == 0
+            print("random state", trial, flush=True)
+            assert same_bits(s.compute_dij(u), d_ref)
+            assert s.compute_dt(u) == orc.compute_dt(u)[0]
+
+
+def test_euler_every_mode_bitwise():
+    from swe_amd.solver import SWE_ENEG_H, SweError
+    z = golden("par41_modes.npz")
+    for k in range(2):
+        for m, mode in enumerate(z["modes"]):
+            visc, alpha, vt = str(mode).split("|")
+            print("euler state", k, "mode", mode, flush=True)
+            with solver_from(z, visc, bool(int(alpha)), 5, viscous_type=1 if vt == "type1" else 2) as s:
+                if z["status"][k, m] == 1:
+                    with pytest.raises(SweError) as e:
+                        s.euler(z["states"][k], float(z["dts"][k]))
+                    assert e.value.status == SWE_ENEG_H
+                else:
+                    assert same_bits(s.euler(z["states"][k], float(z["dts"][k])), z["unext"][k, m]), mode
+
+
+def test_fct_positivity_bitwise():
+    from swe_amd.solver import Solver
+    z = golden("fct.npz")
+    npn = z["ia"].size - 1
+    ia, ja = z["ia"], z["ja"]
+    with Solver(ia, ja, np.ones((ja.size, 2)), np.ones(npn), np.zeros(npn), gravity=9.81, htiny=float(z["htiny"]),
+                cfl=0.25, max_water_h=1.0) as s:
+        assert same_bits(s.fct_positivity(z["us"], z["f"]), z["lij"])
+
+
+def test_first_order_workload_2000_steps_bitwise():
+    """Test 5 on the 81x81 grid, 2000 steps: the reference's final state and every dt."""
+    from swe_amd import problems
+    z = golden("par81_run.npz")
+    p = problems.paraboloid(81)
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t, dts = s.run(1, time=0.0)
+        assert same_bits(s.get_state(), z["s1"][2])
+        s.set_state(p.u0)
+        done, t = 0, 0.0
+        for snap, step in zip(z["snaps"], z["snap_steps"]):
+            print("paraboloid to step", step, flush=True)
+            t, d = s.run(int(step) - done, time=t)
+            assert same_bits(d, z["dts"][done:int(step)])
+            assert same_bits(s.get_state(), snap), step
+            done = int(step)
+        assert same_bits(s.get_state(), z["final"])
+        assert t == float(z["time"])
+
+
+@pytest.mark.parametrize("name", sorted(RUNS))
+def test_runs_against_reference(name):
+    z = golden(name)
+    visc, alpha, tt, mannings, bc = RUNS[name]
+    with solver_from(z, visc, alpha, tt, mannings, bc=bc) as s:
+        s.set_state(z["u0"])
+        t, done = float(z["time0"]), 0
+        for snap, step in zip(z["snaps"], z["snap_steps"]):
+            print(name, "to step", step, flush=True)
+            t, d = s.run(int(step) - done, time=t)
+            u = s.get_state()
+            if mannings > 0:
+                l1, linf = rel_diff(u, snap)
+                assert l1 <= TOL and linf <= TOL, (step, l1, linf)
+            else:
+                assert same_bits(d, z["dts"][done:int(step)])
+                assert same_bits(u, snap), (name, step)
+            done = int(step)
+        t, d = s.run(z["dts"].size - done, time=t)
+        u = s.get_state()
+        l1, linf = rel_diff(u, z["final"])
+        assert l1 <= TOL and linf <= TOL
+        if mannings == 0:
+            assert same_bits(u, z["final"]) and t == float(z["time"])
+        assert u[:, 0].min() >= 0.0
+
+
+def test_dam_break_63k_nodes_against_oracle(oracle):
+    """Jittered 561x113 dam break (63,393 nodes), FCT + alpha + Manning, 30 steps."""
+    from swe_amd import problems
+    p = problems.dam_break(561, 113, seed=9)
+    hd, uxd, uyd = p.dirichlet()
+    orc = oracle.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0,
+                        viscosity_type=oracle.FCT, alpha_limit=True, type_test=10, mannings=0.02)
+    ref, rdts, rt, err = orc.run(p.u0, 30, bc_kind=1, nodes_ux=uxd, nodes_uy=uyd)
+    assert err == 0
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t, d = s.run(30, time=0.0)
+        u = s.get_state()
+    l1, linf = rel_diff(u, ref)
+    print("dam break 63k: rel L1 %.3g Linf %.3g, bitwise fraction %.6f" % (l1, linf, np.mean(u == ref)), flush=True)
+    assert l1 <= TOL and linf <= TOL
+    assert np.abs(d - rdts).max() <= TOL * rdts.max()
+
+
+def test_bumps_inflow_66k_nodes_bitwise(oracle):
+    """257x257 Delaunay mesh over Gaussian bumps with inflow, FCT + alpha, 20 steps."""
+    from swe_amd import problems
+    p = problems.bumps(n_side=257, L=1000.0, seed=4)
+    hd, uxd, uyd = p.dirichlet()
+    orc = oracle.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0,
+                        viscosity_type=oracle.FCT, alpha_limit=True, type_test=4)
+    kind, amp, period, level = p.bc
+    ref, rdts, rt, err = orc.run(p.u0, 20, bc_kind=kind, amp=amp, period=period, level=level, nodes_h=hd,
+                                 nodes_ux=uxd, nodes_uy=uyd)
+    assert err == 0
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t, d = s.run(20, time=0.0)
+        assert same_bits(d, rdts) and t == rt
+        assert same_bits(s.get_state(), ref)
+
+
+def test_negative_depth_is_reported():
+    """check_hmin: an FCT stage whose low-order depth goes negative returns SWE_ENEG_H."""
+    from swe_amd.solver import SWE_ENEG_H, SweError
+    z = golden("par41_modes.npz")
+    with solver_from(z, "fct", True, 5) as s:
+        s.set_state(z["u0"])
+        with pytest.raises(SweError) as e:
+            s.run(1, time=0.0)
+        assert e.value.status == SWE_ENEG_H
