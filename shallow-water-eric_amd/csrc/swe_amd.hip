@@ -40,17 +40,31 @@ using namespace swe;
 namespace {
 
 constexpr int kBlock = 256;
+#ifndef SWE_SORT_WINDOW
+#define SWE_SORT_WINDOW 64
+#endif
+constexpr int kSortWindow = SWE_SORT_WINDOW;   // rows sorted by length inside windows of this size
 
 enum SmbKind : int { SMB_ROUNDOFF = 0, SMB_2 = 1, SMB_1 = 2, SMB_GALERKIN = 3 };
 enum SrcKind : int { SRC_NONE = 0, SRC_MANNING = 1, SRC_LINEAR = 2, SRC_CORIOLIS = 3 };
 
+// Entry storage: sliced ELL with 64-row slices.  Row r (internal numbering) owns entries
+// sptr[r/64] + (r%64) + 64*q for q < len[r], in the row's original CSR order, so a wave's 64
+// lanes read 64 consecutive entries per step of q.
 struct Mesh {
-    const int* __restrict__ rowptr;   // 0-based, np+1
-    const int* __restrict__ col;      // 0-based, nnz
-    const double2* __restrict__ c;    // (cx, cy) per entry
+    const int* __restrict__ len;      // entries of each row (0 for the padding rows of the last slice)
+    const int64_t* __restrict__ sptr; // first entry of each slice
+    const int* __restrict__ col;      // internal column index of each entry
+    const double2* __restrict__ c;    // (cx, cy) of each entry
+    const double2* __restrict__ n;    // c_ij/|c_ij| of each entry
+    const double* __restrict__ nrm;   // |c_ij| of each entry
     const double* __restrict__ m;     // lumped mass
     int np;
 };
+
+__device__ __forceinline__ int64_t row_base(const Mesh& M, int i) {
+    return M.sptr[i >> 6] + (i & 63);
+}
 
 __device__ __forceinline__ double read_dt(const double* dtp) { return *dtp; }
 
@@ -101,13 +115,14 @@ k_compute_dt(Mesh M, const Node* __restrict__ U, double* __restrict__ d0, Consts
         const double ovh_i = ovh_of(I.h, K.htiny);
         const double sql = sqrt(K.g * fabs(I.h));
         double s = 0.0;
-        const int pe = M.rowptr[i + 1];
-        for (int p = M.rowptr[i]; p < pe; ++p) {
+        const int64_t pb = row_base(M, i);
+        const int len = M.len[i];
+        for (int q = 0; q < len; ++q) {
+            const int64_t p = pb + 64 * (int64_t)q;
             const int j = M.col[p];
             if (j == i) continue;
             const Node J = U[j];
-            const double2 c = M.c[p];
-            const double d = dij_of(c.x, c.y, I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
+            const double d = dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
             if (d0) d0[p] = d;
             s = s + d;
         }
@@ -151,13 +166,15 @@ k_dij_full(Mesh M, const Node* __restrict__ U, double* __restrict__ dij, Consts 
     const double ovh_i = ovh_of(I.h, K.htiny);
     const double sql = sqrt(K.g * fabs(I.h));
     double s = 0.0;
-    int pd = -1;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
+    int64_t pd = -1;
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
         if (j == i) { pd = p; dij[p] = 0.0; continue; }
         const Node J = U[j];
-        const double2 c = M.c[p];
-        const double d = dij_of(c.x, c.y, I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
+        const double d = dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
         dij[p] = d;
         s = s + d;
     }
@@ -176,8 +193,10 @@ k_alpha(Mesh M, const Node* __restrict__ U, double* __restrict__ alpha, Consts K
     double a = 1.0;
     if (!(hi <= K.htiny)) {
         double num = 0.0, denom = 0.0;
-        for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
-            const double hj = U[M.col[p]].h;
+        const int64_t pb = row_base(M, i);
+        const int len = M.len[i];
+        for (int q = 0; q < len; ++q) {
+            const double hj = U[M.col[pb + 64 * (int64_t)q]].h;
             num = num + hj - hi;
             denom = denom + fabs(hj - hi);
         }
@@ -198,7 +217,7 @@ __device__ __forceinline__ void first_order_dmu(const Mesh& M, int p, const doub
                                                 const double* __restrict__ d0, double ai,
                                                 const double* __restrict__ alpha, int j,
                                                 const Consts& K, int& err, double& d, double& mu) {
-    d = d0 ? d0[p] : dij_of(c.x, c.y, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
+    d = d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
     mu = mu_of(c.x, c.y, v1i, v2i, v1j, v2j);
     d = fmaxF(d, mu);
     if constexpr (ALPHA) {
@@ -228,7 +247,10 @@ k_stage_lo(Mesh M, const Node* __restrict__ U, const Node* __restrict__ Uo, Node
     const double ai = ALPHA ? alpha[i] : 1.0;
     const double thr = (SMB == SMB_ROUNDOFF || SMB == SMB_1) ? K.htiny : 0.0;
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
         const Node J = U[j];
         const double2 c = M.c[p];
@@ -351,8 +373,10 @@ k_fct_lo(Mesh M, const Node* __restrict__ U, Node* __restrict__ ulo, double* __r
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;   // smb_2_roundoff (or smb_2)
     double e1 = 0.0, e2 = 0.0, e3 = 0.0;              // smb_2 of entropy_residual
     double emax = 0.0, emin = 0.0;
-    const int p0 = M.rowptr[i], pe = M.rowptr[i + 1];
-    for (int p = p0; p < pe; ++p) {
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
         const Node J = U[j];
         const double2 c = M.c[p];
@@ -421,7 +445,7 @@ k_fct_lo(Mesh M, const Node* __restrict__ U, Node* __restrict__ ulo, double* __r
         Ej = Ej + v1j * J.qx;
         Ej = Ej + v2j * J.qy;
         Ej = 0.5 * Ej;
-        if (p == p0) { emax = Ej; emin = Ej; }
+        if (q == 0) { emax = Ej; emin = Ej; }
         else {
             emax = (Ej > emax) ? Ej : emax;
             emin = (Ej < emin) ? Ej : emin;
@@ -500,7 +524,10 @@ k_fct_rminus(Mesh M, const Node* __restrict__ U, const Node* __restrict__ ulo,
     const double scaled = (M.m[i] / dt) * ulo[i].h;
     const double Q = 0.0 - scaled;
     double P = -1.0e-15;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
         if (j == i) continue;    // f_ii is a zero, never < 0
         const Node J = U[j];
@@ -529,7 +556,10 @@ k_fct_apply(Mesh M, const Node* __restrict__ U, const Node* __restrict__ Uo, con
     const double resi = res[i];
     const double Ri = Rm[i];
     double x1 = 0.0, x2 = 0.0, x3 = 0.0;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
         if (j == i) continue;    // contributes d*(0) + mu*(0)
         const Node J = U[j];
@@ -581,7 +611,10 @@ k_ev_update(Mesh M, const Node* __restrict__ U, const Node* __restrict__ Uo, Nod
     const double resi = res[i];
     const double thr = roundoff ? K.htiny : 0.0;
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
         const Node J = U[j];
         const double2 c = M.c[p];
@@ -671,24 +704,45 @@ __global__ void k_bdy(Node* __restrict__ u, const int* __restrict__ hl, int nh, 
     if (q < nuy) u[uyl[q]].qy = (kind == SWE_BC_VALUES) ? uyv[q] : 0.0;
 }
 
-__global__ void k_pack(const double* __restrict__ un3, const double* __restrict__ z, Node* __restrict__ u, int np) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= np) return;
+// un(3,np) in the caller's numbering <-> internal Node records; perm maps internal -> caller.
+__global__ void k_pack(const double* __restrict__ un3, const double* __restrict__ z, const int* __restrict__ perm,
+                       Node* __restrict__ u, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np) return;
+    const size_t i = (size_t)perm[r];
     Node o;
-    o.h = un3[3 * (size_t)i];
-    o.qx = un3[3 * (size_t)i + 1];
-    o.qy = un3[3 * (size_t)i + 2];
-    o.z = z[i];
-    u[i] = o;
+    o.h = un3[3 * i];
+    o.qx = un3[3 * i + 1];
+    o.qy = un3[3 * i + 2];
+    o.z = z[r];
+    u[r] = o;
 }
 
-__global__ void k_unpack(const Node* __restrict__ u, double* __restrict__ un3, int np) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= np) return;
-    const Node o = u[i];
-    un3[3 * (size_t)i] = o.h;
-    un3[3 * (size_t)i + 1] = o.qx;
-    un3[3 * (size_t)i + 2] = o.qy;
+__global__ void k_unpack(const Node* __restrict__ u, const int* __restrict__ perm, double* __restrict__ un3, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np) return;
+    const size_t i = (size_t)perm[r];
+    const Node o = u[r];
+    un3[3 * i] = o.h;
+    un3[3 * i + 1] = o.qx;
+    un3[3 * i + 2] = o.qy;
+}
+
+// node array in the caller's numbering -> internal numbering
+__global__ void k_perm_nodes(const double* __restrict__ src, const int* __restrict__ perm, double* __restrict__ dst, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r < np) dst[r] = src[perm[r]];
+}
+
+// entry array in CSR order <-> sliced layout, through the CSR -> slot map
+__global__ void k_csr_to_sell(const double* __restrict__ src, const int64_t* __restrict__ map, double* __restrict__ dst, int64_t nnz) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) dst[map[p]] = src[p];
+}
+
+__global__ void k_sell_to_csr(const double* __restrict__ src, const int64_t* __restrict__ map, double* __restrict__ dst, int64_t nnz) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) dst[p] = src[map[p]];
 }
 
 // FCT_positivity on arbitrary fluxes (swe_fct_positivity)
@@ -698,7 +752,10 @@ __global__ void k_fctpos_r(Mesh M, const double* __restrict__ f, const double* _
     if (i >= M.np) return;
     const double Q = 0.0 - us[i];
     double P = -1.0e-15;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p) {
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         const double fij = f[p];
         if (fij < 0.0) P = P + fij;
     }
@@ -711,8 +768,12 @@ __global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* _
                            double* __restrict__ lij) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= M.np) return;
-    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; ++p)
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
         lij[p] = (f[p] >= 0.0) ? Rm[M.col[p]] : Rm[i];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -756,10 +817,17 @@ struct swe_handle {
     int np = 0, nnz = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    // mesh
-    int* rowptr = nullptr;
+    // mesh in the internal numbering and the sliced entry layout
+    int np_pad = 0;                 // np rounded up to whole 64-row slices
+    int64_t nslot = 0;              // entry slots of the sliced layout (>= nnz)
+    int* len = nullptr;
+    int64_t* sptr = nullptr;
+    int* perm = nullptr;            // internal row -> caller's node
+    int64_t* csr2slot = nullptr;    // caller's CSR entry -> slot
     int* col = nullptr;
     double2* c = nullptr;
+    double2* nvec = nullptr;        // c_ij/|c_ij|
+    double* nrm = nullptr;          // |c_ij|
     double* m = nullptr;
     double* z = nullptr;
     // states: S[0] holds U^n between steps
@@ -797,7 +865,7 @@ struct swe_handle {
 
 namespace {
 
-Mesh mesh_of(const swe_handle* h) { return Mesh{h->rowptr, h->col, h->c, h->m, h->np}; }
+Mesh mesh_of(const swe_handle* h) { return Mesh{h->len, h->sptr, h->col, h->c, h->nvec, h->nrm, h->m, h->np}; }
 
 int make_consts(const swe_params& p, Consts& K) {
     if (!(p.gravity > 0.0)) return fail(SWE_EINVAL, "gravity must be > 0");
@@ -1004,12 +1072,12 @@ int ensure_dt_log(swe_handle* h, int n) {
 int upload_state(swe_handle* h, const double* un, Node* dst) {
     HIPCHK(hipMemcpyAsync(h->un3, un, sizeof(double) * 3 * (size_t)h->np, hipMemcpyHostToDevice, h->stream));
     hipLaunchKernelGGL(k_pack, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, (const double*)h->un3,
-                       (const double*)h->z, dst, h->np);
+                       (const double*)h->z, (const int*)h->perm, dst, h->np);
     return SWE_OK;
 }
 
 int download_state(swe_handle* h, const Node* src, double* un) {
-    hipLaunchKernelGGL(k_unpack, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, src, h->un3, h->np);
+    hipLaunchKernelGGL(k_unpack, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, src, (const int*)h->perm, h->un3, h->np);
     HIPCHK(hipMemcpyAsync(un, h->un3, sizeof(double) * 3 * (size_t)h->np, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SWE_OK;
@@ -1075,7 +1143,7 @@ int queue_steps(swe_handle* h, int nsteps, int* first) {
 
 void free_handle(swe_handle* h) {
     if (!h) return;
-    void* bufs[] = {h->rowptr, h->col, h->c, h->m, h->z, h->S[0], h->S[1], h->S[2], h->scratch, h->ulo, h->un3,
+    void* bufs[] = {h->len, h->sptr, h->perm, h->csr2slot, h->col, h->nvec, h->nrm, h->c, h->m, h->z, h->S[0], h->S[1], h->S[2], h->scratch, h->ulo, h->un3,
                     h->d0, h->dL, h->muL, h->ework, h->alpha, h->res, h->Rm, h->nwork, h->vmax, h->dtp,
                     h->dt_log, h->flags, h->hl, h->uxl, h->uyl, h->hv, h->uxv, h->uyv};
     for (void* b : bufs)
@@ -1141,20 +1209,74 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
     if (e != hipSuccess) { delete h; return fail(SWE_EHIP, "hipGetDevice: %s", hipGetErrorString(e)); }
     e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete h; return fail(SWE_EHIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+    // Internal numbering: inside consecutive windows of kSortWindow rows of the caller's numbering,
+    // rows are stably sorted by length, so the 64 rows of a slice (one wave) have equal length.
+    // Only the processing order changes; each row keeps its entries in the caller's order.
+    std::vector<int> perm((size_t)np), iperm((size_t)np);
+    for (int w0 = 0; w0 < np; w0 += kSortWindow) {
+        const int w1 = std::min(np, w0 + kSortWindow);
+        for (int i = w0; i < w1; ++i) perm[(size_t)i] = i;
+        std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int a, int b) {
+            return (rp[(size_t)a + 1] - rp[(size_t)a]) < (rp[(size_t)b + 1] - rp[(size_t)b]);
+        });
+    }
+    for (int r = 0; r < np; ++r) iperm[(size_t)perm[(size_t)r]] = r;
+    const int np_pad = (np + 63) / 64 * 64;
+    const int nsl = np_pad / 64;
+    std::vector<int> len((size_t)np_pad, 0);
+    std::vector<int64_t> sptr((size_t)nsl + 1, 0);
+    for (int r = 0; r < np; ++r) len[(size_t)r] = rp[(size_t)perm[(size_t)r] + 1] - rp[(size_t)perm[(size_t)r]];
+    for (int sl = 0; sl < nsl; ++sl) {
+        int w = 0;
+        for (int l = 0; l < 64; ++l) w = std::max(w, len[(size_t)sl * 64 + l]);
+        sptr[(size_t)sl + 1] = sptr[(size_t)sl] + 64 * (int64_t)w;
+    }
+    const int64_t nslot = sptr[(size_t)nsl];
+    std::vector<int> col((size_t)nslot);
+    std::vector<double2> cc((size_t)nslot, make_double2(0.0, 0.0));
+    std::vector<double2> nv((size_t)nslot, make_double2(0.0, 0.0));
+    std::vector<double> nr((size_t)nslot, 0.0);
+    std::vector<int64_t> csr2slot((size_t)nnz);
+    std::vector<double> m_int((size_t)np), z_int((size_t)np);
+    for (int64_t q = 0; q < nslot; ++q) col[(size_t)q] = 0;
+    for (int r = 0; r < np; ++r) {
+        const int i = perm[(size_t)r];
+        const int64_t base = sptr[(size_t)(r >> 6)] + (r & 63);
+        for (int q = 0; q < len[(size_t)r]; ++q) {
+            const int p = rp[(size_t)i] + q;
+            const int64_t slot = base + 64 * (int64_t)q;
+            col[(size_t)slot] = iperm[(size_t)cl[(size_t)p]];
+            cc[(size_t)slot] = make_double2(d->cx[p], d->cy[p]);
+            // compute_dij's normalisation (SOURCES/update.f90:306-310), same expressions and order
+            const double norm = std::sqrt(0.0 + d->cx[p] * d->cx[p] + d->cy[p] * d->cy[p]);
+            nr[(size_t)slot] = norm;
+            nv[(size_t)slot] = make_double2(d->cx[p] / norm, d->cy[p] / norm);
+            csr2slot[(size_t)p] = slot;
+        }
+        m_int[(size_t)r] = d->lumped[i];
+        z_int[(size_t)r] = d->bath[i];
+    }
+    h->np_pad = np_pad;
+    h->nslot = nslot;
 #define ALLOC(ptr, n) do { if ((rc = dalloc(&(ptr), (n)))) { free_handle(h); return rc; } } while (0)
-    ALLOC(h->rowptr, (size_t)np + 1);
-    ALLOC(h->col, (size_t)nnz);
-    ALLOC(h->c, (size_t)nnz);
+    ALLOC(h->len, (size_t)np_pad);
+    ALLOC(h->sptr, (size_t)nsl + 1);
+    ALLOC(h->perm, (size_t)np);
+    ALLOC(h->csr2slot, (size_t)nnz);
+    ALLOC(h->col, (size_t)nslot);
+    ALLOC(h->c, (size_t)nslot);
+    ALLOC(h->nvec, (size_t)nslot);
+    ALLOC(h->nrm, (size_t)nslot);
     ALLOC(h->m, (size_t)np);
     ALLOC(h->z, (size_t)np);
     for (int k = 0; k < 3; ++k) ALLOC(h->S[k], (size_t)np);
     ALLOC(h->scratch, (size_t)np);
     ALLOC(h->ulo, (size_t)np);
     ALLOC(h->un3, 3 * (size_t)np);
-    ALLOC(h->d0, (size_t)nnz);
-    ALLOC(h->dL, (size_t)nnz);
-    ALLOC(h->muL, (size_t)nnz);
-    ALLOC(h->ework, (size_t)nnz);
+    ALLOC(h->d0, (size_t)nslot);
+    ALLOC(h->dL, (size_t)nslot);
+    ALLOC(h->muL, (size_t)nslot);
+    ALLOC(h->ework, (size_t)std::max<int64_t>(nslot, nnz));
     ALLOC(h->alpha, (size_t)np);
     ALLOC(h->res, (size_t)np);
     ALLOC(h->Rm, (size_t)np);
@@ -1173,14 +1295,13 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
         free_handle(h);
         return fail(SWE_ENOMEM, "hipHostMalloc failed");
     }
-    std::vector<double2> cc((size_t)nnz);
-    for (int p = 0; p < nnz; ++p) cc[(size_t)p] = make_double2(d->cx[p], d->cy[p]);
+    // Dirichlet lists in the internal numbering; values given later follow the caller's list order
     std::vector<int> lists[3];
     const int* src[3] = {d->h_dir, d->ux_dir, d->uy_dir};
     const int ns[3] = {d->n_h_dir, d->n_ux_dir, d->n_uy_dir};
     for (int k = 0; k < 3; ++k) {
         lists[k].resize((size_t)ns[k]);
-        for (int q = 0; q < ns[k]; ++q) lists[k][(size_t)q] = src[k][q] - 1;
+        for (int q = 0; q < ns[k]; ++q) lists[k][(size_t)q] = iperm[(size_t)src[k][q] - 1];
     }
     h->nh = ns[0];
     h->nux = ns[1];
@@ -1191,9 +1312,13 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
         if (e2 != hipSuccess) return fail(SWE_EHIP, "hipMemcpy: %s", hipGetErrorString(e2));
         return SWE_OK;
     };
-    if ((rc = up(h->rowptr, rp.data(), sizeof(int) * rp.size())) || (rc = up(h->col, cl.data(), sizeof(int) * cl.size())) ||
-        (rc = up(h->c, cc.data(), sizeof(double2) * cc.size())) || (rc = up(h->m, d->lumped, sizeof(double) * (size_t)np)) ||
-        (rc = up(h->z, d->bath, sizeof(double) * (size_t)np)) || (rc = up(h->hl, lists[0].data(), sizeof(int) * lists[0].size())) ||
+    if ((rc = up(h->len, len.data(), sizeof(int) * len.size())) || (rc = up(h->sptr, sptr.data(), sizeof(int64_t) * sptr.size())) ||
+        (rc = up(h->perm, perm.data(), sizeof(int) * perm.size())) ||
+        (rc = up(h->csr2slot, csr2slot.data(), sizeof(int64_t) * csr2slot.size())) ||
+        (rc = up(h->col, col.data(), sizeof(int) * col.size())) || (rc = up(h->c, cc.data(), sizeof(double2) * cc.size())) ||
+        (rc = up(h->nvec, nv.data(), sizeof(double2) * nv.size())) || (rc = up(h->nrm, nr.data(), sizeof(double) * nr.size())) ||
+        (rc = up(h->m, m_int.data(), sizeof(double) * (size_t)np)) || (rc = up(h->z, z_int.data(), sizeof(double) * (size_t)np)) ||
+        (rc = up(h->hl, lists[0].data(), sizeof(int) * lists[0].size())) ||
         (rc = up(h->uxl, lists[1].data(), sizeof(int) * lists[1].size())) ||
         (rc = up(h->uyl, lists[2].data(), sizeof(int) * lists[2].size()))) {
         free_handle(h);
@@ -1264,7 +1389,9 @@ int swe_compute_dij(swe_handle* h, const double* un, double* dij) {
     int rc = upload_state(h, un, h->scratch);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dij_full, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, mesh_of(h),
-                       (const Node*)h->scratch, h->ework, h->K, h->flags);
+                       (const Node*)h->scratch, h->dL, h->K, h->flags);
+    hipLaunchKernelGGL(k_sell_to_csr, dim3((unsigned)((h->nnz + 255) / 256)), dim3(256), 0, h->stream,
+                       (const double*)h->dL, (const int64_t*)h->csr2slot, h->ework, (int64_t)h->nnz);
     HIPCHK(hipMemcpyAsync(dij, h->ework, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost, h->stream));
     return check_flags(h);
 }
@@ -1302,14 +1429,21 @@ int swe_euler(swe_handle* h, const double* un, double* unext, double dt) {
 
 int swe_fct_positivity(swe_handle* h, const double* us, const double* f, double* lij) {
     if (!h || !us || !f || !lij) return fail(SWE_EINVAL, "null argument");
-    HIPCHK(hipMemcpyAsync(h->nwork, us, sizeof(double) * (size_t)h->np, hipMemcpyHostToDevice, h->stream));
+    const unsigned ge = (unsigned)((h->nnz + 255) / 256);
+    HIPCHK(hipMemcpyAsync(h->un3, us, sizeof(double) * (size_t)h->np, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->ework, f, sizeof(double) * (size_t)h->nnz, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_perm_nodes, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, (const double*)h->un3,
+                       (const int*)h->perm, h->nwork, h->np);
+    hipLaunchKernelGGL(k_csr_to_sell, dim3(ge), dim3(256), 0, h->stream, (const double*)h->ework,
+                       (const int64_t*)h->csr2slot, h->d0, (int64_t)h->nnz);
     const Mesh M = mesh_of(h);
-    hipLaunchKernelGGL(k_fctpos_r, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, M, (const double*)h->ework,
+    hipLaunchKernelGGL(k_fctpos_r, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, M, (const double*)h->d0,
                        (const double*)h->nwork, h->Rm, h->K.htiny);
-    hipLaunchKernelGGL(k_fctpos_l, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, M, (const double*)h->ework,
+    hipLaunchKernelGGL(k_fctpos_l, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, M, (const double*)h->d0,
                        (const double*)h->Rm, h->dL);
-    HIPCHK(hipMemcpyAsync(lij, h->dL, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost, h->stream));
+    hipLaunchKernelGGL(k_sell_to_csr, dim3(ge), dim3(256), 0, h->stream, (const double*)h->dL,
+                       (const int64_t*)h->csr2slot, h->ework, (int64_t)h->nnz);
+    HIPCHK(hipMemcpyAsync(lij, h->ework, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SWE_OK;
 }

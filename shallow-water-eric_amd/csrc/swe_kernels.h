@@ -45,23 +45,22 @@ __device__ __forceinline__ double ovh_of(double h, double htiny) {
     return 2.0 * h / (h * h + m * m);
 }
 
-// phi(h, ul, hl, ur, hr); ovhl and ovhr are compute_one_over_h of hl and hr.
+// phi(h, ul, hl, ur, hr); ovhl, ovhr are compute_one_over_h of hl, hr and sql, sqr are
+// SQRT(g*ABS(hl)), SQRT(g*ABS(hr)).  Each side takes one square root of the argument its branch
+// uses, so a wave never runs both branches: fl = (h-hl)*SQRT((g/2)*(ovh+ovhl)) if h > hl, else
+// 2*(SQRT(g*h) - SQRT(g*hl)), where SQRT(g*hl) is sql for hl >= 0 (NaN otherwise, as in the reference).
 __device__ __forceinline__ double phi_f(double h, double ul, double hl, double ur, double hr,
-                                        double ovhl, double ovhr, const Consts& K) {
-    double fl, fr;
-    double ovh = 0.0;
+                                        double ovhl, double ovhr, double sql, double sqr, const Consts& K) {
     const bool gl = h > hl, gr = h > hr;
-    if (gl || gr) ovh = ovh_of(h, K.htiny);
-    if (gl) {
-        fl = (h - hl) * sqrt(K.g_half * (ovh + ovhl));
-    } else {
-        fl = 2.0 * (sqrt(K.g * h) - sqrt(K.g * hl));
-    }
-    if (gr) {
-        fr = (h - hr) * sqrt(K.g_half * (ovh + ovhr));
-    } else {
-        fr = 2.0 * (sqrt(K.g * h) - sqrt(K.g * hr));
-    }
+    const double ovh = ovh_of(h, K.htiny);
+    const double gh = K.g * h;
+    const double Sl = sqrt(gl ? K.g_half * (ovh + ovhl) : gh);
+    const double Sr = sqrt(gr ? K.g_half * (ovh + ovhr) : gh);
+    const double nan = __builtin_nan("");
+    const double sgl = (hl >= 0.0) ? sql : nan;
+    const double sgr = (hr >= 0.0) ? sqr : nan;
+    const double fl = gl ? (h - hl) * Sl : 2.0 * (Sl - sgl);
+    const double fr = gr ? (h - hr) * Sr : 2.0 * (Sr - sgr);
     return fl + fr + ur - ul;
 }
 
@@ -78,15 +77,15 @@ __device__ __forceinline__ double lambda_vacc(double hl, double qxl, double qyl,
     if (hl <= hr) { hmin = hl; hmax = hr; }
     else { hmin = hr; hmax = hl; }
     double ht;
-    double fh = phi_f(K.x0 * hmin, vl, hl, vr, hr, ovhl, ovhr, K);
+    const double fh = phi_f(K.x0 * hmin, vl, hl, vr, hr, ovhl, ovhr, sql, sqr, K);
     if (0.0 <= fh) {
         const double t = fmaxF(vl - vr + 2.0 * sql + 2.0 * sqr, 0.0);
         ht = fminF(K.x0 * hmin, t * t / K.g16);
     } else {
-        fh = phi_f(K.x0 * hmax, vl, hl, vr, hr, ovhl, ovhr, K);
+        const double fh2 = phi_f(K.x0 * hmax, vl, hl, vr, hr, ovhl, ovhr, sql, sqr, K);
         const double sqrmin = sqrt(hmin);
         const double sqrmax = sqrt(hmax);
-        if (0.0 <= fh) {
+        if (0.0 <= fh2) {
             const double cc = -hmin * K.a_c2 - sqrmin * sqrmax + sqrmin * (vr - vl) / K.two_sqrt_g;
             const double Delta = hmin - 4.0 * K.a_c2 * cc;
             if (Delta < 0.0) {
@@ -107,14 +106,12 @@ __device__ __forceinline__ double lambda_vacc(double hl, double qxl, double qyl,
 }
 
 // First-order graph viscosity of one off-diagonal entry: |c_ij| * lambda(U_i, U_j, c_ij/|c_ij|)
-// (compute_dij, SOURCES/update.f90:305-315).
-__device__ __forceinline__ double dij_of(double cx, double cy, const Node& I, const Node& J,
+// (compute_dij, SOURCES/update.f90:305-315).  norm = SQRT(SUM(c_ij**2)) and n = c_ij/norm depend
+// only on the mesh and are evaluated once on the host with the same expressions.
+__device__ __forceinline__ double dij_of(double2 n, double norm, const Node& I, const Node& J,
                                          double ovh_i, double ovh_j, double sql, double sqr,
                                          const Consts& K, int& err) {
-    const double norm = sqrt(0.0 + cx * cx + cy * cy);
-    const double n1 = cx / norm;
-    const double n2 = cy / norm;
-    const double lam = lambda_vacc(I.h, I.qx, I.qy, J.h, J.qx, J.qy, n1, n2, ovh_i, ovh_j, sql, sqr, K, err);
+    const double lam = lambda_vacc(I.h, I.qx, I.qy, J.h, J.qx, J.qy, n.x, n.y, ovh_i, ovh_j, sql, sqr, K, err);
     return norm * lam;
 }
 

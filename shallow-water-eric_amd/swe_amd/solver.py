@@ -74,11 +74,13 @@ SIGNATURES = {
 _lib = None
 
 
-def load_library(path: str = LIB_PATH):
-    """Load libswe_amd.so and bind every declared symbol (raises if the library is missing)."""
+def load_library(path: str | None = None):
+    """Load libswe_amd.so and bind every declared symbol (raises if the library is missing).
+    SWE_AMD_LIB names another build of the library to load instead of the in-tree one."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("SWE_AMD_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise SweError(SWE_EHIP, "libswe_amd.so is not built at %s (run __graft_entry__.build())" % path)
     lib = C.CDLL(path)

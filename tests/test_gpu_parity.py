@@ -173,10 +173,11 @@ def test_bumps_inflow_66k_nodes_bitwise(oracle):
 
 
 def test_negative_depth_is_reported():
-    """check_hmin: an FCT stage whose low-order depth goes negative returns SWE_ENEG_H."""
+    """check_hmin: the entropy-viscosity stage the reference stops on (Min h<0) returns SWE_ENEG_H."""
     from swe_amd.solver import SWE_ENEG_H, SweError
     z = golden("par41_modes.npz")
-    with solver_from(z, "fct", True, 5) as s:
+    assert z["status"][0, list(z["modes"]).index("entropy_visc|1|type2")] == 1
+    with solver_from(z, "entropy_visc", True, 5) as s:
         s.set_state(z["u0"])
         with pytest.raises(SweError) as e:
             s.run(1, time=0.0)
