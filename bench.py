@@ -45,6 +45,25 @@ def step_bytes_per_node_stage(k, fct=True, alpha=True):
     return (124 + 30.67 * k) if alpha else (96 + 26.67 * k)
 
 
+def measured_traffic(kernel, workload, nodes):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated
+    against a plain device copy) for this workload and mesh size, from the committed summaries in
+    profiles/, or None when no summary covers it."""
+    pdir = os.path.join(REPO, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".txt"):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 5 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel
+                        and parts[2] == workload and int(parts[3]) == nodes):
+                    return float(parts[4])
+    return None
+
+
 def build_problem(name, size=None):
     from swe_amd import problems
     if name == "bumps":
@@ -87,6 +106,27 @@ def cpu_baseline(name, seconds=15.0):
                       % (name, p.np, n, el)}
 
 
+def copy_calibration(gib=1, reps=3):
+    """A plain device-to-device copy of a known size through the HIP runtime, to calibrate the
+    FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes `gib` GiB)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(gib << 30)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), n) != 0:
+            raise SystemExit("hipMalloc failed")
+    if hip.hipMemset(a, 1, n) != 0:
+        raise SystemExit("hipMemset failed")
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, n, 3) != 0:          # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copied %d x %d GiB device to device" % (reps, gib))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,7 +136,12 @@ def main():
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 1 GiB device copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding of a single mesh is a design note "
               "only (see DESIGN.md), so --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -146,8 +191,8 @@ def main():
         "config": {"workload": p.name, "nodes": p.np, "entries": p.nnz, "viscosity": p.params["viscosity"],
                    "alpha_limit": bool(alpha), "steps": args.steps, "hmin_end": hmin},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_ms": avg_s * 1e3,
-                     "bytes_per_node": bpn},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
+                     "launch_ms": avg_s * 1e3, "bytes_per_node": bpn},
         "step_effective_GBs": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9,
     }
     s.close()

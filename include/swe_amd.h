@@ -113,6 +113,11 @@ int swe_euler(swe_handle *h, const double *un, double *unext, double dt);
  * (lumped/dt)*h of the low-order solution, f(nnz) the antidiffusive fluxes, lij(nnz) out. */
 int swe_fct_positivity(swe_handle *h, const double *unext_scaled, const double *f, double *lij);
 
+/* FCT_positivity (SOURCES/fct.f90:5-42) on any 1-based CSR matrix ia(np+1), ja(nnz) without a
+ * solver handle: the MODULE fct entry point of the Fortran binding. */
+int swe_fct_positivity_csr(int np, int nnz, const int *ia, const int *ja, const double *unext_scaled,
+                           const double *f, double htiny, double *lij);
+
 /* Device-resident state. */
 int swe_set_state(swe_handle *h, const double *un);
 int swe_get_state(swe_handle *h, double *un);

@@ -59,8 +59,53 @@ struct Mesh {
     const double2* __restrict__ n;    // c_ij/|c_ij| of each entry
     const double* __restrict__ nrm;   // |c_ij| of each entry
     const double* __restrict__ m;     // lumped mass
+    const uint16_t* __restrict__ elist; // per slice: slot offsets of its off-diagonal entries
+    const int64_t* __restrict__ eoff;   // first elist entry of each slice
+    const int* __restrict__ dpos;       // position q of the diagonal in each row
     int np;
+    int wmax;                           // longest row (slots per lane in a slice)
 };
+
+// LDS of one wave for the compacted lambda phase: per-row one_over_h and SQRT(g*ABS(h)), then
+// one double per slot of the slice (slot offset q*64 + lane).
+__device__ __forceinline__ double* wave_lds(double* smem, int wmax) {
+    return smem + (size_t)(threadIdx.x >> 6) * (size_t)(2 + wmax) * 64;
+}
+
+// Phase 0 (one lane per row): row quantities into LDS, and an exact 0.0 in the diagonal slot,
+// which is what the reference's row sums add at that position.
+__device__ __forceinline__ void lambda_rows(const Mesh& M, const Node& I, bool valid, int i, const Consts& K,
+                                            double* lw) {
+    const int lane = (int)(threadIdx.x & 63);
+    if (valid) {
+        lw[lane] = ovh_of(I.h, K.htiny);
+        lw[64 + lane] = sqrt(K.g * fabs(I.h));
+        lw[128 + 64 * M.dpos[i] + lane] = 0.0;
+    }
+}
+
+// Phase 1 (entry-parallel): d_ij = |c_ij| lambda_ij of every off-diagonal entry of the wave's
+// slice, taken from the slice's compact entry list so that all 64 lanes work on real entries.
+// Results go to LDS (and to d_out when given).
+__device__ __forceinline__ void lambda_entries(const Mesh& M, const Node* __restrict__ U, const Consts& K,
+                                               double* lw, double* __restrict__ d_out, int& err) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int slice = (int)(((int64_t)xcd_block() * kBlock + (threadIdx.x & ~63u)) >> 6);
+    if (slice * 64 >= M.np) return;
+    const int64_t e1 = M.eoff[slice + 1];
+    const int64_t sb = M.sptr[slice];
+    for (int64_t e = M.eoff[slice] + lane; e < e1; e += 64) {
+        const int off = M.elist[e];
+        const int r = off & 63;
+        const int64_t slot = sb + off;
+        const Node I = U[slice * 64 + r];
+        const Node J = U[M.col[slot]];
+        const double d = dij_of(M.n[slot], M.nrm[slot], I, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
+                                sqrt(K.g * fabs(J.h)), K, err);
+        lw[128 + off] = d;
+        if (d_out) d_out[slot] = d;
+    }
+}
 
 __device__ __forceinline__ int64_t row_base(const Mesh& M, int i) {
     return M.sptr[i >> 6] + (i & 63);
@@ -107,25 +152,24 @@ __device__ __forceinline__ void add_sources(const Node& I, double mi, double v1,
 __global__ void __launch_bounds__(kBlock)
 k_compute_dt(Mesh M, const Node* __restrict__ U, double* __restrict__ d0, Consts K,
              unsigned long long* __restrict__ vmax, int* __restrict__ flags) {
+    extern __shared__ double smem[];
+    double* lw = wave_lds(smem, M.wmax);
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
-    double v = 0.0;
+    const bool valid = i < M.np;
+    const int lane = (int)(threadIdx.x & 63);
     int err = 0;
-    if (i < M.np) {
-        const Node I = U[i];
-        const double ovh_i = ovh_of(I.h, K.htiny);
-        const double sql = sqrt(K.g * fabs(I.h));
+    Node I{};
+    if (valid) I = U[i];
+    lambda_rows(M, I, valid, i, K, lw);
+    __syncthreads();
+    lambda_entries(M, U, K, lw, d0, err);
+    __syncthreads();
+    double v = 0.0;
+    if (valid) {
+        // -SUM(dij%aa(ia(i):ia(i+1)-1)) with the diagonal slot holding 0.0, in entry order
         double s = 0.0;
-        const int64_t pb = row_base(M, i);
         const int len = M.len[i];
-        for (int q = 0; q < len; ++q) {
-            const int64_t p = pb + 64 * (int64_t)q;
-            const int j = M.col[p];
-            if (j == i) continue;
-            const Node J = U[j];
-            const double d = dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
-            if (d0) d0[p] = d;
-            s = s + d;
-        }
+        for (int q = 0; q < len; ++q) s = s + lw[128 + 64 * q + lane];
         v = fabs(-s) / M.m[i];
     }
     raise_flag(flags, err);
@@ -192,10 +236,25 @@ k_alpha(Mesh M, const Node* __restrict__ U, double* __restrict__ alpha, Consts K
     const double hi = U[i].h;
     double a = 1.0;
     if (!(hi <= K.htiny)) {
-        double num = 0.0, denom = 0.0;
         const int64_t pb = row_base(M, i);
         const int len = M.len[i];
-        for (int q = 0; q < len; ++q) {
+        // all index loads, then all gathers, then the sums in entry order
+        constexpr int kUnroll = 12;
+        int js[kUnroll];
+        double hs[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) js[q] = (q < len) ? M.col[pb + 64 * q] : i;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) hs[q] = U[js[q]].h;
+        double num = 0.0, denom = 0.0;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            if (q < len) {
+                num = num + hs[q] - hi;
+                denom = denom + fabs(hs[q] - hi);
+            }
+        }
+        for (int q = kUnroll; q < len; ++q) {
             const double hj = U[M.col[pb + 64 * (int64_t)q]].h;
             num = num + hj - hi;
             denom = denom + fabs(hj - hi);
@@ -209,15 +268,12 @@ k_alpha(Mesh M, const Node* __restrict__ U, double* __restrict__ alpha, Consts K
     alpha[i] = a;
 }
 
-// First-order d_ij and mu_ij of an off-diagonal entry: MAX(d,mu), then the alpha scaling.
+// dlam is compute_dij's value for this entry (from the dt pass or from the lambda phase).
 template <bool ALPHA>
-__device__ __forceinline__ void first_order_dmu(const Mesh& M, int p, const double2 c, const Node& I,
-                                                const Node& J, double ovh_i, double ovh_j, double sql,
-                                                double v1i, double v2i, double v1j, double v2j,
-                                                const double* __restrict__ d0, double ai,
-                                                const double* __restrict__ alpha, int j,
-                                                const Consts& K, int& err, double& d, double& mu) {
-    d = d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
+__device__ __forceinline__ void first_order_dmu(const double2 c, double v1i, double v2i, double v1j, double v2j,
+                                                double dlam, double ai, const double* __restrict__ alpha, int j,
+                                                double& d, double& mu) {
+    d = dlam;
     mu = mu_of(c.x, c.y, v1i, v2i, v1j, v2j);
     d = fmaxF(d, mu);
     if constexpr (ALPHA) {
@@ -236,14 +292,15 @@ k_stage_lo(Mesh M, const Node* __restrict__ U, const Node* __restrict__ Uo, Node
            const double* __restrict__ d0, const double* __restrict__ alpha, const double* __restrict__ dtp,
            Consts K, int stage, int* __restrict__ flags) {
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
-    if (i >= M.np) return;
+    const bool valid = i < M.np;
     int err = 0;
-    const double dt = read_dt(dtp);
+    if (!valid) return;
     const Node I = U[i];
+    const double sql = sqrt(K.g * fabs(I.h));
+    const double dt = read_dt(dtp);
     const double mi = M.m[i];
     const double ovh_i = ovh_of(I.h, K.htiny);
     const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
-    const double sql = sqrt(K.g * fabs(I.h));
     const double ai = ALPHA ? alpha[i] : 1.0;
     const double thr = (SMB == SMB_ROUNDOFF || SMB == SMB_1) ? K.htiny : 0.0;
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;
@@ -264,7 +321,9 @@ k_stage_lo(Mesh M, const Node* __restrict__ U, const Node* __restrict__ Uo, Node
         double d = 0.0, mu = 0.0;
         const bool off = (j != i);
         if (SMB != SMB_GALERKIN && off)
-            first_order_dmu<ALPHA>(M, p, c, I, J, ovh_i, ovh_j, sql, v1i, v2i, v1j, v2j, d0, ai, alpha, j, K, err, d, mu);
+            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
+                                   d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   ai, alpha, j, d, mu);
         if constexpr (SMB == SMB_1) {
             // smb_1 (update.f90:378-409)
             double xx = 0.0;
@@ -360,14 +419,15 @@ k_fct_lo(Mesh M, const Node* __restrict__ U, Node* __restrict__ ulo, double* __r
          const double* __restrict__ alpha, const double* __restrict__ dtp, Consts K, int roundoff,
          int* __restrict__ flags) {
     const int i = xcd_block() * kBlock + (int)threadIdx.x;
-    if (i >= M.np) return;
+    const bool valid = i < M.np;
     int err = 0;
-    const double dt = read_dt(dtp);
+    if (!valid) return;
     const Node I = U[i];
+    const double sql = sqrt(K.g * fabs(I.h));
+    const double dt = read_dt(dtp);
     const double mi = M.m[i];
     const double ovh_i = ovh_of(I.h, K.htiny);
     const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
-    const double sql = sqrt(K.g * fabs(I.h));
     const double ai = ALPHA ? alpha[i] : 1.0;
     const double thr_lo = roundoff ? K.htiny : 0.0;
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;   // smb_2_roundoff (or smb_2)
@@ -393,7 +453,9 @@ k_fct_lo(Mesh M, const Node* __restrict__ U, Node* __restrict__ ulo, double* __r
         const bool off = (j != i);
         double d = 0.0, mu = 0.0;
         if (off) {
-            first_order_dmu<ALPHA>(M, p, c, I, J, ovh_i, ovh_j, sql, v1i, v2i, v1j, v2j, d0, ai, alpha, j, K, err, d, mu);
+            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
+                                   d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   ai, alpha, j, d, mu);
             dL[p] = d;
             muL[p] = mu;
         }
@@ -776,6 +838,29 @@ __global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* _
     }
 }
 
+// FCT_positivity on a caller's 1-based CSR matrix (swe_fct_positivity_csr)
+__global__ void k_fctpos_csr_r(int np, const int* __restrict__ ia, const double* __restrict__ f,
+                               const double* __restrict__ us, double* __restrict__ Rm, double htiny) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    const double Q = 0.0 - us[i];
+    double P = -1.0e-15;
+    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) {
+        const double fij = f[p];
+        if (fij < 0.0) P = P + fij;
+    }
+    double R = fminF(Q / P, 1.0);
+    if (us[i] <= htiny) R = 0.0;
+    Rm[i] = R;
+}
+
+__global__ void k_fctpos_csr_l(int np, const int* __restrict__ ia, const int* __restrict__ ja,
+                               const double* __restrict__ f, const double* __restrict__ Rm, double* __restrict__ lij) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) lij[p] = (f[p] >= 0.0) ? Rm[ja[p] - 1] : Rm[i];
+}
+
 // ---------------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------------
@@ -828,6 +913,11 @@ struct swe_handle {
     double2* c = nullptr;
     double2* nvec = nullptr;        // c_ij/|c_ij|
     double* nrm = nullptr;          // |c_ij|
+    uint16_t* elist = nullptr;      // per slice: slot offsets of its off-diagonal entries
+    int64_t* eoff = nullptr;
+    int* dpos = nullptr;            // diagonal position in each row
+    int wmax = 0;                   // longest row
+    size_t lds_bytes = 0;           // dynamic LDS of the kernels with a lambda phase
     double* m = nullptr;
     double* z = nullptr;
     // states: S[0] holds U^n between steps
@@ -865,7 +955,9 @@ struct swe_handle {
 
 namespace {
 
-Mesh mesh_of(const swe_handle* h) { return Mesh{h->len, h->sptr, h->col, h->c, h->nvec, h->nrm, h->m, h->np}; }
+Mesh mesh_of(const swe_handle* h) {
+    return Mesh{h->len, h->sptr, h->col, h->c, h->nvec, h->nrm, h->m, h->elist, h->eoff, h->dpos, h->np, h->wmax};
+}
 
 int make_consts(const swe_params& p, Consts& K) {
     if (!(p.gravity > 0.0)) return fail(SWE_EINVAL, "gravity must be > 0");
@@ -1101,7 +1193,7 @@ int queue_steps(swe_handle* h, int nsteps, int* first) {
     Node* Cc = h->S[2];
     for (int it = 0; it < nsteps; ++it) {
         timed_begin(h, "k_compute_dt");
-        hipLaunchKernelGGL(k_compute_dt, dim3(g), dim3(kBlock), 0, h->stream, M, (const Node*)A,
+        hipLaunchKernelGGL(k_compute_dt, dim3(g), dim3(kBlock), h->lds_bytes, h->stream, M, (const Node*)A,
                            lam ? h->d0 : (double*)nullptr, h->K, h->vmax, h->flags);
         timed_end(h, "k_compute_dt");
         hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(64), 0, h->stream, h->vmax, h->K.cfl, h->dtp, h->dt_log, base + it);
@@ -1143,7 +1235,7 @@ int queue_steps(swe_handle* h, int nsteps, int* first) {
 
 void free_handle(swe_handle* h) {
     if (!h) return;
-    void* bufs[] = {h->len, h->sptr, h->perm, h->csr2slot, h->col, h->nvec, h->nrm, h->c, h->m, h->z, h->S[0], h->S[1], h->S[2], h->scratch, h->ulo, h->un3,
+    void* bufs[] = {h->len, h->sptr, h->perm, h->csr2slot, h->col, h->nvec, h->nrm, h->elist, h->eoff, h->dpos, h->c, h->m, h->z, h->S[0], h->S[1], h->S[2], h->scratch, h->ulo, h->un3,
                     h->d0, h->dL, h->muL, h->ework, h->alpha, h->res, h->Rm, h->nwork, h->vmax, h->dtp,
                     h->dt_log, h->flags, h->hl, h->uxl, h->uyl, h->hv, h->uxv, h->uyv};
     for (void* b : bufs)
@@ -1232,6 +1324,13 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
         sptr[(size_t)sl + 1] = sptr[(size_t)sl] + 64 * (int64_t)w;
     }
     const int64_t nslot = sptr[(size_t)nsl];
+    int wmax = 0;
+    for (int sl = 0; sl < nsl; ++sl) wmax = std::max(wmax, (int)((sptr[(size_t)sl + 1] - sptr[(size_t)sl]) / 64));
+    if ((size_t)(2 + wmax) * 64 * sizeof(double) * (kBlock / 64) > 65536) {
+        free_handle(h);
+        return fail(SWE_EINVAL, "a row has %d entries; at most %d are supported", wmax,
+                    (int)(65536 / (64 * sizeof(double) * (kBlock / 64))) - 2);
+    }
     std::vector<int> col((size_t)nslot);
     std::vector<double2> cc((size_t)nslot, make_double2(0.0, 0.0));
     std::vector<double2> nv((size_t)nslot, make_double2(0.0, 0.0));
@@ -1239,9 +1338,12 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
     std::vector<int64_t> csr2slot((size_t)nnz);
     std::vector<double> m_int((size_t)np), z_int((size_t)np);
     for (int64_t q = 0; q < nslot; ++q) col[(size_t)q] = 0;
+    std::vector<int> dpos((size_t)np, 0);
     for (int r = 0; r < np; ++r) {
         const int i = perm[(size_t)r];
         const int64_t base = sptr[(size_t)(r >> 6)] + (r & 63);
+        for (int q = 0; q < len[(size_t)r]; ++q)
+            if (cl[(size_t)rp[(size_t)i] + q] == i) dpos[(size_t)r] = q;
         for (int q = 0; q < len[(size_t)r]; ++q) {
             const int p = rp[(size_t)i] + q;
             const int64_t slot = base + 64 * (int64_t)q;
@@ -1256,8 +1358,23 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
         m_int[(size_t)r] = d->lumped[i];
         z_int[(size_t)r] = d->bath[i];
     }
+    // compact per-slice lists of off-diagonal slots, in slot order (coalesced in the lambda phase)
+    std::vector<int64_t> eoff((size_t)nsl + 1, 0);
+    std::vector<uint16_t> elist;
+    elist.reserve((size_t)(nnz - np));
+    for (int sl = 0; sl < nsl; ++sl) {
+        const int w = (int)((sptr[(size_t)sl + 1] - sptr[(size_t)sl]) / 64);
+        for (int q = 0; q < w; ++q)
+            for (int l = 0; l < 64; ++l) {
+                const int r = sl * 64 + l;
+                if (r < np && q < len[(size_t)r] && q != dpos[(size_t)r]) elist.push_back((uint16_t)(q * 64 + l));
+            }
+        eoff[(size_t)sl + 1] = (int64_t)elist.size();
+    }
     h->np_pad = np_pad;
     h->nslot = nslot;
+    h->wmax = wmax;
+    h->lds_bytes = (size_t)(2 + wmax) * 64 * sizeof(double) * (kBlock / 64);
 #define ALLOC(ptr, n) do { if ((rc = dalloc(&(ptr), (n)))) { free_handle(h); return rc; } } while (0)
     ALLOC(h->len, (size_t)np_pad);
     ALLOC(h->sptr, (size_t)nsl + 1);
@@ -1265,6 +1382,9 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
     ALLOC(h->csr2slot, (size_t)nnz);
     ALLOC(h->col, (size_t)nslot);
     ALLOC(h->c, (size_t)nslot);
+    ALLOC(h->elist, std::max<size_t>(elist.size(), 1));
+    ALLOC(h->eoff, (size_t)nsl + 1);
+    ALLOC(h->dpos, (size_t)np);
     ALLOC(h->nvec, (size_t)nslot);
     ALLOC(h->nrm, (size_t)nslot);
     ALLOC(h->m, (size_t)np);
@@ -1316,7 +1436,9 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
         (rc = up(h->perm, perm.data(), sizeof(int) * perm.size())) ||
         (rc = up(h->csr2slot, csr2slot.data(), sizeof(int64_t) * csr2slot.size())) ||
         (rc = up(h->col, col.data(), sizeof(int) * col.size())) || (rc = up(h->c, cc.data(), sizeof(double2) * cc.size())) ||
-        (rc = up(h->nvec, nv.data(), sizeof(double2) * nv.size())) || (rc = up(h->nrm, nr.data(), sizeof(double) * nr.size())) ||
+        (rc = up(h->nvec, nv.data(), sizeof(double2) * nv.size())) ||
+        (rc = up(h->elist, elist.data(), sizeof(uint16_t) * elist.size())) ||
+        (rc = up(h->eoff, eoff.data(), sizeof(int64_t) * eoff.size())) || (rc = up(h->dpos, dpos.data(), sizeof(int) * dpos.size())) || (rc = up(h->nrm, nr.data(), sizeof(double) * nr.size())) ||
         (rc = up(h->m, m_int.data(), sizeof(double) * (size_t)np)) || (rc = up(h->z, z_int.data(), sizeof(double) * (size_t)np)) ||
         (rc = up(h->hl, lists[0].data(), sizeof(int) * lists[0].size())) ||
         (rc = up(h->uxl, lists[1].data(), sizeof(int) * lists[1].size())) ||
@@ -1406,7 +1528,7 @@ int swe_compute_dt(swe_handle* h, const double* un, double* dt) {
     } else if (!h->state_set) {
         return fail(SWE_EINVAL, "no state has been set");
     }
-    hipLaunchKernelGGL(k_compute_dt, dim3(grid_of(h->np)), dim3(kBlock), 0, h->stream, mesh_of(h), U,
+    hipLaunchKernelGGL(k_compute_dt, dim3(grid_of(h->np)), dim3(kBlock), h->lds_bytes, h->stream, mesh_of(h), U,
                        (double*)nullptr, h->K, h->vmax, h->flags);
     hipLaunchKernelGGL(k_finalize_dt, dim3(1), dim3(64), 0, h->stream, h->vmax, h->K.cfl, h->dtp, (double*)nullptr, 0);
     HIPCHK(hipMemcpyAsync(h->host_pinned, h->dtp, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1446,6 +1568,41 @@ int swe_fct_positivity(swe_handle* h, const double* us, const double* f, double*
     HIPCHK(hipMemcpyAsync(lij, h->ework, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return SWE_OK;
+}
+
+int swe_fct_positivity_csr(int np, int nnz, const int* ia, const int* ja, const double* us, const double* f,
+                           double htiny, double* lij) {
+    if (np <= 0 || nnz <= 0 || !ia || !ja || !us || !f || !lij) return fail(SWE_EINVAL, "null or empty argument");
+    if (ia[0] != 1 || ia[np] - 1 != nnz) return fail(SWE_EINVAL, "ia must start at 1 and end at nnz+1");
+    for (int i = 0; i < np; ++i)
+        if (ia[i + 1] < ia[i]) return fail(SWE_EINVAL, "ia is not monotone at row %d", i + 1);
+    for (int p = 0; p < nnz; ++p)
+        if (ja[p] < 1 || ja[p] > np) return fail(SWE_EINVAL, "ja(%d)=%d out of range", p + 1, ja[p]);
+    int *dia = nullptr, *dja = nullptr;
+    double *dus = nullptr, *df = nullptr, *dR = nullptr, *dl = nullptr;
+    int rc = SWE_OK;
+    if ((rc = dalloc(&dia, (size_t)np + 1)) || (rc = dalloc(&dja, (size_t)nnz)) || (rc = dalloc(&dus, (size_t)np)) ||
+        (rc = dalloc(&df, (size_t)nnz)) || (rc = dalloc(&dR, (size_t)np)) || (rc = dalloc(&dl, (size_t)nnz))) {
+    } else {
+        hipError_t e = hipMemcpy(dia, ia, sizeof(int) * ((size_t)np + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dja, ja, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dus, us, sizeof(double) * (size_t)np, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(df, f, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            const unsigned g = (unsigned)((np + 255) / 256);
+            hipLaunchKernelGGL(k_fctpos_csr_r, dim3(g), dim3(256), 0, 0, np, (const int*)dia, (const double*)df,
+                               (const double*)dus, dR, htiny);
+            hipLaunchKernelGGL(k_fctpos_csr_l, dim3(g), dim3(256), 0, 0, np, (const int*)dia, (const int*)dja,
+                               (const double*)df, (const double*)dR, dl);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(lij, dl, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(SWE_EHIP, "swe_fct_positivity_csr: %s", hipGetErrorString(e));
+    }
+    void* bufs[] = {dia, dja, dus, df, dR, dl};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    return rc;
 }
 
 int swe_set_time(swe_handle* h, double t) {

@@ -1,0 +1,127 @@
+!==============================================================================================
+! swe_amd_c.f90 -- iso_c_binding interface to libswe_amd.so (include/swe_amd.h), the MI355X
+! explicit shallow-water step.  The types mirror swe_mesh_desc and swe_params; every routine is a
+! BIND(C) interface to the C-ABI entry point of the same name.
+!==============================================================================================
+MODULE swe_amd_c
+  USE iso_c_binding
+  IMPLICIT NONE
+
+  INTEGER(c_int), PARAMETER :: SWE_OK = 0, SWE_ENEG_H = 1, SWE_ELAMBDA = 2, SWE_EBADOPT = 3, &
+       SWE_EHIP = 4, SWE_EINVAL = 5, SWE_ENOMEM = 6
+  INTEGER(c_int), PARAMETER :: SWE_GALERKIN = 0, SWE_VISCOUS = 1, SWE_ENTROPY_VISC = 2, SWE_FCT = 3
+  INTEGER(c_int), PARAMETER :: SWE_BC_NONE = 0, SWE_BC_WALLS = 1, SWE_BC_INFLOW = 2, SWE_BC_VALUES = 3
+
+  TYPE, BIND(C) :: swe_params
+     REAL(c_double)    :: gravity, htiny, cfl, max_water_h, mannings
+     INTEGER(c_int)    :: type_test, viscosity_type, viscous_type, alpha_limit, roundoff_fix
+  END TYPE swe_params
+
+  TYPE, BIND(C) :: swe_mesh_desc
+     INTEGER(c_int)    :: np, nnz
+     TYPE(c_ptr)       :: ia, ja, cx, cy, lumped, bath
+     INTEGER(c_int)    :: n_h_dir, n_ux_dir, n_uy_dir
+     TYPE(c_ptr)       :: h_dir, ux_dir, uy_dir
+  END TYPE swe_mesh_desc
+
+  INTERFACE
+     FUNCTION swe_last_error() BIND(C, NAME='swe_last_error') RESULT(msg)
+       IMPORT :: c_ptr
+       TYPE(c_ptr) :: msg
+     END FUNCTION swe_last_error
+
+     FUNCTION swe_assemble_nnz(np, me, jj) BIND(C, NAME='swe_assemble_nnz') RESULT(nnz)
+       IMPORT :: c_int, c_long
+       INTEGER(c_int), VALUE :: np, me
+       INTEGER(c_int), DIMENSION(*), INTENT(IN) :: jj
+       INTEGER(c_long) :: nnz
+     END FUNCTION swe_assemble_nnz
+
+     FUNCTION swe_assemble(np, me, jj, rr, ia, ja, diag, mass, lumped, cx, cy) &
+          BIND(C, NAME='swe_assemble') RESULT(rc)
+       IMPORT :: c_int, c_double
+       INTEGER(c_int), VALUE :: np, me
+       INTEGER(c_int), DIMENSION(*), INTENT(IN) :: jj
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: rr
+       INTEGER(c_int), DIMENSION(*) :: ia, ja, diag
+       REAL(c_double), DIMENSION(*) :: mass, lumped, cx, cy
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_assemble
+
+     FUNCTION swe_create(mesh, prm, handle) BIND(C, NAME='swe_create') RESULT(rc)
+       IMPORT :: c_int, c_ptr, swe_mesh_desc, swe_params
+       TYPE(swe_mesh_desc), INTENT(IN) :: mesh
+       TYPE(swe_params), INTENT(IN) :: prm
+       TYPE(c_ptr) :: handle
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_create
+
+     FUNCTION swe_destroy(handle) BIND(C, NAME='swe_destroy') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: handle
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_destroy
+
+     FUNCTION swe_set_params(handle, prm) BIND(C, NAME='swe_set_params') RESULT(rc)
+       IMPORT :: c_int, c_ptr, swe_params
+       TYPE(c_ptr), VALUE :: handle
+       TYPE(swe_params), INTENT(IN) :: prm
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_set_params
+
+     FUNCTION swe_compute_dij(handle, un, dij) BIND(C, NAME='swe_compute_dij') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: un
+       REAL(c_double), DIMENSION(*) :: dij
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_compute_dij
+
+     FUNCTION swe_compute_dt(handle, un, dt) BIND(C, NAME='swe_compute_dt') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: un
+       REAL(c_double) :: dt
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_compute_dt
+
+     FUNCTION swe_euler(handle, un, unext, dt) BIND(C, NAME='swe_euler') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: un
+       REAL(c_double), DIMENSION(*) :: unext
+       REAL(c_double), VALUE :: dt
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_euler
+
+     FUNCTION swe_fct_positivity_csr(np, nnz, ia, ja, unext_scaled, f, htiny, lij) &
+          BIND(C, NAME='swe_fct_positivity_csr') RESULT(rc)
+       IMPORT :: c_int, c_double
+       INTEGER(c_int), VALUE :: np, nnz
+       INTEGER(c_int), DIMENSION(*), INTENT(IN) :: ia, ja
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: unext_scaled, f
+       REAL(c_double), VALUE :: htiny
+       REAL(c_double), DIMENSION(*) :: lij
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_fct_positivity_csr
+  END INTERFACE
+
+CONTAINS
+
+  ! The library's last error message as a Fortran string.
+  FUNCTION swe_error_message() RESULT(s)
+    CHARACTER(LEN=512) :: s
+    CHARACTER(KIND=c_char), DIMENSION(:), POINTER :: p
+    INTEGER :: n
+    TYPE(c_ptr) :: m
+    s = ' '
+    m = swe_last_error()
+    IF (.NOT. c_associated(m)) RETURN
+    CALL c_f_pointer(m, p, [512])
+    DO n = 1, 512
+       IF (p(n) == c_null_char) EXIT
+       s(n:n) = p(n)
+    END DO
+  END FUNCTION swe_error_message
+
+END MODULE swe_amd_c

@@ -59,6 +59,7 @@ SIGNATURES = {
     "swe_compute_dt": (_I, [_P, _P, C.POINTER(_D)]),
     "swe_euler": (_I, [_P, _P, _P, _D]),
     "swe_fct_positivity": (_I, [_P, _P, _P, _P]),
+    "swe_fct_positivity_csr": (_I, [_I, _I, _P, _P, _P, _P, _D, _P]),
     "swe_set_state": (_I, [_P, _P]),
     "swe_get_state": (_I, [_P, _P]),
     "swe_set_time": (_I, [_P, _D]),
@@ -121,6 +122,19 @@ def assemble(jj0: np.ndarray, rr: np.ndarray):
     _check(lib.swe_assemble(npn, me, _ptr(jj1), _ptr(rr), _ptr(ia), _ptr(ja), _ptr(diag), _ptr(mass),
                             _ptr(lumped), _ptr(cx), _ptr(cy)))
     return ia - 1, ja - 1, diag - 1, mass, lumped, np.stack([cx, cy], 1)
+
+
+def fct_positivity_csr(ia, ja, unext_scaled, f, htiny):
+    """FCT_positivity (fct.f90:5-42) on a 0-based CSR matrix, without a solver handle."""
+    lib = load_library()
+    ia1 = np.ascontiguousarray(ia, np.int32) + 1
+    ja1 = np.ascontiguousarray(ja, np.int32) + 1
+    us = np.ascontiguousarray(unext_scaled, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    lij = np.zeros(ja1.size)
+    _check(lib.swe_fct_positivity_csr(ia1.size - 1, ja1.size, _ptr(ia1), _ptr(ja1), _ptr(us), _ptr(f),
+                                      float(htiny), _ptr(lij)))
+    return lij
 
 
 class Solver:
