@@ -39,7 +39,9 @@ def kernel_bytes_per_node(kernel, k, alpha=True):
 
 
 def step_bytes_per_node_stage(k, fct=True, alpha=True):
-    """Compulsory bytes per node-stage of the whole step ('Algorithmic bytes')."""
+    """Compulsory HBM bytes per node-stage of the whole step, k = stored entries per row: the sum of
+    the per-launch figures of kernel_bytes_per_node over one stage (FCT: three graph sweeps plus
+    the alpha pass and a third of the dt pass), i.e. 324 + 70.67k with FCT and alpha."""
     if fct:
         return 324 + 70.67 * k
     return (124 + 30.67 * k) if alpha else (96 + 26.67 * k)
@@ -144,7 +146,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding of a single mesh is a design note "
-              "only (see DESIGN.md), so --gpus %d is not supported" % args.gpus, file=sys.stderr)
+              "only, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from swe_amd.solver import load_library

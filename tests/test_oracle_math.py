@@ -1,0 +1,253 @@
+"""Checks of the oracle against what the mathematics fixes, independent of the reference build.
+
+The reference's own outputs pin the oracle bit for bit on the fixtures (test_oracle_reference.py);
+the checks here hold for any input and are chosen so that a plausible slip (a transposed c_ij, a
+sign error on the normal, a dropped bathymetry term, an unlimited antidiffusive flux) fails one of
+them.  Each `planted` test feeds such a mistake through the same check and asserts that it fails.
+
+* P1 geometry: c_ij = int phi_i grad phi_j, M_ij and the lumped mass against a numpy formula
+  written from the element gradients (update.f90:18-132, construct_matrices, builds them from
+  Gauss points instead).
+* Maximum wave speed (condlim.f90:879-957, compute_lambda_vacc, called at update.f90:314): an
+  upper bound of the exact Riemann solution's fastest wave, equal to it when both waves are
+  rarefactions; exact closed form for identical states; invariant under rotation and under the
+  left/right mirror.
+* Galerkin consistency: for a linear discharge and flat water, the lumped Galerkin step changes
+  h by exactly -dt div q at interior nodes (smb_2, update.f90:626-682; apply_viscosity adds
+  nothing when h is constant).
+* Lake at rest (well balancing, update.f90:684-718): a fully wet flat free surface over a bump
+  stays at rest.
+* FCT positivity (fct.f90:5-42): 0 <= l_ij <= 1 and the limited update keeps h >= 0.
+"""
+import numpy as np
+import pytest
+from scipy.sparse import coo_matrix
+
+import pyoracle as O
+from swe_amd import mesh as M
+
+G = 9.81
+HTINY = 4.0e-4
+
+
+@pytest.fixture(scope="module")
+def square(oracle):
+    m = M.renumber_first_touch(M.structured_mesh(41, 41, 0.0, 4.0, 0.0, 4.0, jitter=0.2, seed=1))
+    ia, ja, diag, mass, lumped, cij = O.assemble(m.jj, m.rr)
+    return m, ia, ja, mass, lumped, cij
+
+
+def p1_matrices(m):
+    """c_ij = sum_K |K|/3 grad phi_j|_K, M_ij = |K|(1+delta_ij)/12, lumped_i = sum_K |K|/3."""
+    x = m.rr[m.jj]
+    area2 = ((x[:, 1, 0] - x[:, 0, 0]) * (x[:, 2, 1] - x[:, 0, 1])
+             - (x[:, 2, 0] - x[:, 0, 0]) * (x[:, 1, 1] - x[:, 0, 1]))
+    area = area2 / 2
+    grad = np.zeros((m.me, 3, 2))
+    for a in range(3):
+        b, c = (a + 1) % 3, (a + 2) % 3
+        grad[:, a, 0] = (x[:, b, 1] - x[:, c, 1]) / area2
+        grad[:, a, 1] = (x[:, c, 0] - x[:, b, 0]) / area2
+    rows, cols, vx, vy, vm = [], [], [], [], []
+    for a in range(3):
+        for b in range(3):
+            rows.append(m.jj[:, a])
+            cols.append(m.jj[:, b])
+            vx.append(area / 3 * grad[:, b, 0])
+            vy.append(area / 3 * grad[:, b, 1])
+            vm.append(area * (1 / 6 if a == b else 1 / 12))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    mk = lambda v: coo_matrix((np.concatenate(v), (rows, cols)), shape=(m.np, m.np)).tocsr()
+    lumped = np.bincount(m.jj.ravel(), weights=np.repeat(area / 3, 3), minlength=m.np)
+    return mk(vx), mk(vy), mk(vm), lumped
+
+
+def csr_rows(ia):
+    return np.repeat(np.arange(ia.size - 1), np.diff(ia))
+
+
+def geometry_error(m, ia, ja, mass, lumped, cij):
+    Cx, Cy, Mm, lm = p1_matrices(m)
+    r = csr_rows(ia)
+    sc = np.abs(cij).max()
+    return max(np.abs(cij[:, 0] - np.asarray(Cx[r, ja]).ravel()).max() / sc,
+               np.abs(cij[:, 1] - np.asarray(Cy[r, ja]).ravel()).max() / sc,
+               np.abs(mass - np.asarray(Mm[r, ja]).ravel()).max() / np.abs(mass).max(),
+               np.abs(lumped - lm).max() / lm.max())
+
+
+def transpose_index(ia, ja):
+    r = csr_rows(ia)
+    pos = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(r, ja))}
+    return np.array([pos[(int(b), int(a))] for a, b in zip(r, ja)])
+
+
+def test_p1_geometry_matches_formula(square):
+    m, ia, ja, mass, lumped, cij = square
+    assert geometry_error(m, ia, ja, mass, lumped, cij) < 1e-14
+    assert abs(lumped.sum() - 16.0) < 1e-12                      # area of [0,4]^2
+    rs = np.add.reduceat(cij, ia[:-1], axis=0)                   # sum_j grad phi_j = 0
+    assert np.abs(rs).max() < 1e-14 * np.abs(cij).max() * 10
+
+
+def test_planted_transposed_cij_fails_geometry(square):
+    m, ia, ja, mass, lumped, cij = square
+    assert geometry_error(m, ia, ja, mass, lumped, cij[transpose_index(ia, ja)]) > 1e-3
+
+
+# --- maximum wave speed ---------------------------------------------------------------------
+
+def exact_max_speed(hl, ul, hr, ur):
+    """Fastest wave of the exact Riemann solution (depth h* by bisection on the textbook
+    two-wave function; vacuum when the rarefaction curves do not meet)."""
+    def f(h, hk):
+        return np.where(h <= hk, 2 * (np.sqrt(G * h) - np.sqrt(G * hk)),
+                        (h - hk) * np.sqrt(0.5 * G * (h + hk) / (h * hk)))
+    phi = lambda h: f(h, hl) + f(h, hr) + ur - ul
+    vac = phi(np.full_like(hl, 1e-300)) > 0
+    lo, hi = np.zeros_like(hl), np.ones_like(hl)
+    while np.any(phi(hi) < 0):
+        hi = np.where(phi(hi) < 0, 2 * hi, hi)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        s = phi(mid) < 0
+        lo, hi = np.where(s, mid, lo), np.where(s, hi, mid)
+    hs = np.where(vac, 0.0, 0.5 * (lo + hi))
+    grow = lambda hk: np.sqrt((1 + np.maximum((hs - hk) / (2 * hk), 0)) * (1 + np.maximum((hs - hk) / hk, 0)))
+    l1 = ul - np.sqrt(G * hl) * grow(hl)
+    l3 = ur + np.sqrt(G * hr) * grow(hr)
+    return np.maximum(np.abs(l1), np.abs(l3)), hs
+
+
+def riemann_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0, 2 * np.pi, n)
+    hl, hr = rng.uniform(0.01, 2, n), rng.uniform(0.01, 2, n)
+    ql = rng.normal(0, 2, (n, 2)) * hl[:, None]
+    qr = rng.normal(0, 2, (n, 2)) * hr[:, None]
+    return hl, ql, hr, qr, np.cos(th), np.sin(th)
+
+
+def check_wave_speed(lam, hl, ql, hr, qr, nx, ny):
+    ul = (ql[:, 0] * nx + ql[:, 1] * ny) / hl
+    ur = (qr[:, 0] * nx + qr[:, 1] * ny) / hr
+    ex, hs = exact_max_speed(hl, ul, hr, ur)
+    rr = hs <= np.minimum(hl, hr)
+    return (lam / ex).min() >= 1 - 1e-14 and np.abs(lam / ex - 1)[rr].max() < 1e-13
+
+
+def test_wave_speed_bounds_exact_riemann(oracle):
+    hl, ql, hr, qr, nx, ny = riemann_batch(20000, 1)
+    lam, err = O.lambda_batch(np.column_stack([hl, ql, hr, qr, nx, ny]), G, HTINY)
+    assert err == 0
+    assert check_wave_speed(lam, hl, ql, hr, qr, nx, ny)
+
+
+def test_planted_normal_sign_fails_wave_speed(oracle):
+    hl, ql, hr, qr, nx, ny = riemann_batch(20000, 1)
+    lam, _ = O.lambda_batch(np.column_stack([hl, ql, hr, qr, -nx, -ny]), G, HTINY)
+    assert not check_wave_speed(lam, hl, ql, hr, qr, nx, ny)
+
+
+def test_wave_speed_closed_forms_and_symmetries(oracle):
+    hl, ql, hr, qr, nx, ny = riemann_batch(4000, 2)
+    # identical states: |u.n| + sqrt(g h)
+    lam, err = O.lambda_batch(np.column_stack([hl, ql, hl, ql, nx, ny]), G, HTINY)
+    exact = np.abs((ql[:, 0] * nx + ql[:, 1] * ny) / hl) + np.sqrt(G * hl)
+    assert err == 0 and np.abs(lam - exact).max() < 1e-15 * exact.max() * 4
+    a, _ = O.lambda_batch(np.column_stack([hl, ql, hr, qr, nx, ny]), G, HTINY)
+    b, _ = O.lambda_batch(np.column_stack([hr, qr, hl, ql, -nx, -ny]), G, HTINY)
+    assert np.abs(a - b).max() < 1e-15 * a.max() * 4
+    c0, s0 = np.cos(0.7), np.sin(0.7)
+    rot = lambda q: np.column_stack([c0 * q[:, 0] - s0 * q[:, 1], s0 * q[:, 0] + c0 * q[:, 1]])
+    c, _ = O.lambda_batch(np.column_stack([hl, rot(ql), hr, rot(qr),
+                                           c0 * nx - s0 * ny, s0 * nx + c0 * ny]), G, HTINY)
+    assert np.abs(a - c).max() < 1e-14 * a.max()
+    d, err = O.lambda_batch(np.zeros((3, 8)) + [0, 0, 0, 0, 0, 0, 1, 0], G, HTINY)
+    assert err == 0 and np.all(d == 0.0)                         # dry | dry
+
+
+# --- Galerkin consistency and well balancing -------------------------------------------------
+
+def linear_discharge_error(m, ia, ja, lumped, cij, viscosity):
+    x, y = m.rr[:, 0], m.rr[:, 1]
+    inner = (x > 1e-9) & (x < 4 - 1e-9) & (y > 1e-9) & (y < 4 - 1e-9)
+    b, f = 0.3, -0.7
+    u = np.column_stack([np.ones(m.np), 0.1 + b * x + 0.05 * y, -0.2 + 0.02 * x + f * y])
+    o = O.Oracle(ia, ja, cij, lumped, np.zeros(m.np), gravity=G, htiny=HTINY, cfl=0.25,
+                 max_water_h=2.0, viscosity_type=viscosity, type_test=5)
+    dt = 0.05
+    un, _, err = o.euler(u, dt)
+    assert err == 0
+    return np.abs((un[inner, 0] - 1.0) / dt + (b + f)).max()
+
+
+@pytest.mark.parametrize("viscosity", [O.GALERKIN, O.VISCOUS])
+def test_linear_discharge_divergence_exact(square, viscosity):
+    m, ia, ja, mass, lumped, cij = square
+    assert linear_discharge_error(m, ia, ja, lumped, cij, viscosity) < 1e-12
+
+
+def test_planted_transposed_cij_fails_divergence(square):
+    m, ia, ja, mass, lumped, cij = square
+    assert linear_discharge_error(m, ia, ja, lumped, cij[transpose_index(ia, ja)], O.GALERKIN) > 0.1
+
+
+def lake_motion(m, ia, ja, lumped, cij, viscosity, alpha, bath_seen):
+    z = -0.1 * (1 - ((m.rr[:, 0] - 2) ** 2 + (m.rr[:, 1] - 2) ** 2))
+    u = np.zeros((m.np, 3))
+    u[:, 0] = 1.0 - z                                             # surface at 1 > max z = 0.7
+    o = O.Oracle(ia, ja, cij, lumped, bath_seen(z), gravity=G, htiny=HTINY, cfl=0.25,
+                 max_water_h=2.0, viscosity_type=viscosity, alpha_limit=alpha, type_test=5)
+    dt, err = o.compute_dt(u)
+    assert err == 0
+    un, _, err = o.euler(u, dt)
+    assert err == 0
+    return max(np.abs(un[:, 0] - u[:, 0]).max(), np.abs(un[:, 1:]).max())
+
+
+@pytest.mark.parametrize("viscosity,alpha", [(O.VISCOUS, False), (O.VISCOUS, True),
+                                             (O.ENTROPY_VISC, True), (O.FCT, True)])
+def test_lake_at_rest(square, viscosity, alpha):
+    m, ia, ja, mass, lumped, cij = square
+    assert lake_motion(m, ia, ja, lumped, cij, viscosity, alpha, lambda z: z) < 1e-14
+
+
+def test_planted_dropped_bathymetry_fails_lake(square):
+    m, ia, ja, mass, lumped, cij = square
+    assert lake_motion(m, ia, ja, lumped, cij, O.FCT, True, np.zeros_like) > 1e-4
+
+
+# --- FCT positivity --------------------------------------------------------------------------
+
+def fct_case(square, seed):
+    m, ia, ja, *_ = square
+    rng = np.random.default_rng(seed)
+    r = csr_rows(ia)
+    a = rng.normal(0, 1, ja.size)
+    f = a - a[transpose_index(ia, ja)]                           # antisymmetric, f_ii = 0
+    us = np.maximum(rng.normal(3.0, 2.0, m.np), 0.0)             # m_i h_low_i / dt, some zeros
+    return ia, ja, r, f, us
+
+
+def fct_violation(ia, ja, r, f, us, lij):
+    upd = us + np.bincount(r, weights=lij * f, minlength=us.size)
+    scale = np.bincount(r, weights=np.abs(f), minlength=us.size) + us
+    return (-upd / scale).max()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fct_limiter_bounds(square, seed):
+    ia, ja, r, f, us, = fct_case(square, seed)
+    lij = O.fct_positivity(ia, ja, f, us, HTINY)
+    assert lij.min() >= 0.0 and lij.max() <= 1.0
+    assert fct_violation(ia, ja, r, f, us, lij) < 1e-15
+    assert np.all(lij[us[r] <= HTINY][f[us[r] <= HTINY] < 0] == 0.0)
+    assert (lij == 1.0).mean() > 0.1                              # not limiting everything
+    free = O.fct_positivity(ia, ja, np.abs(f), np.full(us.size, 1.0), HTINY)
+    assert np.all(free[r != ja] == 1.0)                           # nothing to limit
+
+
+def test_planted_unlimited_fct_fails_bound(square):
+    ia, ja, r, f, us = fct_case(square, 0)
+    assert fct_violation(ia, ja, r, f, us, np.ones_like(f)) > 1e-3
