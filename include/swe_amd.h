@@ -12,7 +12,7 @@
  * with the reference CPU step except where a source term calls pow (Manning friction), where
  * they agree to rounding.
  *
- * Threading: a handle owns its device buffers and one HIP stream; use a handle from one host
+ * Threading: a handle (or batch) owns its device buffers and its HIP streams; use it from one host
  * thread at a time.  No host pointer is kept after a call returns.
  * Errors: every call returns a swe_status; swe_last_error() gives a message for the calling
  * thread.
@@ -31,7 +31,8 @@ typedef enum swe_status {
     SWE_EBADOPT = 3,    /* option strings the reference rejects (SOURCES/update.f90:199,217,243,273) */
     SWE_EHIP = 4,       /* a HIP runtime call failed */
     SWE_EINVAL = 5,     /* malformed arguments (sizes, indices, null pointers) */
-    SWE_ENOMEM = 6      /* device allocation failed */
+    SWE_ENOMEM = 6,     /* device allocation failed */
+    SWE_EFULL = 7       /* a batch has no room for this mesh now (retry after removing instances) */
 } swe_status;
 
 typedef enum swe_viscosity {      /* inputs%viscosity_type (SOURCES/read_data.f90:57-58) */
@@ -133,9 +134,8 @@ int swe_get_time(swe_handle *h, double *t);
  * SWE_ENEG_H / SWE_ELAMBDA if any stage tripped those checks. */
 int swe_run(swe_handle *h, int nsteps, double *time, double *dts);
 
-/* The same steps queued without waiting for completion (a time-dependent inflow still reads
- * each step's dt back while the step's first stage runs).  swe_sync waits for the queued work,
- * folds the queued dts into the handle's time and reports the checks. */
+/* The same steps queued without waiting for completion (dt, the time and the inflow values are
+ * evaluated on the device).  swe_sync waits for the queued work and reports the checks. */
 int swe_run_async(swe_handle *h, int nsteps);
 int swe_sync(swe_handle *h);
 
@@ -147,6 +147,78 @@ void *swe_stream(swe_handle *h);
  * number of launches since timing was enabled and their summed duration in milliseconds. */
 int swe_enable_kernel_timing(swe_handle *h, const char *kernel);
 int swe_kernel_timing(swe_handle *h, const char *kernel, int *launches, double *ms);
+
+/* Launch schedules: per kernel, the workgroup size (128, 256 or 512) and whether workgroups are
+ * ordered XCD-aware (1) or in dispatch order (0).  Kernels, in this order: k_compute_dt, k_alpha,
+ * k_stage_lo, k_fct_lo, k_fct_rminus, k_fct_apply, k_ev_update.  swe_tune times every schedule
+ * of each kernel the scheme uses on the handle's own mesh and state (the state and time are
+ * restored afterwards) and keeps the fastest; results do not depend on the schedule. */
+#define SWE_NUM_KERNELS 7
+int swe_tune(swe_handle *h, int steps);
+int swe_get_schedule(swe_handle *h, int *blocks, int *remap);
+int swe_set_schedule(swe_handle *h, const int *blocks, const int *remap);
+/* Queue steps as one captured HIP graph per step (1, the default) or as individual launches (0).
+ * Timed kernels (swe_enable_kernel_timing) always launch individually. */
+int swe_use_graph(swe_handle *h, int on);
+
+/* -------------------------------------------------------------------------------------------
+ * Batches: many meshes of different sizes resident in one device arena and advanced together,
+ * each with its own parameters, boundary program, dt (its own CFL reduction), time and stopping
+ * rule.  The reference runs one mesh per process; a batch runs its time loop (SOURCES/main.f90:
+ * 61-136) for every instance at once, and each instance's results are bitwise those of the same
+ * mesh run alone.  All instances of a batch share the scheme: viscosity_type, viscous_type,
+ * alpha_limit and roundoff_fix (type_test, and with it the source term, may differ).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct swe_batch swe_batch;
+
+typedef struct swe_batch_caps {
+    long long max_nodes;      /* node rows (each mesh takes its np rounded up to 64) */
+    long long max_entries;    /* entry slots (a mesh takes about nnz * 1.05) */
+    long long max_boundary;   /* Dirichlet list entries (h, ux and uy lists together) */
+    int max_instances;        /* meshes resident at once */
+} swe_batch_caps;
+
+typedef enum swe_instance_state {
+    SWE_INSTANCE_RUNNING = 1, /* takes steps while time < tfinal and steps < max_steps */
+    SWE_INSTANCE_DONE = 2,    /* reached tfinal or max_steps; its state is final */
+    SWE_INSTANCE_FAILED = 3   /* a check tripped (see swe_batch_status) */
+} swe_instance_state;
+
+/* Device bytes a batch with these capacities allocates. */
+unsigned long long swe_batch_bytes(const swe_batch_caps *caps);
+int swe_batch_create(const swe_batch_caps *caps, swe_batch **out);
+int swe_batch_destroy(swe_batch *b);
+
+/* Admit a mesh with state un(3,np) at time t0.  It steps while time < tfinal (the loop condition
+ * of SOURCES/main.f90:61) and at most max_steps steps.  Returns SWE_EFULL when the batch has no
+ * room now.  The instance joins the steps queued after this call. */
+int swe_batch_add(swe_batch *b, const swe_mesh_desc *mesh, const swe_params *prm, const double *un,
+                  double t0, double tfinal, long long max_steps, int *id);
+int swe_batch_set_bc(swe_batch *b, int id, int kind, double amp, double period, double level);
+int swe_batch_set_bc_values(swe_batch *b, int id, const double *h_vals, const double *ux_vals,
+                            const double *uy_vals);
+/* Free an instance's rows (after reading its state). */
+int swe_batch_remove(swe_batch *b, int id);
+
+/* Queue nsteps steps of every running instance; returns without waiting. */
+int swe_batch_step(swe_batch *b, int nsteps);
+int swe_batch_sync(swe_batch *b);
+
+/* State, time and step count of an instance (waits for queued work).  For a failed instance
+ * *error is SWE_ENEG_H or SWE_ELAMBDA, else SWE_OK. */
+int swe_batch_status(swe_batch *b, int id, int *state, double *time, long long *steps, int *error);
+/* Instances that are done or failed, up to cap of them (waits for queued work). */
+int swe_batch_poll(swe_batch *b, int *ids, int cap, int *n);
+int swe_batch_get_state(swe_batch *b, int id, double *un);
+
+/* Free capacity: largest free run of node rows and entry slots, free instance records. */
+int swe_batch_free(swe_batch *b, long long *rows, long long *entries, int *instances);
+
+int swe_batch_tune(swe_batch *b, int steps);
+int swe_batch_use_graph(swe_batch *b, int on);
+void *swe_batch_stream(swe_batch *b);
+int swe_batch_enable_kernel_timing(swe_batch *b, const char *kernel);
+int swe_batch_kernel_timing(swe_batch *b, const char *kernel, int *launches, double *ms);
 
 #ifdef __cplusplus
 }

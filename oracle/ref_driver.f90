@@ -33,7 +33,16 @@ PROGRAM ref_driver
   USE boundary_conditions
   USE fct
   USE matrix_type
+  USE iso_c_binding, ONLY : c_double
   IMPLICIT NONE
+  INTERFACE
+     ! sin rounded to nearest, from the C checker (swe_oracle.c): the inflow value's definition
+     FUNCTION orc_sin_rn(x) BIND(C, NAME='orc_sin_rn') RESULT(s)
+       IMPORT :: c_double
+       REAL(c_double), VALUE :: x
+       REAL(c_double) :: s
+     END FUNCTION orc_sin_rn
+  END INTERFACE
   CHARACTER(LEN=512) :: cmd
   INTEGER :: bc_kind = 0
   REAL(KIND=8) :: bc_amp = 0.d0, bc_period = 1.d0, bc_level = 0.d0
@@ -168,7 +177,7 @@ CONTAINS
 
   ! Dirichlet overwrite.  bc_kind 0 is the reference bdy (SOURCES/main.f90:220-232); kinds 1 and
   ! 2 give values for the problems without a sol_anal case: walls (q.n = 0 on the listed nodes)
-  ! and walls plus a sinusoidal free-surface inflow on the h list.
+  ! and walls plus a sinusoidal free-surface inflow on the h list (sin rounded to nearest).
   SUBROUTINE bdy_ref(uu, t)
     REAL(KIND=8), DIMENSION(:,:), INTENT(INOUT) :: uu
     REAL(KIND=8), INTENT(IN) :: t
@@ -184,7 +193,7 @@ CONTAINS
     IF (bc_kind == 2) THEN
        DO i = 1, SIZE(h_js_D)
           n = h_js_D(i)
-          uu(1,n) = MAX(bc_level - bath(n) + bc_amp*SIN(2.d0*pi*t/bc_period), 0.d0)
+          uu(1,n) = MAX(bc_level - bath(n) + bc_amp*orc_sin_rn(2.d0*pi*t/bc_period), 0.d0)
        END DO
     END IF
     DO i = 1, SIZE(ux_js_D)

@@ -881,6 +881,82 @@ double orc_compute_dt(orc_ctx *c, const double *un, int *err)
     return c->p.cfl * 1 / mx;
 }
 
+/* ---------------------------------------------------------------------------------------------
+ * Correctly rounded sin for the inflow boundary value (a boundary program of this checker, not a
+ * reference case).  libm's sin is not correctly rounded for about one argument in a thousand, so
+ * the inflow value is defined with RN(sin(x)): x - k*pi/2 in double-double (pi/2 as three
+ * doubles), Taylor series of sin or cos in double-double, one final rounding.  Coefficients are
+ * (-1)^n/(2n+1)! and (-1)^n/(2n)!, n = 14..0, as double-double pairs.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct { double hi, lo; } dd;
+static dd dd2(double a, double b) { dd r = {a, b}; return r; }
+static dd dd_prod(double a, double b) { double p = a * b; return dd2(p, fma(a, b, -p)); }
+static dd dd_sum(dd a, dd b)
+{
+    double s = a.hi + b.hi, bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    double lo = e + a.lo + b.lo, hi = s + lo;
+    return dd2(hi, lo - (hi - s));
+}
+static dd dd_times(dd a, dd b)
+{
+    dd p = dd_prod(a.hi, b.hi);
+    double lo = p.lo + (a.hi * b.lo + a.lo * b.hi), hi = p.hi + lo;
+    return dd2(hi, lo - (hi - p.hi));
+}
+static const double SIN_DD[15][2] = {
+    {1.1309962886447716e-31, 1.0498015412959506e-47},
+    {-9.183689863795546e-29, -1.4303150396787322e-45},
+    {6.446950284384474e-26, -1.9330404233703465e-42},
+    {-3.868170170630684e-23, 8.843177655482344e-40},
+    {1.9572941063391263e-20, -1.3643503830087908e-36},
+    {-8.22063524662433e-18, -2.2141894119604265e-34},
+    {2.8114572543455206e-15, 1.6508842730861433e-31},
+    {-7.647163731819816e-13, -7.03872877733453e-30},
+    {1.6059043836821613e-10, 1.2585294588752098e-26},
+    {-2.505210838544172e-08, 1.448814070935912e-24},
+    {2.7557319223985893e-06, -1.858393274046472e-22},
+    {-0.0001984126984126984, -1.7209558293420705e-22},
+    {0.008333333333333333, 1.1564823173178714e-19},
+    {-0.16666666666666666, -9.25185853854297e-18},
+    {1.0, 0.0}
+};
+static const double COS_DD[15][2] = {
+    {3.279889237069838e-30, 1.5117542744029879e-46},
+    {-2.4795962632247976e-27, 1.2953730964765229e-43},
+    {1.6117375710961184e-24, -3.6846573564509766e-41},
+    {-8.896791392450574e-22, 7.911402614872376e-38},
+    {4.110317623312165e-19, 1.4412973378659527e-36},
+    {-1.5619206968586225e-16, -1.1910679660273754e-32},
+    {4.779477332387385e-14, 4.399205485834081e-31},
+    {-1.1470745597729725e-11, -2.0655512752830745e-28},
+    {2.08767569878681e-09, -1.20734505911326e-25},
+    {-2.755731922398589e-07, -2.3767714622250297e-23},
+    {2.48015873015873e-05, 2.1511947866775882e-23},
+    {-0.001388888888888889, 5.300543954373577e-20},
+    {0.041666666666666664, 2.3129646346357427e-18},
+    {-0.5, 0.0},
+    {1.0, 0.0}
+};
+double orc_sin_rn(double x)
+{
+    if (!(fabs(x) < 536870912.0)) return sin(x);
+    double k = rint(x * 0.6366197723675814);
+    dd p0 = dd_prod(k, 1.5707963267948966), p1 = dd_prod(k, 6.123233995736766e-17);
+    dd r = dd2(x - p0.hi, 0.0);
+    r = dd_sum(r, dd2(-p0.lo, 0.0));
+    r = dd_sum(r, dd2(-p1.hi, -p1.lo));
+    r = dd_sum(r, dd2(k * 1.4973849048591698e-33, 0.0));
+    dd r2 = dd_times(r, r);
+    int q = (int)((long long)k & 3);
+    const double (*c)[2] = (q & 1) ? COS_DD : SIN_DD;
+    dd s = dd2(c[0][0], c[0][1]);
+    for (int n = 1; n < 15; n++) s = dd_sum(dd_times(s, r2), dd2(c[n][0], c[n][1]));
+    if (!(q & 1)) s = dd_times(s, r);
+    double v = s.hi + s.lo;
+    return (q & 2) ? -v : v;
+}
+
 /* Dirichlet overwrite of bdy (main.f90:220-232) for the problems driven by this checker */
 static void bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double period, double level,
                 int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
@@ -889,7 +965,7 @@ static void bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double
         const double pi = acos(-1.0);
         for (int q = 0; q < nh; q++) {
             int n = nodes_h[q];
-            u[3 * n] = fmaxF(level - c->bath[n] + amp * sin(2.0 * pi * t / period), 0.0);
+            u[3 * n] = fmaxF(level - c->bath[n] + amp * orc_sin_rn(2.0 * pi * t / period), 0.0);
         }
     }
     if (bc_kind >= 1) {

@@ -58,10 +58,13 @@ int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *di
 
 /* Fixed-step SSP-RK3 loop of SOURCES/main.f90:105-136.  bc_kind 0: no Dirichlet lists allowed;
  * 1: walls (qx = 0 on nodes_ux, qy = 0 on nodes_uy); 2: walls plus the inflow
- * h = max(level - z + amp*sin(2*pi*t/period), 0) on nodes_h.  dts receives dt per step. */
+ * h = max(level - z + amp*orc_sin_rn(2*pi*t/period), 0) on nodes_h.  dts receives dt per step. */
 int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
             int bc_kind, double amp, double period, double level,
             int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy);
+
+/* sin(x) rounded to nearest (double-double evaluation, one final rounding), |x| < 2^29. */
+double orc_sin_rn(double x);
 
 #ifdef __cplusplus
 }

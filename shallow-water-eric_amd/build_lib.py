@@ -15,8 +15,10 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libswe_amd.so")
-SOURCES = [os.path.join(PKG, "csrc", "swe_amd.hip"), os.path.join(PKG, "csrc", "swe_setup.cpp")]
-HEADERS = [os.path.join(REPO, "include", "swe_amd.h"), os.path.join(PKG, "csrc", "swe_kernels.h")]
+CSRC = os.path.join(PKG, "csrc")
+SOURCES = [os.path.join(CSRC, f) for f in ("swe_kernels.hip", "swe_engine.cpp", "swe_capi.cpp", "swe_setup.cpp")]
+HEADERS = [os.path.join(REPO, "include", "swe_amd.h")] + [os.path.join(CSRC, f) for f in
+                                                          ("swe_kernels.h", "swe_layout.h", "swe_engine.h")]
 
 
 def hipcc():

@@ -1,0 +1,179 @@
+// swe_engine.h -- host side of the MI355X step: the pool (an HBM arena of instances), its
+// allocators, the launch wrappers of the kernels and the step sequence.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "swe_amd.h"
+#include "swe_layout.h"
+
+namespace swe {
+
+// ---- errors --------------------------------------------------------------------------------
+int fail(int code, const char* fmt, ...);
+const char* last_error();
+void clear_error();
+
+#define SWE_HIPCHK(x)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return ::swe::fail(SWE_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- launch schedules (chosen per kernel by the autotuner) -----------------------------------
+enum KernelId : int { KID_DT = 0, KID_ALPHA, KID_STAGE_LO, KID_FCT_LO, KID_RMINUS, KID_APPLY, KID_EV, KID_COUNT };
+extern const char* const kKernelNames[KID_COUNT];
+
+struct LaunchCfg {
+    int block = 256;    // threads per workgroup (whole waves; 4 slices at 256)
+    int remap = 1;      // XCD-aware workgroup order
+};
+struct Tuning {
+    LaunchCfg k[KID_COUNT];
+};
+// Workgroup sizes a kernel can be launched with.
+bool block_supported(int kid, int block);
+
+// ---- first-fit allocator over [0, cap) --------------------------------------------------------
+struct RangeAlloc {
+    std::map<int64_t, int64_t> free_;   // start -> length
+    int64_t cap = 0;
+    void init(int64_t c);
+    int64_t alloc(int64_t n);           // start, or -1
+    void release(int64_t start, int64_t n);
+    int64_t high_water() const;         // end of the last allocated range
+    int64_t largest_free() const;
+};
+
+// ---- host layout of one mesh ------------------------------------------------------------------
+struct HostLayout {
+    int np = 0, nnz = 0, np_pad = 0, nsl = 0, wmax = 0;
+    int64_t nslot = 0;
+    std::vector<int> perm, iperm, len, dpos, col;     // col: local internal numbering
+    std::vector<int64_t> sptr, eoff, csr2slot;        // local slots / list offsets
+    std::vector<double2> c, nv;
+    std::vector<double> nr, m, z;
+    std::vector<uint16_t> elist;
+    std::vector<int> lists[3];                        // Dirichlet lists, internal numbering
+};
+int validate_mesh(const swe_mesh_desc& d);
+int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L);
+int make_consts(const swe_params& p, Consts& K);
+
+// ---- the pool ---------------------------------------------------------------------------------
+struct InstHost {
+    bool used = false;
+    int np = 0, nnz = 0, np_pad = 0, wmax = 0;
+    int64_t slice0 = 0, nslices = 0, slot0 = 0, nslots = 0, e0 = 0, ne = 0, bc0 = 0, nbc = 0;
+    int nh = 0, nux = 0, nuy = 0;
+    swe_params prm{};
+};
+
+struct Mode {
+    int viscosity_type = SWE_VISCOUS, viscous_type = 2, alpha = 0, roundoff = 1;
+};
+int mode_of(const swe_params& p, Mode& m);
+
+struct TimedKernel {
+    std::vector<hipEvent_t> ev;   // start/stop pairs
+    int used = 0;
+};
+
+struct Pool {
+    int device = 0;
+    hipStream_t stream = nullptr;     // steps and every ordered edit of the tables
+    hipStream_t copy = nullptr;       // bulk uploads of admitted instances
+    hipEvent_t released = nullptr;    // recorded on `stream` when ranges are given back
+    bool mode_set = false;
+    Mode mode{};
+    int sort_window = 64;
+    // capacities and allocators
+    int64_t rows_cap = 0, slots_cap = 0, elist_cap = 0, bc_cap = 0;
+    int inst_cap = 0;
+    RangeAlloc a_slices, a_slots, a_elist, a_bc;
+    std::vector<InstHost> inst;
+    int wmax = 0;
+    // device arrays: mesh
+    SliceInfo* sl = nullptr;
+    int* len = nullptr;
+    int* dpos = nullptr;
+    int* perm = nullptr;              // internal row -> caller's node (local numbering)
+    double* m = nullptr;
+    double* z = nullptr;
+    int* col = nullptr;
+    double2* c = nullptr;
+    double2* nvec = nullptr;
+    double* nrm = nullptr;
+    uint16_t* elist = nullptr;
+    // states and work arrays
+    Node* S[3] = {nullptr, nullptr, nullptr};   // S[0] holds U^n between steps
+    Node* scratch = nullptr;
+    Node* ulo = nullptr;
+    double *d0 = nullptr, *dL = nullptr, *muL = nullptr, *ework = nullptr;
+    double *alpha = nullptr, *res = nullptr, *Rm = nullptr, *nwork = nullptr, *stage3 = nullptr;
+    // instances and boundaries
+    InstDev* tab = nullptr;
+    unsigned long long* vmax = nullptr;
+    int* flags = nullptr;
+    BcEntry* bc = nullptr;
+    double* bcv = nullptr;
+    InstDev* tab_host = nullptr;      // pinned read-back of the table
+    int* flags_host = nullptr;        // pinned read-back of the flags
+    // schedules, graph, timing
+    Tuning tune;
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    int64_t graph_key[6] = {-1, -1, -1, -1, -1, -1};
+    bool timing = false;
+    std::string timed_name;
+    TimedKernel tk;
+
+    int nrows() const { return (int)(a_slices.high_water() * 64); }
+    int ninst() const;                // 1 + highest used instance index
+    int64_t nbc() const { return a_bc.high_water(); }
+};
+
+struct PoolCaps {
+    int64_t rows = 0, slots = 0, elist = 0, bc = 0;
+    int instances = 1;
+};
+size_t pool_bytes(const PoolCaps& caps);
+int pool_create(const PoolCaps& caps, Pool** out);
+void pool_destroy(Pool* P);
+// Admit a mesh: bulk data goes up on the copy stream; the instance starts stepping at the next
+// step queued after this call.  un may be null (state set later).  keep receives the layout
+// when the caller needs its maps (csr2slot, lists).
+int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const double* un, double t0,
+             double tfinal, long long max_steps, int* id, HostLayout* keep);
+int pool_remove(Pool* P, int id);
+int pool_set_params(Pool* P, int id, const swe_params& prm);
+int pool_set_bc(Pool* P, int id, int kind, double amp, double period, double level);
+int pool_set_field(Pool* P, int id, size_t offset, const void* src, size_t bytes);
+int pool_upload_state(Pool* P, int id, const double* un, Node* dst);
+int pool_download_state(Pool* P, int id, const Node* src, double* un);
+int pool_queue_steps(Pool* P, int nsteps);
+int pool_read_table(Pool* P);         // sync, then tab_host / flags_host hold the table
+int pool_tune(Pool* P, int steps, int trials);
+
+// ---- launch wrappers (swe_kernels.hip) --------------------------------------------------------
+Mesh mesh_view(const Pool& P);
+void launch_compute_dt(Pool& P, const Node* U, double* d0, int all);
+void launch_finalize_dt(Pool& P, int advance);
+int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage);
+void launch_bdy(Pool& P, Node* u, int stage);
+void launch_dij_full(Pool& P, const Node* U, double* dij);
+void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst);
+void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3);
+void launch_perm_nodes(Pool& P, const InstHost& I, const double* src, double* dst);
+void launch_csr_to_sell(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz);
+void launch_sell_to_csr(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz);
+void launch_fctpos(Pool& P, const double* f, const double* us, double htiny, double* lij);
+void launch_fill_slices(Pool& P, int64_t s0, int64_t n, int inst);
+void launch_fill_bc(Pool& P, int64_t b0, int64_t n);
+int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, const double* us, double htiny,
+                      double* lij);
+
+}  // namespace swe

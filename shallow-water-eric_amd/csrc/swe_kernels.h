@@ -6,36 +6,15 @@
 //   lambda_vacc    compute_lambda_vacc         SOURCES/condlim.f90:879-957
 //   mu_of          compute_muij (one entry)    SOURCES/update.f90:338-356
 //   pow43          MAX(h,htiny)**(4/3) of friction, SOURCES/update.f90:437
+//   sin_rn         sin rounded to nearest, for the inflow boundary value
 // Fortran MAX(a,b) is (a > b ? a : b) and MIN(a,b) is (a < b ? a : b), as flang lowers them.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "swe_layout.h"
+
 namespace swe {
-
-// One node of a state buffer: (h, qx, qy) and the bathymetry z, 32 bytes, so a neighbour gather
-// is one aligned 32-byte load.
-struct alignas(32) Node {
-    double h, qx, qy, z;
-};
-
-// Scalars fixed per parameter set, computed on the host with the same expressions the
-// reference evaluates at run time (host sqrt is correctly rounded).
-struct Consts {
-    double g, htiny, cfl, max_water_h, mannings;
-    double x0;          // (2*SQRT(2)-1)**2
-    double a_c2;        // 1/(2*SQRT(2))
-    double two_sqrt_g;  // 2*SQRT(g)
-    double sqrt_2_g;    // SQRT(2/g)
-    double g_half;      // g/2
-    double g16;         // 16*g
-    double fric;        // g*mannings**2
-    double ent_cst;     // 1e-10*g*max_water_h**2
-    int type_test;
-    int check_h;        // check_hmin is active for type_test 1..13
-};
-
-enum : int { FLAG_NEG_H = 1, FLAG_LAMBDA = 2 };
 
 __device__ __forceinline__ double fmaxF(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double fminF(double a, double b) { return (a < b) ? a : b; }
@@ -179,6 +158,65 @@ __host__ __device__ __forceinline__ double pow43(double x) {
     dd_t p = dd_mul(M, y);            // m^(4/3)
     // x^(4/3) = m^(4/3) * 2^(4k); the final rounding happens in hi+lo below
     return ldexp(p.hi + p.lo, 4 * k);
+}
+
+// ---- sin(x) rounded to nearest, for the inflow boundary value --------------------------------
+// The host's libm sin (called where the boundary value is evaluated on the CPU) returns the
+// correctly rounded result in all but rare near-halfway cases.  sin_rn evaluates the same value
+// on the device: k = nearest integer to x*2/pi, r = x - k*pi/2 in double-double with pi/2 split
+// into three doubles (k*P0 is exact through fma, and x - k*P0 is exact by Sterbenz' lemma), then
+// the Taylor series of sin(r) or cos(r) (|r| <= pi/4, 15 terms, coefficients (-1)^n/(2n+1)! and
+// (-1)^n/(2n)! as double-double) in double-double Horner form, rounded once.  The error before
+// the final rounding is below 2^-100 relative for |x| < 2^29.
+__host__ __device__ inline double sin_rn(double x) {
+    if (!(fabs(x) < 536870912.0)) return sin(x);     // outside the reduction's range; NaN, inf
+    const double k = rint(x * 0.6366197723675814);   // 2/pi
+    const dd_t p0 = dd_two_prod(k, 1.5707963267948966);
+    const dd_t p1 = dd_two_prod(k, 6.123233995736766e-17);
+    dd_t r = {x - p0.hi, 0.0};
+    r = dd_add(r, {-p0.lo, 0.0});
+    r = dd_add(r, {-p1.hi, -p1.lo});
+    r = dd_add(r, {-(k * -1.4973849048591698e-33), 0.0});
+    const dd_t r2 = dd_mul(r, r);
+    const int q = (int)((long long)k & 3);
+    double v;
+    if (q & 1) {
+        dd_t s = {3.279889237069838e-30, 1.5117542744029879e-46};
+        s = dd_add(dd_mul(s, r2), {-2.4795962632247976e-27, 1.2953730964765229e-43});
+        s = dd_add(dd_mul(s, r2), {1.6117375710961184e-24, -3.6846573564509766e-41});
+        s = dd_add(dd_mul(s, r2), {-8.896791392450574e-22, 7.911402614872376e-38});
+        s = dd_add(dd_mul(s, r2), {4.110317623312165e-19, 1.4412973378659527e-36});
+        s = dd_add(dd_mul(s, r2), {-1.5619206968586225e-16, -1.1910679660273754e-32});
+        s = dd_add(dd_mul(s, r2), {4.779477332387385e-14, 4.399205485834081e-31});
+        s = dd_add(dd_mul(s, r2), {-1.1470745597729725e-11, -2.0655512752830745e-28});
+        s = dd_add(dd_mul(s, r2), {2.08767569878681e-09, -1.20734505911326e-25});
+        s = dd_add(dd_mul(s, r2), {-2.755731922398589e-07, -2.3767714622250297e-23});
+        s = dd_add(dd_mul(s, r2), {2.48015873015873e-05, 2.1511947866775882e-23});
+        s = dd_add(dd_mul(s, r2), {-0.001388888888888889, 5.300543954373577e-20});
+        s = dd_add(dd_mul(s, r2), {0.041666666666666664, 2.3129646346357427e-18});
+        s = dd_add(dd_mul(s, r2), {-0.5, 0.0});
+        s = dd_add(dd_mul(s, r2), {1.0, 0.0});
+        v = s.hi + s.lo;
+    } else {
+        dd_t s = {1.1309962886447716e-31, 1.0498015412959506e-47};
+        s = dd_add(dd_mul(s, r2), {-9.183689863795546e-29, -1.4303150396787322e-45});
+        s = dd_add(dd_mul(s, r2), {6.446950284384474e-26, -1.9330404233703465e-42});
+        s = dd_add(dd_mul(s, r2), {-3.868170170630684e-23, 8.843177655482344e-40});
+        s = dd_add(dd_mul(s, r2), {1.9572941063391263e-20, -1.3643503830087908e-36});
+        s = dd_add(dd_mul(s, r2), {-8.22063524662433e-18, -2.2141894119604265e-34});
+        s = dd_add(dd_mul(s, r2), {2.8114572543455206e-15, 1.6508842730861433e-31});
+        s = dd_add(dd_mul(s, r2), {-7.647163731819816e-13, -7.03872877733453e-30});
+        s = dd_add(dd_mul(s, r2), {1.6059043836821613e-10, 1.2585294588752098e-26});
+        s = dd_add(dd_mul(s, r2), {-2.505210838544172e-08, 1.448814070935912e-24});
+        s = dd_add(dd_mul(s, r2), {2.7557319223985893e-06, -1.858393274046472e-22});
+        s = dd_add(dd_mul(s, r2), {-0.0001984126984126984, -1.7209558293420705e-22});
+        s = dd_add(dd_mul(s, r2), {0.008333333333333333, 1.1564823173178714e-19});
+        s = dd_add(dd_mul(s, r2), {-0.16666666666666666, -9.25185853854297e-18});
+        s = dd_add(dd_mul(s, r2), {1.0, 0.0});
+        s = dd_mul(s, r);
+        v = s.hi + s.lo;
+    }
+    return (q & 2) ? -v : v;
 }
 
 // XCD-aware block order: MI355X dispatches workgroup b to XCD (b mod 8); each XCD gets a

@@ -1,0 +1,1188 @@
+// swe_kernels.hip -- MI355X (gfx950) kernels of the explicit shallow-water step and their launch
+// wrappers.
+//
+// One SSP-RK3 step (SOURCES/main.f90:111-135) runs as a short chain of row kernels over every
+// instance of a pool at once.  A kernel gives each CSR row to one lane, walks the row's entries
+// in their stored order and accumulates every sum sequentially, so each node's result is bitwise
+// the reference's:
+//
+//   k_compute_dt    COMPUTE_DT + compute_dij  (main.f90:210-218, update.f90:291-326); keeps d_ij
+//                   for the first stage and reduces max |d_ii|/m_i per instance
+//   k_finalize_dt   dt = CFL/max per instance, the stage times of bdy, time = to + dt
+//   k_alpha         alpha_limit, first loop   (update.f90:758-781)
+//   k_stage_lo      first-order stage: compute_dij/compute_muij/MAX/alpha scaling, smb_2_roundoff
+//                   (or smb_2, smb_1), sources, divide_by_lumped, update, RK combination
+//                   (update.f90:180-223 and main.f90:121-133)
+//   k_fct_lo        FCT stage, pass 1: low-order solution as above plus entropy_residual
+//                   (update.f90:825-878) for the high-order viscosity
+//   k_fct_rminus    FCT stage, pass 2: compute_fct_matrix + first loop of FCT_positivity
+//                   (update.f90:721-748, fct.f90:15-30)
+//   k_fct_apply     FCT stage, pass 3: l_ij (fct.f90:31-41), limited viscosities
+//                   (update.f90:278-279), apply_viscosity (684-718), update, RK combination
+//   k_ev_update     entropy-viscosity-only stage, pass 2 (update.f90:241-262)
+//   k_bdy           bdy (main.f90:220-232)
+//
+// Every row kernel starts the same way: the lane's row, the wave's instance (one per 64-row
+// slice, read once and kept in scalar registers), and an exit for free slices, instances that do
+// not take this step and padding rows.
+//
+// Diagonal entries contribute to the reference's sums only through terms that multiply an exact
+// zero (U_i*rat_ii - U_i*rat_ii); those products are skipped, which can change nothing but the
+// sign of a zero.
+#include <hip/hip_runtime.h>
+
+#include "swe_engine.h"
+#include "swe_kernels.h"
+
+namespace swe {
+
+namespace {
+
+enum SmbKind : int { SMB_ROUNDOFF = 0, SMB_2 = 1, SMB_1 = 2, SMB_GALERKIN = 3 };
+
+constexpr double kPi = 3.141592653589793;   // ACOS(-1.d0)
+
+template <int BS>
+__device__ __forceinline__ int lane_row(int remap) {
+    const int b = remap ? xcd_block() : (int)blockIdx.x;
+    return b * BS + (int)threadIdx.x;
+}
+
+// Instance of the wave holding row i (rows i..i|63 share it); -1 for a free slice.
+__device__ __forceinline__ int wave_inst(const Mesh& M, int i) {
+    return __builtin_amdgcn_readfirstlane(M.sl[i >> 6].inst);
+}
+
+__device__ __forceinline__ bool takes_steps(const InstDev& D, int flags) {
+    return D.state == INST_RUNNING && flags == 0 && D.time < D.tfinal && D.steps < D.max_steps;
+}
+
+__device__ __forceinline__ int64_t row_base(const Mesh& M, int i) {
+    return M.sl[i >> 6].sbase + (i & 63);
+}
+
+__device__ __forceinline__ void raise_flag(int* flags, int inst, int f) {
+    if (f) atomicOr(flags + inst, f);
+}
+
+// LDS of one wave for the compacted lambda phase: per-row one_over_h and SQRT(g*ABS(h)), then
+// one double per slot of the slice (slot offset q*64 + lane).
+__device__ __forceinline__ double* wave_lds(double* smem, int wmax) {
+    return smem + (size_t)(threadIdx.x >> 6) * (size_t)(2 + wmax) * 64;
+}
+
+// RK combination of SOURCES/main.f90:127,133 (stage 0 keeps the Euler output)
+__device__ __forceinline__ double rk_combine(int stage, double uo, double ui) {
+    if (stage == 1) return (3.0 * uo + ui) / 4.0;
+    if (stage == 2) return (uo + 2.0 * ui) / 3.0;
+    return ui;
+}
+
+// Source terms of one node, dispatched on type_test (update.f90:614-623): friction (425-452),
+// my_friction (454-469), coriolis (471-486).  v1, v2 = velocity at U.  src is uniform per wave.
+__device__ __forceinline__ void add_sources(int src, const Node& I, double mi, double v1, double v2, double dt,
+                                            const Consts& K, double& r2, double& r3) {
+    if (src == SRC_MANNING) {
+        const double hs = pow43(fmaxF(I.h, K.htiny));
+        const double vel = K.fric * sqrt(v1 * v1 + v2 * v2);
+        const double hloc = 1.0 * vel * dt;
+        const double den = hs + fmaxF(hs, hloc);
+        r2 = r2 - mi * I.qx * vel * 2.0 / den;
+        r3 = r3 - mi * I.qy * vel * 2.0 / den;
+    } else if (src == SRC_LINEAR) {
+        r2 = r2 - mi * 0.2 * I.qx;
+        r3 = r3 - mi * 0.2 * I.qy;
+    } else if (src == SRC_CORIOLIS) {
+        const double f = 2.0;
+        r2 = r2 + mi * f * I.qy;
+        r3 = r3 - mi * f * I.qx;
+    }
+}
+
+// dlam is compute_dij's value for this entry (from the dt pass or computed inline).
+template <bool ALPHA>
+__device__ __forceinline__ void first_order_dmu(const double2 c, double v1i, double v2i, double v1j, double v2j,
+                                                double dlam, double ai, const double* __restrict__ alpha, int j,
+                                                double& d, double& mu) {
+    d = dlam;
+    mu = mu_of(c.x, c.y, v1i, v2i, v1j, v2j);
+    d = fmaxF(d, mu);
+    if constexpr (ALPHA) {
+        const double s = fmaxF(ai, alpha[j]);
+        d = d * s;
+        mu = mu * s;
+    }
+}
+
+// =============================================================================================
+// COMPUTE_DT: d_ij for every off-diagonal entry, row sums, max |d_ii|/m_i per instance.
+// Phase 0 (one lane per row) puts the row's one_over_h and SQRT(g*ABS(h)) in LDS and an exact 0.0
+// in the diagonal slot (what the reference's row sum adds there); phase 1 (entry-parallel over
+// the slice's compact off-diagonal list) evaluates d_ij = |c_ij| lambda_ij into LDS (and d0);
+// phase 2 sums each row in entry order.  all = 1 (host-array calls) covers every admitted
+// instance; otherwise only the instances that take the step.
+// =============================================================================================
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flags, const Node* __restrict__ U,
+             double* __restrict__ d0, unsigned long long* __restrict__ vmax, int* __restrict__ flags_out, int all,
+             int remap) {
+    extern __shared__ double smem[];
+    double* lw = wave_lds(smem, M.wmax);
+    const int i = lane_row<BS>(remap);
+    const int lane = (int)(threadIdx.x & 63);
+    bool wave_live = false;
+    int inst = -1;
+    if (i < M.nrows) {
+        inst = wave_inst(M, i);
+        if (inst >= 0) wave_live = all ? (T[inst].state != INST_FREE) : takes_steps(T[inst], flags[inst]);
+    }
+    const bool valid = wave_live && M.len[i] > 0;
+    int err = 0;
+    Consts K{};
+    if (wave_live) K = T[inst].K;
+    Node I{};
+    if (valid) {
+        I = U[i];
+        lw[lane] = ovh_of(I.h, K.htiny);
+        lw[64 + lane] = sqrt(K.g * fabs(I.h));
+        lw[128 + 64 * M.dpos[i] + lane] = 0.0;
+    }
+    __syncthreads();
+    if (wave_live) {
+        const int slice = i >> 6;
+        const SliceInfo S = M.sl[slice];
+        const int64_t e1 = S.ebase + S.ecount;
+        for (int64_t e = S.ebase + lane; e < e1; e += 64) {
+            const int off = M.elist[e];
+            const int r = off & 63;
+            const int64_t slot = S.sbase + off;
+            const Node Ir = U[slice * 64 + r];
+            const Node J = U[M.col[slot]];
+            const double d = dij_of(M.n[slot], M.nrm[slot], Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
+                                    sqrt(K.g * fabs(J.h)), K, err);
+            lw[128 + off] = d;
+            if (d0) d0[slot] = d;
+        }
+    }
+    __syncthreads();
+    double v = 0.0;
+    if (valid) {
+        // -SUM(dij%aa(ia(i):ia(i+1)-1)) with the diagonal slot holding 0.0, in entry order
+        double s = 0.0;
+        const int len = M.len[i];
+        for (int q = 0; q < len; ++q) s = s + lw[128 + 64 * q + lane];
+        v = fabs(-s) / M.m[i];
+    }
+    if (err) atomicOr(flags_out + inst, err);
+    // wave max, then per block one atomic per run of waves of the same instance (v >= 0 orders
+    // like its bit pattern)
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(b, off, 64);
+        b = (o > b) ? o : b;
+    }
+    __shared__ unsigned long long red[BS / 64];
+    __shared__ int rinst[BS / 64];
+    if (lane == 0) {
+        red[threadIdx.x >> 6] = b;
+        rinst[threadIdx.x >> 6] = wave_live ? inst : -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int cur = -1;
+        unsigned long long mx = 0ull;
+        for (int w = 0; w < BS / 64; ++w) {
+            if (rinst[w] != cur) {
+                if (cur >= 0) atomicMax(vmax + cur, mx);
+                cur = rinst[w];
+                mx = red[w];
+            } else {
+                mx = (red[w] > mx) ? red[w] : mx;
+            }
+        }
+        if (cur >= 0) atomicMax(vmax + cur, mx);
+    }
+}
+
+// Per-instance step bookkeeping after the dt pass (SOURCES/main.f90:61-72,111-135):
+// an instance takes the step while time < tfinal (and its step limit and error flags allow);
+// dt = CFL*1/MAXVAL(ABS(d_ii)/m_i); the boundary values of the three stages are evaluated at
+// t+dt, (t+dt)+dt/2 and (t+dt/2)+dt; time = to + dt.  advance = 0 only computes dt.
+__global__ void k_finalize_dt(InstDev* __restrict__ T, unsigned long long* __restrict__ vmax,
+                              const int* __restrict__ flags, int ninst, int advance) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= ninst) return;
+    InstDev& D = T[k];
+    if (D.state == INST_FREE) return;
+    const double mx = __longlong_as_double((long long)vmax[k]);
+    vmax[k] = 0ull;
+    if (!advance) {
+        D.dt = D.K.cfl * 1.0 / mx;
+        return;
+    }
+    if (D.state != INST_RUNNING) {
+        D.go = 0;
+        return;
+    }
+    if (!takes_steps(D, flags[k])) {
+        D.go = 0;
+        D.state = flags[k] ? INST_FAILED : INST_DONE;
+        return;
+    }
+    const double dt = D.K.cfl * 1.0 / mx;
+    const double to = D.time;
+    D.dt = dt;
+    if (D.bc_kind == SWE_BC_INFLOW) {
+        double t = to;
+        const double ts0 = t + dt;
+        t = to + dt;
+        const double ts1 = t + dt / 2;
+        t = to + dt / 2;
+        const double ts2 = t + dt;
+        D.bc_val[0] = D.bc_amp * sin_rn(2.0 * kPi * ts0 / D.bc_period);
+        D.bc_val[1] = D.bc_amp * sin_rn(2.0 * kPi * ts1 / D.bc_period);
+        D.bc_val[2] = D.bc_amp * sin_rn(2.0 * kPi * ts2 / D.bc_period);
+    }
+    if (D.dt_log) D.dt_log[D.steps % D.dt_cap] = dt;
+    D.time = to + dt;
+    D.steps = D.steps + 1;
+    D.go = 1;
+}
+
+// compute_dij with the full dij%aa (diagonal = -row sum), for swe_compute_dij.
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ dij,
+           int* __restrict__ flags_out) {
+    const int i = (int)blockIdx.x * BS + (int)threadIdx.x;
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || T[inst].state == INST_FREE) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const Consts K = T[inst].K;
+    int err = 0;
+    const Node I = U[i];
+    const double ovh_i = ovh_of(I.h, K.htiny);
+    const double sql = sqrt(K.g * fabs(I.h));
+    double s = 0.0;
+    int64_t pd = -1;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = M.col[p];
+        if (j == i) { pd = p; dij[p] = 0.0; continue; }
+        const Node J = U[j];
+        const double d = dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
+        dij[p] = d;
+        s = s + d;
+    }
+    if (pd >= 0) dij[pd] = -s;
+    raise_flag(flags_out, inst, err);
+}
+
+// =============================================================================================
+// alpha_limit, first loop: alpha_i = (|sum(h_j-h_i)| / sum|h_j-h_i|)**8, 1 if dry.
+// =============================================================================================
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ alpha, int remap) {
+    const int i = lane_row<BS>(remap);
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || !T[inst].go) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const double htiny = T[inst].K.htiny;
+    const double hi = U[i].h;
+    double a = 1.0;
+    if (!(hi <= htiny)) {
+        const int64_t pb = row_base(M, i);
+        // all index loads, then all gathers, then the sums in entry order
+        constexpr int kUnroll = 12;
+        int js[kUnroll];
+        double hs[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) js[q] = (q < len) ? M.col[pb + 64 * q] : i;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) hs[q] = U[js[q]].h;
+        double num = 0.0, denom = 0.0;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            if (q < len) {
+                num = num + hs[q] - hi;
+                denom = denom + fabs(hs[q] - hi);
+            }
+        }
+        for (int q = kUnroll; q < len; ++q) {
+            const double hj = U[M.col[pb + 64 * (int64_t)q]].h;
+            num = num + hj - hi;
+            denom = denom + fabs(hj - hi);
+        }
+        if (denom <= fabs(num)) a = 1.0;
+        else a = fabs(num) / (denom);
+        a = a * a;
+        a = a * a;
+        a = a * a;
+    }
+    alpha[i] = a;
+}
+
+// =============================================================================================
+// First-order stage ('viscous', or 'galerkin'): one pass per stage.
+// =============================================================================================
+template <int BS, int SMB, bool ALPHA>
+__global__ void __launch_bounds__(BS)
+k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
+           Node* __restrict__ out, const double* __restrict__ d0, const double* __restrict__ alpha, int stage,
+           int* __restrict__ flags_out, int remap) {
+    const int i = lane_row<BS>(remap);
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || !T[inst].go) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const Consts K = T[inst].K;
+    const double dt = T[inst].dt;
+    int err = 0;
+    const Node I = U[i];
+    const double sql = sqrt(K.g * fabs(I.h));
+    const double mi = M.m[i];
+    const double ovh_i = ovh_of(I.h, K.htiny);
+    const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
+    const double ai = ALPHA ? alpha[i] : 1.0;
+    const double thr = (SMB == SMB_ROUNDOFF || SMB == SMB_1) ? K.htiny : 0.0;
+    double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = M.col[p];
+        const Node J = U[j];
+        const double2 c = M.c[p];
+        const double ovh_j = ovh_of(J.h, K.htiny);
+        const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
+        const double zmax = fmaxF(I.z, J.z);
+        const double Hij = fmaxF(0.0, I.h + I.z - zmax);
+        const double Hji = fmaxF(0.0, J.h + J.z - zmax);
+        const double ratij = (I.h <= thr) ? 0.0 : Hij / I.h;
+        const double ratji = (J.h <= thr) ? 0.0 : Hji / J.h;
+        double d = 0.0, mu = 0.0;
+        const bool off = (j != i);
+        if (SMB != SMB_GALERKIN && off)
+            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
+                                   d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   ai, alpha, j, d, mu);
+        if constexpr (SMB == SMB_1) {
+            // smb_1 (update.f90:378-409)
+            double xx = 0.0;
+            xx = xx - c.x * (v1j * J.h * ratji + v1i * I.h * ratij);
+            xx = xx - c.y * (v2j * J.h * ratji + v2i * I.h * ratij);
+            r1 = r1 + xx;
+            if (off) r1 = r1 + d * (J.h * ratji - I.h * ratij);
+            xx = 0.0;
+            xx = xx - c.x * (v1j * J.qx * ratji + v1i * I.qx * ratij);
+            xx = xx - c.y * (v2j * J.qx * ratji + v2i * I.qx * ratij);
+            r2 = r2 + xx;
+            if (off) r2 = r2 + d * (J.qx * ratji - I.qx * ratij);
+            xx = 0.0;
+            xx = xx - c.x * (v1j * J.qy * ratji + v1i * I.qy * ratij);
+            xx = xx - c.y * (v2j * J.qy * ratji + v2i * I.qy * ratij);
+            r3 = r3 + xx;
+            if (off) r3 = r3 + d * (J.qy * ratji - I.qy * ratij);
+            if (off) {
+                const double t = 0.5 * K.g * (Hji * Hji - Hij * Hij);
+                r2 = r2 - t * c.x;
+                r3 = r3 - t * c.y;
+            }
+        } else {
+            if constexpr (SMB == SMB_ROUNDOFF) {
+                // smb_2_roundoff (update.f90:566-611): mass equation with fix_roundoff
+                double xx = 0.0;
+                xx = xx - c.x * (v1j);
+                xx = xx - c.y * (v2j);
+                if (!off) {
+                    fix = fix + xx;
+                } else {
+                    fix = fix - mu + (d - mu) * (-ratij);
+                    r1 = r1 + (xx + mu) * J.h + fmaxF((d - mu) * Hji, 0.0);
+                }
+            } else {
+                // smb_2 (update.f90:656-663), k = 1
+                double xx = 0.0;
+                xx = xx - c.x * (v1j * J.h);
+                xx = xx - c.y * (v2j * J.h);
+                r1 = r1 + xx;
+                if (SMB != SMB_GALERKIN && off)
+                    r1 = r1 + d * (J.h * ratji - I.h * ratij) + mu * (J.h * (1.0 - ratji) - I.h * (1.0 - ratij));
+            }
+            double xx2 = 0.0;
+            xx2 = xx2 - c.x * (v1j * J.qx);
+            xx2 = xx2 - c.y * (v2j * J.qx);
+            double xx3 = 0.0;
+            xx3 = xx3 - c.x * (v1j * J.qy);
+            xx3 = xx3 - c.y * (v2j * J.qy);
+            r2 = r2 + xx2;
+            r3 = r3 + xx3;
+            if (SMB != SMB_GALERKIN && off) {
+                r2 = r2 + d * (J.qx * ratji - I.qx * ratij) + mu * (J.qx * (1.0 - ratji) - I.qx * (1.0 - ratij));
+                r3 = r3 + d * (J.qy * ratji - I.qy * ratij) + mu * (J.qy * (1.0 - ratji) - I.qy * (1.0 - ratij));
+            }
+            const double topo = K.g * I.h * (J.h + J.z - I.h - I.z);
+            r2 = r2 - topo * c.x;
+            r3 = r3 - topo * c.y;
+        }
+    }
+    add_sources(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
+    r1 = r1 / mi;
+    r2 = r2 / mi;
+    r3 = r3 / mi;
+    double h;
+    if constexpr (SMB == SMB_1) h = I.h + dt * r1;
+    else if constexpr (SMB == SMB_ROUNDOFF) h = I.h * (1.0 + dt * fix / mi) + dt * r1;
+    else h = fmaxF(I.h + dt * r1, 0.0);
+    const double qx = I.qx + dt * r2;
+    const double qy = I.qy + dt * r3;
+    Node o;
+    if (stage == 0) {
+        o.h = h; o.qx = qx; o.qy = qy;
+    } else {
+        const Node P = Uo[i];
+        o.h = rk_combine(stage, P.h, h);
+        o.qx = rk_combine(stage, P.qx, qx);
+        o.qy = rk_combine(stage, P.qy, qy);
+    }
+    o.z = I.z;
+    out[i] = o;
+    raise_flag(flags_out, inst, err);
+}
+
+// =============================================================================================
+// FCT / entropy-viscosity stage, pass 1: first-order viscosities (kept in dL, muL), the
+// low-order solution (smb_2_roundoff, update) and the entropy residual res_i.
+// =============================================================================================
+template <int BS, bool ALPHA, bool WRITE_LO>
+__global__ void __launch_bounds__(BS)
+k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node* __restrict__ ulo,
+         double* __restrict__ dL, double* __restrict__ muL, double* __restrict__ res, const double* __restrict__ d0,
+         const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
+    const int i = lane_row<BS>(remap);
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || !T[inst].go) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const Consts K = T[inst].K;
+    const double dt = T[inst].dt;
+    int err = 0;
+    const Node I = U[i];
+    const double sql = sqrt(K.g * fabs(I.h));
+    const double mi = M.m[i];
+    const double ovh_i = ovh_of(I.h, K.htiny);
+    const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
+    const double ai = ALPHA ? alpha[i] : 1.0;
+    const double thr_lo = roundoff ? K.htiny : 0.0;
+    double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;   // smb_2_roundoff (or smb_2)
+    double e1 = 0.0, e2 = 0.0, e3 = 0.0;              // smb_2 of entropy_residual
+    double emax = 0.0, emin = 0.0;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = M.col[p];
+        const Node J = U[j];
+        const double2 c = M.c[p];
+        const double ovh_j = ovh_of(J.h, K.htiny);
+        const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
+        const double zmax = fmaxF(I.z, J.z);
+        const double Hij = fmaxF(0.0, I.h + I.z - zmax);
+        const double Hji = fmaxF(0.0, J.h + J.z - zmax);
+        const double qij = Hij / I.h, qji = Hji / J.h;
+        const double ratij0 = (I.h <= 0.0) ? 0.0 : qij;
+        const double ratji0 = (J.h <= 0.0) ? 0.0 : qji;
+        const double ratijr = (I.h <= thr_lo) ? 0.0 : qij;
+        const double ratjir = (J.h <= thr_lo) ? 0.0 : qji;
+        const bool off = (j != i);
+        double d = 0.0, mu = 0.0;
+        if (off) {
+            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
+                                   d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   ai, alpha, j, d, mu);
+            dL[p] = d;
+            muL[p] = mu;
+        }
+        double xm = 0.0;
+        xm = xm - c.x * (v1j);
+        xm = xm - c.y * (v2j);
+        double xe = 0.0;
+        xe = xe - c.x * (v1j * J.h);
+        xe = xe - c.y * (v2j * J.h);
+        double xx2 = 0.0;
+        xx2 = xx2 - c.x * (v1j * J.qx);
+        xx2 = xx2 - c.y * (v2j * J.qx);
+        double xx3 = 0.0;
+        xx3 = xx3 - c.x * (v1j * J.qy);
+        xx3 = xx3 - c.y * (v2j * J.qy);
+        const double topo = K.g * I.h * (J.h + J.z - I.h - I.z);
+        if (WRITE_LO) {
+            if (roundoff) {
+                if (!off) fix = fix + xm;
+                else {
+                    fix = fix - mu + (d - mu) * (-ratijr);
+                    r1 = r1 + (xm + mu) * J.h + fmaxF((d - mu) * Hji, 0.0);
+                }
+            } else {
+                r1 = r1 + xe;
+                if (off) r1 = r1 + d * (J.h * ratjir - I.h * ratijr) + mu * (J.h * (1.0 - ratjir) - I.h * (1.0 - ratijr));
+            }
+            r2 = r2 + xx2;
+            r3 = r3 + xx3;
+            if (off) {
+                r2 = r2 + d * (J.qx * ratjir - I.qx * ratijr) + mu * (J.qx * (1.0 - ratjir) - I.qx * (1.0 - ratijr));
+                r3 = r3 + d * (J.qy * ratjir - I.qy * ratijr) + mu * (J.qy * (1.0 - ratjir) - I.qy * (1.0 - ratijr));
+            }
+            r2 = r2 - topo * c.x;
+            r3 = r3 - topo * c.y;
+        }
+        e1 = e1 + xe;
+        e2 = e2 + xx2;
+        e3 = e3 + xx3;
+        if (off) {
+            e1 = e1 + d * (J.h * ratji0 - I.h * ratij0) + mu * (J.h * (1.0 - ratji0) - I.h * (1.0 - ratij0));
+            e2 = e2 + d * (J.qx * ratji0 - I.qx * ratij0) + mu * (J.qx * (1.0 - ratji0) - I.qx * (1.0 - ratij0));
+            e3 = e3 + d * (J.qy * ratji0 - I.qy * ratij0) + mu * (J.qy * (1.0 - ratji0) - I.qy * (1.0 - ratij0));
+        }
+        e2 = e2 - topo * c.x;
+        e3 = e3 - topo * c.y;
+        // entropy of the neighbour for maxmin (update.f90:924-935)
+        double Ej = K.g * (J.h * J.h);
+        Ej = Ej + v1j * J.qx;
+        Ej = Ej + v2j * J.qy;
+        Ej = 0.5 * Ej;
+        if (q == 0) { emax = Ej; emin = Ej; }
+        else {
+            emax = (Ej > emax) ? Ej : emax;
+            emin = (Ej < emin) ? Ej : emin;
+        }
+    }
+    add_sources(K.src, I, mi, v1i, v2i, dt, K, e2, e3);
+    if (WRITE_LO) {
+        add_sources(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
+        r1 = r1 / mi;
+        r2 = r2 / mi;
+        r3 = r3 / mi;
+        Node o;
+        if (roundoff) o.h = I.h * (1.0 + dt * fix / mi) + dt * r1;
+        else o.h = fmaxF(I.h + dt * r1, 0.0);
+        o.qx = I.qx + dt * r2;
+        o.qy = I.qy + dt * r3;
+        o.z = I.z;
+        ulo[i] = o;
+        if (K.check_h && o.h < 0.0) err |= FLAG_NEG_H;
+    }
+    // entropy_residual (update.f90:838-868)
+    e1 = e1 / mi;
+    e2 = e2 / mi;
+    e3 = e3 / mi;
+    const double ht = I.h + dt * e1;
+    const double qxt = I.qx + dt * e2;
+    const double qyt = I.qy + dt * e3;
+    const double ovt = ovh_of(ht, K.htiny);
+    const double vn1 = qxt * ovt, vn2 = qyt * ovt;
+    double E = K.g * (I.h * I.h);
+    double En = K.g * (ht * ht);
+    E = E + v1i * I.qx;
+    En = En + vn1 * qxt;
+    E = E + v2i * I.qy;
+    En = En + vn2 * qyt;
+    E = 0.5 * E;
+    En = 0.5 * En;
+    double Ep0 = K.g * I.h;
+    Ep0 = Ep0 - 0.5 * (v1i * v1i);
+    Ep0 = Ep0 - 0.5 * (v2i * v2i);
+    double r = mi * (En - E) / dt;
+    r = r - mi * e1 * Ep0;
+    r = r - mi * e2 * v1i;
+    r = r - mi * e3 * v2i;
+    const double rescale = fabs(emax - emin) / 2.0 + K.ent_cst;
+    res[i] = fabs(r) / rescale;
+    raise_flag(flags_out, inst, err);
+}
+
+// Antidiffusive flux f_ij of compute_fct_matrix (update.f90:732-745) for an off-diagonal entry,
+// with the high-order viscosities MIN(dL, resij), MIN(muL, resij) of update.f90:233-239.
+__device__ __forceinline__ double fct_flux(const Node& I, const Node& J, double dLp, double muLp,
+                                           double rij, double& dH, double& muH, double& ratij, double& ratji) {
+    dH = fminF(dLp, 1.0 * rij);
+    muH = fminF(muLp, 1.0 * rij);
+    const double zmax = fmaxF(I.z, J.z);
+    const double Hij = fmaxF(0.0, I.h + I.z - zmax);
+    const double Hji = fmaxF(0.0, J.h + J.z - zmax);
+    ratij = (I.h <= 0.0) ? 0.0 : Hij / I.h;
+    ratji = (J.h <= 0.0) ? 0.0 : Hji / J.h;
+    return (dH - dLp) * (J.h * ratji - I.h * ratij) + (muH - muLp) * (J.h * (1.0 - ratji) - I.h * (1.0 - ratij));
+}
+
+// =============================================================================================
+// FCT stage, pass 2: P-, Q-, R- of FCT_positivity (fct.f90:15-30).
+// =============================================================================================
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ ulo,
+             const double* __restrict__ dL, const double* __restrict__ muL, const double* __restrict__ res,
+             double* __restrict__ Rm, int remap) {
+    const int i = lane_row<BS>(remap);
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || !T[inst].go) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const double dt = T[inst].dt;
+    const double htiny = T[inst].K.htiny;
+    const Node I = U[i];
+    const double resi = res[i];
+    const double scaled = (M.m[i] / dt) * ulo[i].h;
+    const double Q = 0.0 - scaled;
+    double P = -1.0e-15;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = M.col[p];
+        if (j == i) continue;    // f_ii is a zero, never < 0
+        const Node J = U[j];
+        double dH, muH, rij_, rji_;
+        const double f = fct_flux(I, J, dL[p], muL[p], fmaxF(resi, res[j]), dH, muH, rij_, rji_);
+        if (f < 0.0) P = P + f;
+    }
+    double R = fminF(Q / P, 1.0);
+    if (scaled <= htiny) R = 0.0;
+    Rm[i] = R;
+}
+
+// =============================================================================================
+// FCT stage, pass 3: l_ij, limited viscosities, apply_viscosity, update, RK combination.
+// =============================================================================================
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
+            const Node* __restrict__ ulo, Node* __restrict__ out, const double* __restrict__ dL,
+            const double* __restrict__ muL, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
+            int* __restrict__ flags_out, int remap) {
+    const int i = lane_row<BS>(remap);
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || !T[inst].go) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const double dt = T[inst].dt;
+    const int check_h = T[inst].K.check_h;
+    const Node I = U[i];
+    const double resi = res[i];
+    const double Ri = Rm[i];
+    double x1 = 0.0, x2 = 0.0, x3 = 0.0;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = M.col[p];
+        if (j == i) continue;    // contributes d*(0) + mu*(0)
+        const Node J = U[j];
+        const double dLp = dL[p], muLp = muL[p];
+        double dH, muH, ratij, ratji;
+        const double f = fct_flux(I, J, dLp, muLp, fmaxF(resi, res[j]), dH, muH, ratij, ratji);
+        const double l = (f >= 0.0) ? Rm[j] : Ri;
+        const double d = (dH - dLp) * l;
+        const double mu = (muH - muLp) * l;
+        x1 = x1 + d * (J.h * ratji - I.h * ratij) + mu * (J.h * (1.0 - ratji) - I.h * (1.0 - ratij));
+        x2 = x2 + d * (J.qx * ratji - I.qx * ratij) + mu * (J.qx * (1.0 - ratji) - I.qx * (1.0 - ratij));
+        x3 = x3 + d * (J.qy * ratji - I.qy * ratij) + mu * (J.qy * (1.0 - ratji) - I.qy * (1.0 - ratij));
+    }
+    const double mi = M.m[i];
+    const Node L = ulo[i];      // low-order solution of k_fct_lo
+    const double h = L.h + dt * (x1 / mi);
+    const double qx = L.qx + dt * (x2 / mi);
+    const double qy = L.qy + dt * (x3 / mi);
+    const int err = (check_h && h < 0.0) ? FLAG_NEG_H : 0;
+    Node o;
+    if (stage == 0) {
+        o.h = h; o.qx = qx; o.qy = qy;
+    } else {
+        const Node P = Uo[i];
+        o.h = rk_combine(stage, P.h, h);
+        o.qx = rk_combine(stage, P.qx, qx);
+        o.qy = rk_combine(stage, P.qy, qy);
+    }
+    o.z = I.z;
+    out[i] = o;
+    raise_flag(flags_out, inst, err);
+}
+
+// =============================================================================================
+// Entropy-viscosity-only stage, pass 2 (update.f90:241-262).
+// =============================================================================================
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
+            Node* __restrict__ out, const double* __restrict__ dL, const double* __restrict__ muL,
+            const double* __restrict__ res, int roundoff, int stage, int* __restrict__ flags_out, int remap) {
+    const int i = lane_row<BS>(remap);
+    if (i >= M.nrows) return;
+    const int inst = wave_inst(M, i);
+    if (inst < 0 || !T[inst].go) return;
+    const int len = M.len[i];
+    if (len == 0) return;
+    const Consts K = T[inst].K;
+    const double dt = T[inst].dt;
+    const Node I = U[i];
+    const double mi = M.m[i];
+    const double ovh_i = ovh_of(I.h, K.htiny);
+    const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
+    const double resi = res[i];
+    const double thr = roundoff ? K.htiny : 0.0;
+    double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = M.col[p];
+        const Node J = U[j];
+        const double2 c = M.c[p];
+        const double ovh_j = ovh_of(J.h, K.htiny);
+        const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
+        const double zmax = fmaxF(I.z, J.z);
+        const double Hij = fmaxF(0.0, I.h + I.z - zmax);
+        const double Hji = fmaxF(0.0, J.h + J.z - zmax);
+        const double ratij = (I.h <= thr) ? 0.0 : Hij / I.h;
+        const double ratji = (J.h <= thr) ? 0.0 : Hji / J.h;
+        const bool off = (j != i);
+        double d = 0.0, mu = 0.0;
+        if (off) {
+            const double rij = fmaxF(resi, res[j]);
+            d = fminF(dL[p], 1.0 * rij);
+            mu = fminF(muL[p], 1.0 * rij);
+        }
+        if (roundoff) {
+            double xm = 0.0;
+            xm = xm - c.x * (v1j);
+            xm = xm - c.y * (v2j);
+            if (!off) fix = fix + xm;
+            else {
+                fix = fix - mu + (d - mu) * (-ratij);
+                r1 = r1 + (xm + mu) * J.h + fmaxF((d - mu) * Hji, 0.0);
+            }
+        } else {
+            double xe = 0.0;
+            xe = xe - c.x * (v1j * J.h);
+            xe = xe - c.y * (v2j * J.h);
+            r1 = r1 + xe;
+            if (off) r1 = r1 + d * (J.h * ratji - I.h * ratij) + mu * (J.h * (1.0 - ratji) - I.h * (1.0 - ratij));
+        }
+        double xx2 = 0.0;
+        xx2 = xx2 - c.x * (v1j * J.qx);
+        xx2 = xx2 - c.y * (v2j * J.qx);
+        double xx3 = 0.0;
+        xx3 = xx3 - c.x * (v1j * J.qy);
+        xx3 = xx3 - c.y * (v2j * J.qy);
+        r2 = r2 + xx2;
+        r3 = r3 + xx3;
+        if (off) {
+            r2 = r2 + d * (J.qx * ratji - I.qx * ratij) + mu * (J.qx * (1.0 - ratji) - I.qx * (1.0 - ratij));
+            r3 = r3 + d * (J.qy * ratji - I.qy * ratij) + mu * (J.qy * (1.0 - ratji) - I.qy * (1.0 - ratij));
+        }
+        const double topo = K.g * I.h * (J.h + J.z - I.h - I.z);
+        r2 = r2 - topo * c.x;
+        r3 = r3 - topo * c.y;
+    }
+    add_sources(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
+    r1 = r1 / mi;
+    r2 = r2 / mi;
+    r3 = r3 / mi;
+    double h;
+    if (roundoff) h = I.h * (1.0 + dt * fix / mi) + dt * r1;
+    else h = fmaxF(I.h + dt * r1, 0.0);
+    const double qx = I.qx + dt * r2;
+    const double qy = I.qy + dt * r3;
+    const int err = (K.check_h && h < 0.0) ? FLAG_NEG_H : 0;
+    Node o;
+    if (stage == 0) {
+        o.h = h; o.qx = qx; o.qy = qy;
+    } else {
+        const Node P = Uo[i];
+        o.h = rk_combine(stage, P.h, h);
+        o.qx = rk_combine(stage, P.qx, qx);
+        o.qy = rk_combine(stage, P.qy, qy);
+    }
+    o.z = I.z;
+    out[i] = o;
+    raise_flag(flags_out, inst, err);
+}
+
+// =============================================================================================
+// bdy: Dirichlet overwrites after a stage, for every instance that takes the step.
+// =============================================================================================
+__global__ void k_bdy(Node* __restrict__ u, const BcEntry* __restrict__ E, int n, const double* __restrict__ vals,
+                      const InstDev* __restrict__ T, int stage) {
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= n) return;
+    const BcEntry e = E[q];
+    if (e.inst < 0) return;
+    const InstDev& D = T[e.inst];
+    if (!D.go) return;
+    const int kind = D.bc_kind;
+    if (kind == SWE_BC_NONE) return;
+    Node& x = u[e.node];
+    if (e.comp == 0) {
+        if (kind == SWE_BC_INFLOW) x.h = fmaxF(D.bc_level - x.z + D.bc_val[stage], 0.0);
+        else if (kind == SWE_BC_VALUES) x.h = vals[e.vidx];
+    } else if (e.comp == 1) {
+        x.qx = (kind == SWE_BC_VALUES) ? vals[e.vidx] : 0.0;
+    } else {
+        x.qy = (kind == SWE_BC_VALUES) ? vals[e.vidx] : 0.0;
+    }
+}
+
+// =============================================================================================
+// Layout conversion and table edits
+// =============================================================================================
+// un(3,np) in the caller's numbering <-> internal Node records of one instance; perm maps its
+// internal rows to the caller's nodes.
+__global__ void k_pack(const double* __restrict__ un3, const double* __restrict__ z, const int* __restrict__ perm,
+                       Node* __restrict__ u, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np) return;
+    const int k = perm[r];
+    Node o;
+    o.h = un3[3 * (size_t)k];
+    o.qx = un3[3 * (size_t)k + 1];
+    o.qy = un3[3 * (size_t)k + 2];
+    o.z = z[r];
+    u[r] = o;
+}
+
+__global__ void k_unpack(const Node* __restrict__ u, const int* __restrict__ perm, double* __restrict__ un3, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np) return;
+    const int k = perm[r];
+    const Node o = u[r];
+    un3[3 * (size_t)k] = o.h;
+    un3[3 * (size_t)k + 1] = o.qx;
+    un3[3 * (size_t)k + 2] = o.qy;
+}
+
+__global__ void k_perm_nodes(const double* __restrict__ src, const int* __restrict__ perm, double* __restrict__ dst,
+                             int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r < np) dst[r] = src[perm[r]];
+}
+
+__global__ void k_csr_to_sell(const double* __restrict__ src, const int64_t* __restrict__ map, double* __restrict__ dst,
+                              int64_t nnz) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) dst[map[p]] = src[p];
+}
+
+__global__ void k_sell_to_csr(const double* __restrict__ src, const int64_t* __restrict__ map, double* __restrict__ dst,
+                              int64_t nnz) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) dst[p] = src[map[p]];
+}
+
+__global__ void k_fill_slices(SliceInfo* __restrict__ sl, int64_t s0, int64_t n, int inst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) sl[s0 + k].inst = inst;
+}
+
+__global__ void k_fill_bc(BcEntry* __restrict__ bc, int64_t b0, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) bc[b0 + k].inst = -1;
+}
+
+// FCT_positivity on the pool's sliced layout (fct.f90:15-41): R- per row, then l_ij.
+__global__ void k_fctpos_r(Mesh M, const double* __restrict__ f, const double* __restrict__ us, double htiny,
+                           double* __restrict__ Rm) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= M.nrows || M.len[i] == 0) return;
+    const double Q = 0.0 - us[i];
+    double P = -1.0e-15;
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    for (int q = 0; q < len; ++q) {
+        const double fij = f[pb + 64 * (int64_t)q];
+        if (fij < 0.0) P = P + fij;
+    }
+    double R = fminF(Q / P, 1.0);
+    if (us[i] <= htiny) R = 0.0;
+    Rm[i] = R;
+}
+
+__global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* __restrict__ Rm, double* __restrict__ lij) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= M.nrows || M.len[i] == 0) return;
+    const int64_t pb = row_base(M, i);
+    const int len = M.len[i];
+    const double Ri = Rm[i];
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        lij[p] = (f[p] >= 0.0) ? Rm[M.col[p]] : Ri;
+    }
+}
+
+// The same on a plain 1-based CSR matrix (swe_fct_positivity_csr).
+__global__ void k_fctpos_csr_r(int np, const int* __restrict__ ia, const double* __restrict__ f,
+                               const double* __restrict__ us, double* __restrict__ Rm, double htiny) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    const double Q = 0.0 - us[i];
+    double P = -1.0e-15;
+    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) {
+        const double fij = f[p];
+        if (fij < 0.0) P = P + fij;
+    }
+    double R = fminF(Q / P, 1.0);
+    if (us[i] <= htiny) R = 0.0;
+    Rm[i] = R;
+}
+
+__global__ void k_fctpos_csr_l(int np, const int* __restrict__ ia, const int* __restrict__ ja,
+                               const double* __restrict__ f, const double* __restrict__ Rm, double* __restrict__ lij) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) lij[p] = (f[p] >= 0.0) ? Rm[ja[p] - 1] : Rm[i];
+}
+
+inline unsigned blocks_of(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Dispatch a kernel template over the supported workgroup sizes.
+#define SWE_BS_SWITCH(bs, ...)                    \
+    switch (bs) {                                 \
+        case 128: {                               \
+            constexpr int BS = 128;               \
+            __VA_ARGS__;                          \
+        } break;                                  \
+        case 512: {                               \
+            constexpr int BS = 512;               \
+            __VA_ARGS__;                          \
+        } break;                                  \
+        default: {                                \
+            constexpr int BS = 256;               \
+            __VA_ARGS__;                          \
+        } break;                                  \
+    }
+
+void timed_begin(Pool& P, int kid) {
+    if (!P.timing || P.timed_name != kKernelNames[kid]) return;
+    TimedKernel& T = P.tk;
+    if ((size_t)(2 * T.used + 2) > T.ev.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            T.ev.push_back(e);
+        }
+    }
+    (void)hipEventRecord(T.ev[(size_t)2 * T.used], P.stream);
+}
+
+void timed_end(Pool& P, int kid) {
+    if (!P.timing || P.timed_name != kKernelNames[kid]) return;
+    TimedKernel& T = P.tk;
+    (void)hipEventRecord(T.ev[(size_t)2 * T.used + 1], P.stream);
+    T.used++;
+}
+
+template <int SMB, bool ALPHA>
+void launch_stage_lo(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage) {
+    const Mesh M = mesh_view(P);
+    const LaunchCfg& L = P.tune.k[KID_STAGE_LO];
+    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+                                              P.stream, M, (const InstDev*)P.tab, U, Uo, out, d0,
+                                              (const double*)P.alpha, stage, P.flags, L.remap))
+}
+
+template <bool ALPHA, bool WRITE_LO>
+void launch_fct_lo(Pool& P, const Node* U, Node* ulo, const double* d0) {
+    const Mesh M = mesh_view(P);
+    const LaunchCfg& L = P.tune.k[KID_FCT_LO];
+    const int ro = P.mode.roundoff;
+    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO>), dim3(blocks_of(M.nrows, BS)), dim3(BS),
+                                              0, P.stream, M, (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
+                                              (const double*)P.alpha, ro, P.flags, L.remap))
+}
+
+}  // namespace
+
+const char* const kKernelNames[KID_COUNT] = {"k_compute_dt", "k_alpha", "k_stage_lo", "k_fct_lo",
+                                             "k_fct_rminus", "k_fct_apply", "k_ev_update"};
+
+bool block_supported(int kid, int block) {
+    (void)kid;
+    return block == 128 || block == 256 || block == 512;
+}
+
+Mesh mesh_view(const Pool& P) {
+    return Mesh{P.sl, P.len, P.col, P.c, P.nvec, P.nrm, P.m, P.elist, P.dpos, P.nrows(), P.wmax};
+}
+
+void launch_compute_dt(Pool& P, const Node* U, double* d0, int all) {
+    const Mesh M = mesh_view(P);
+    if (M.nrows == 0) return;
+    const LaunchCfg& L = P.tune.k[KID_DT];
+    // the dynamic LDS of a workgroup stays within 64 KiB: fall back to smaller workgroups
+    int block = L.block;
+    while (block > 128 && (size_t)(2 + M.wmax) * 64 * sizeof(double) * (block / 64) > 65536) block /= 2;
+    timed_begin(P, KID_DT);
+    SWE_BS_SWITCH(block, {
+        const size_t lds = (size_t)(2 + M.wmax) * 64 * sizeof(double) * (BS / 64);
+        hipLaunchKernelGGL((k_compute_dt<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), lds, P.stream, M,
+                           (const InstDev*)P.tab, (const int*)P.flags, U, d0, P.vmax, P.flags, all, L.remap);
+    })
+    timed_end(P, KID_DT);
+}
+
+void launch_finalize_dt(Pool& P, int advance) {
+    const int n = P.ninst();
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_finalize_dt, dim3(blocks_of(n, 64)), dim3(64), 0, P.stream, P.tab, P.vmax,
+                       (const int*)P.flags, n, advance);
+}
+
+// One Euler stage from U to out (out != U) for every instance that takes the step.  d0 (may be
+// null) holds compute_dij at U.  stage: 0 keeps the Euler output, 1 and 2 apply the RK
+// combinations with Uo.
+int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage) {
+    const Mode& md = P.mode;
+    const Mesh M = mesh_view(P);
+    if (M.nrows == 0) return SWE_OK;
+    const InstDev* T = P.tab;
+    const bool alpha = md.alpha != 0 && md.viscosity_type != SWE_GALERKIN;
+    if (alpha) {
+        const LaunchCfg& L = P.tune.k[KID_ALPHA];
+        timed_begin(P, KID_ALPHA);
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_alpha<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0, P.stream,
+                                                  M, T, U, P.alpha, L.remap))
+        timed_end(P, KID_ALPHA);
+    }
+    if (md.viscosity_type == SWE_GALERKIN) {
+        launch_stage_lo<SMB_GALERKIN, false>(P, U, Uo, out, nullptr, stage);
+    } else if (md.viscosity_type == SWE_VISCOUS) {
+        const int smb = (md.viscous_type == 1) ? SMB_1 : (md.roundoff ? SMB_ROUNDOFF : SMB_2);
+        timed_begin(P, KID_STAGE_LO);
+        if (smb == SMB_1) {
+            if (alpha) launch_stage_lo<SMB_1, true>(P, U, Uo, out, d0, stage);
+            else launch_stage_lo<SMB_1, false>(P, U, Uo, out, d0, stage);
+        } else if (smb == SMB_ROUNDOFF) {
+            if (alpha) launch_stage_lo<SMB_ROUNDOFF, true>(P, U, Uo, out, d0, stage);
+            else launch_stage_lo<SMB_ROUNDOFF, false>(P, U, Uo, out, d0, stage);
+        } else {
+            if (alpha) launch_stage_lo<SMB_2, true>(P, U, Uo, out, d0, stage);
+            else launch_stage_lo<SMB_2, false>(P, U, Uo, out, d0, stage);
+        }
+        timed_end(P, KID_STAGE_LO);
+    } else if (md.viscosity_type == SWE_FCT) {
+        timed_begin(P, KID_FCT_LO);
+        if (alpha) launch_fct_lo<true, true>(P, U, P.ulo, d0);
+        else launch_fct_lo<false, true>(P, U, P.ulo, d0);
+        timed_end(P, KID_FCT_LO);
+        {
+            const LaunchCfg& L = P.tune.k[KID_RMINUS];
+            timed_begin(P, KID_RMINUS);
+            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_rminus<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+                                                      P.stream, M, T, U, (const Node*)P.ulo, (const double*)P.dL,
+                                                      (const double*)P.muL, (const double*)P.res, P.Rm, L.remap))
+            timed_end(P, KID_RMINUS);
+        }
+        {
+            const LaunchCfg& L = P.tune.k[KID_APPLY];
+            timed_begin(P, KID_APPLY);
+            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_apply<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+                                                      P.stream, M, T, U, Uo, (const Node*)P.ulo, out,
+                                                      (const double*)P.dL, (const double*)P.muL, (const double*)P.res,
+                                                      (const double*)P.Rm, stage, P.flags, L.remap))
+            timed_end(P, KID_APPLY);
+        }
+    } else {  // entropy_visc
+        if (alpha) launch_fct_lo<true, false>(P, U, out, d0);
+        else launch_fct_lo<false, false>(P, U, out, d0);
+        const LaunchCfg& L = P.tune.k[KID_EV];
+        timed_begin(P, KID_EV);
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_ev_update<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+                                                  P.stream, M, T, U, Uo, out, (const double*)P.dL,
+                                                  (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
+                                                  P.flags, L.remap))
+        timed_end(P, KID_EV);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SWE_EHIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return SWE_OK;
+}
+
+void launch_bdy(Pool& P, Node* u, int stage) {
+    const int64_t n = P.nbc();
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_bdy, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, u, (const BcEntry*)P.bc, (int)n,
+                       (const double*)P.bcv, (const InstDev*)P.tab, stage);
+}
+
+void launch_dij_full(Pool& P, const Node* U, double* dij) {
+    const Mesh M = mesh_view(P);
+    if (M.nrows == 0) return;
+    hipLaunchKernelGGL((k_dij_full<256>), dim3(blocks_of(M.nrows, 256)), dim3(256), 0, P.stream, M,
+                       (const InstDev*)P.tab, U, dij, P.flags);
+}
+
+void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst) {
+    const int64_t r0 = I.slice0 * 64;
+    hipLaunchKernelGGL(k_pack, dim3(blocks_of(I.np, 256)), dim3(256), 0, P.stream, un3, (const double*)(P.z + r0),
+                       (const int*)(P.perm + r0), dst + r0, I.np);
+}
+
+void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3) {
+    const int64_t r0 = I.slice0 * 64;
+    hipLaunchKernelGGL(k_unpack, dim3(blocks_of(I.np, 256)), dim3(256), 0, P.stream, src + r0,
+                       (const int*)(P.perm + r0), un3, I.np);
+}
+
+void launch_perm_nodes(Pool& P, const InstHost& I, const double* src, double* dst) {
+    const int64_t r0 = I.slice0 * 64;
+    hipLaunchKernelGGL(k_perm_nodes, dim3(blocks_of(I.np, 256)), dim3(256), 0, P.stream, src,
+                       (const int*)(P.perm + r0), dst + r0, I.np);
+}
+
+void launch_csr_to_sell(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz) {
+    hipLaunchKernelGGL(k_csr_to_sell, dim3(blocks_of(nnz, 256)), dim3(256), 0, P.stream, src, map, dst, nnz);
+}
+
+void launch_sell_to_csr(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz) {
+    hipLaunchKernelGGL(k_sell_to_csr, dim3(blocks_of(nnz, 256)), dim3(256), 0, P.stream, src, map, dst, nnz);
+}
+
+void launch_fctpos(Pool& P, const double* f, const double* us, double htiny, double* lij) {
+    const Mesh M = mesh_view(P);
+    if (M.nrows == 0) return;
+    hipLaunchKernelGGL(k_fctpos_r, dim3(blocks_of(M.nrows, 256)), dim3(256), 0, P.stream, M, f, us, htiny, P.Rm);
+    hipLaunchKernelGGL(k_fctpos_l, dim3(blocks_of(M.nrows, 256)), dim3(256), 0, P.stream, M, f, (const double*)P.Rm,
+                       lij);
+}
+
+void launch_fill_slices(Pool& P, int64_t s0, int64_t n, int inst) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_slices, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, P.sl, s0, n, inst);
+}
+
+void launch_fill_bc(Pool& P, int64_t b0, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_bc, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, P.bc, b0, n);
+}
+
+int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, const double* us, double htiny,
+                      double* lij) {
+    double* dR = nullptr;
+    hipError_t e = hipMalloc((void**)&dR, sizeof(double) * (size_t)np);
+    if (e != hipSuccess) return fail(SWE_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+    const unsigned g = blocks_of(np, 256);
+    hipLaunchKernelGGL(k_fctpos_csr_r, dim3(g), dim3(256), 0, 0, np, ia, f, us, dR, htiny);
+    hipLaunchKernelGGL(k_fctpos_csr_l, dim3(g), dim3(256), 0, 0, np, ia, ja, f, (const double*)dR, lij);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dR);
+    if (e != hipSuccess) return fail(SWE_EHIP, "FCT_positivity kernels: %s", hipGetErrorString(e));
+    return SWE_OK;
+}
+
+}  // namespace swe

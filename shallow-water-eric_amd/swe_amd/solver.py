@@ -16,7 +16,10 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libswe_amd.so")
 
-SWE_OK, SWE_ENEG_H, SWE_ELAMBDA, SWE_EBADOPT, SWE_EHIP, SWE_EINVAL, SWE_ENOMEM = range(7)
+SWE_OK, SWE_ENEG_H, SWE_ELAMBDA, SWE_EBADOPT, SWE_EHIP, SWE_EINVAL, SWE_ENOMEM, SWE_EFULL = range(8)
+INSTANCE_RUNNING, INSTANCE_DONE, INSTANCE_FAILED = 1, 2, 3
+NUM_KERNELS = 7
+KERNELS = ("k_compute_dt", "k_alpha", "k_stage_lo", "k_fct_lo", "k_fct_rminus", "k_fct_apply", "k_ev_update")
 GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
 BC_NONE, BC_WALLS, BC_INFLOW, BC_VALUES = 0, 1, 2, 3
 VISCOSITY = {"galerkin": GALERKIN, "viscous": VISCOUS, "entropy_visc": ENTROPY_VISC, "fct": FCT}
@@ -33,6 +36,11 @@ class Params(C.Structure):
                 ("max_water_h", C.c_double), ("mannings", C.c_double),
                 ("type_test", C.c_int), ("viscosity_type", C.c_int), ("viscous_type", C.c_int),
                 ("alpha_limit", C.c_int), ("roundoff_fix", C.c_int)]
+
+
+class BatchCaps(C.Structure):
+    _fields_ = [("max_nodes", C.c_longlong), ("max_entries", C.c_longlong), ("max_boundary", C.c_longlong),
+                ("max_instances", C.c_int)]
 
 
 class MeshDesc(C.Structure):
@@ -70,6 +78,28 @@ SIGNATURES = {
     "swe_stream": (_P, [_P]),
     "swe_enable_kernel_timing": (_I, [_P, C.c_char_p]),
     "swe_kernel_timing": (_I, [_P, C.c_char_p, C.POINTER(_I), C.POINTER(_D)]),
+    "swe_tune": (_I, [_P, _I]),
+    "swe_get_schedule": (_I, [_P, _P, _P]),
+    "swe_set_schedule": (_I, [_P, _P, _P]),
+    "swe_use_graph": (_I, [_P, _I]),
+    "swe_batch_bytes": (C.c_ulonglong, [C.POINTER(BatchCaps)]),
+    "swe_batch_create": (_I, [C.POINTER(BatchCaps), C.POINTER(_P)]),
+    "swe_batch_destroy": (_I, [_P]),
+    "swe_batch_add": (_I, [_P, C.POINTER(MeshDesc), C.POINTER(Params), _P, _D, _D, C.c_longlong, C.POINTER(_I)]),
+    "swe_batch_set_bc": (_I, [_P, _I, _I, _D, _D, _D]),
+    "swe_batch_set_bc_values": (_I, [_P, _I, _P, _P, _P]),
+    "swe_batch_remove": (_I, [_P, _I]),
+    "swe_batch_step": (_I, [_P, _I]),
+    "swe_batch_sync": (_I, [_P]),
+    "swe_batch_status": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_D), C.POINTER(C.c_longlong), C.POINTER(_I)]),
+    "swe_batch_poll": (_I, [_P, _P, _I, C.POINTER(_I)]),
+    "swe_batch_get_state": (_I, [_P, _I, _P]),
+    "swe_batch_free": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(_I)]),
+    "swe_batch_tune": (_I, [_P, _I]),
+    "swe_batch_use_graph": (_I, [_P, _I]),
+    "swe_batch_stream": (_P, [_P]),
+    "swe_batch_enable_kernel_timing": (_I, [_P, C.c_char_p]),
+    "swe_batch_kernel_timing": (_I, [_P, C.c_char_p, C.POINTER(_I), C.POINTER(_D)]),
 }
 
 _lib = None
@@ -137,6 +167,34 @@ def fct_positivity_csr(ia, ja, unext_scaled, f, htiny):
     return lij
 
 
+def make_params(*, gravity, htiny, cfl, max_water_h, viscosity="viscous", viscous_type=2, alpha_limit=False,
+                type_test=5, mannings=0.0, roundoff_fix=True) -> Params:
+    vt = VISCOSITY[viscosity] if isinstance(viscosity, str) else int(viscosity)
+    return Params(gravity, htiny, cfl, max_water_h, mannings, type_test, vt, viscous_type,
+                  int(bool(alpha_limit)), int(bool(roundoff_fix)))
+
+
+class MeshArrays:
+    """A mesh in the C-ABI's form (1-based CSR, c_ij split, Dirichlet lists), kept alive with the
+    descriptor that points into it."""
+
+    def __init__(self, ia, ja, cij, lumped, bath, h_dir=(), ux_dir=(), uy_dir=()):
+        self.ia1 = np.ascontiguousarray(ia, np.int32) + 1
+        self.ja1 = np.ascontiguousarray(ja, np.int32) + 1
+        self.np = self.ia1.size - 1
+        self.nnz = self.ja1.size
+        cij = np.asarray(cij, np.float64).reshape(self.nnz, 2)
+        self.cx = np.ascontiguousarray(cij[:, 0])
+        self.cy = np.ascontiguousarray(cij[:, 1])
+        self.lumped = np.ascontiguousarray(lumped, np.float64)
+        self.bath = np.ascontiguousarray(bath, np.float64)
+        self.lists = [np.ascontiguousarray(x, np.int32) + 1 for x in (h_dir, ux_dir, uy_dir)]
+        self.desc = MeshDesc(self.np, self.nnz, _ptr(self.ia1), _ptr(self.ja1), _ptr(self.cx), _ptr(self.cy),
+                             _ptr(self.lumped), _ptr(self.bath),
+                             self.lists[0].size, self.lists[1].size, self.lists[2].size,
+                             _ptr(self.lists[0]), _ptr(self.lists[1]), _ptr(self.lists[2]))
+
+
 class Solver:
     """One mesh, one parameter set, one device-resident state.
 
@@ -149,25 +207,14 @@ class Solver:
                  viscosity="viscous", viscous_type=2, alpha_limit=False, type_test=5, mannings=0.0,
                  roundoff_fix=True, h_dir=(), ux_dir=(), uy_dir=()):
         lib = load_library()
-        self.ia1 = np.ascontiguousarray(ia, np.int32) + 1
-        self.ja1 = np.ascontiguousarray(ja, np.int32) + 1
-        self.np = self.ia1.size - 1
-        self.nnz = self.ja1.size
-        cij = np.asarray(cij, np.float64).reshape(self.nnz, 2)
-        self._cx = np.ascontiguousarray(cij[:, 0])
-        self._cy = np.ascontiguousarray(cij[:, 1])
-        self._lumped = np.ascontiguousarray(lumped, np.float64)
-        self._bath = np.ascontiguousarray(bath, np.float64)
-        self._lists = [np.ascontiguousarray(x, np.int32) + 1 for x in (h_dir, ux_dir, uy_dir)]
-        vt = VISCOSITY[viscosity] if isinstance(viscosity, str) else int(viscosity)
-        self.params = Params(gravity, htiny, cfl, max_water_h, mannings, type_test, vt, viscous_type,
-                             int(bool(alpha_limit)), int(bool(roundoff_fix)))
-        desc = MeshDesc(self.np, self.nnz, _ptr(self.ia1), _ptr(self.ja1), _ptr(self._cx), _ptr(self._cy),
-                        _ptr(self._lumped), _ptr(self._bath),
-                        self._lists[0].size, self._lists[1].size, self._lists[2].size,
-                        _ptr(self._lists[0]), _ptr(self._lists[1]), _ptr(self._lists[2]))
+        self.mesh = MeshArrays(ia, ja, cij, lumped, bath, h_dir, ux_dir, uy_dir)
+        self.np = self.mesh.np
+        self.nnz = self.mesh.nnz
+        self.params = make_params(gravity=gravity, htiny=htiny, cfl=cfl, max_water_h=max_water_h,
+                                  viscosity=viscosity, viscous_type=viscous_type, alpha_limit=alpha_limit,
+                                  type_test=type_test, mannings=mannings, roundoff_fix=roundoff_fix)
         h = C.c_void_p()
-        _check(lib.swe_create(C.byref(desc), C.byref(self.params), C.byref(h)))
+        _check(lib.swe_create(C.byref(self.mesh.desc), C.byref(self.params), C.byref(h)))
         self.h = h
 
     # -- lifetime -------------------------------------------------------------------------------
@@ -280,3 +327,25 @@ class Solver:
         ms = C.c_double()
         _check(load_library().swe_kernel_timing(self.h, kernel.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    # -- launch schedules ---------------------------------------------------------------------------
+    def tune(self, steps=3):
+        """Autotune the launch schedule of every kernel on this mesh and state (state restored)."""
+        _check(load_library().swe_tune(self.h, int(steps)))
+        return self.schedule
+
+    @property
+    def schedule(self):
+        b = np.zeros(NUM_KERNELS, np.int32)
+        r = np.zeros(NUM_KERNELS, np.int32)
+        _check(load_library().swe_get_schedule(self.h, _ptr(b), _ptr(r)))
+        return {k: (int(b[i]), int(r[i])) for i, k in enumerate(KERNELS)}
+
+    @schedule.setter
+    def schedule(self, sched):
+        b = np.array([sched[k][0] for k in KERNELS], np.int32)
+        r = np.array([sched[k][1] for k in KERNELS], np.int32)
+        _check(load_library().swe_set_schedule(self.h, _ptr(b), _ptr(r)))
+
+    def use_graph(self, on=True):
+        _check(load_library().swe_use_graph(self.h, int(bool(on))))

@@ -1,0 +1,176 @@
+"""Batches on the GPU: every instance of a batch must give bitwise the result of its mesh run
+alone (and, through the fixtures, the reference's), whatever else shares the batch, whenever it
+was admitted, whatever launch schedule or graph replay ran the steps.  Sizes are small so the
+whole file runs in seconds."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_diff, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def fct_problems():
+    from swe_amd import problems
+    return [problems.dam_break(101, 21, seed=7), problems.bumps(n_side=65, L=1000.0, seed=11),
+            problems.dam_break(61, 13, seed=3), problems.bumps(n_side=33, L=500.0, seed=5, period=7.0)]
+
+
+def run_alone(p, nsteps):
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t, dts = s.run(nsteps, time=0.0)
+        return s.get_state(), t, dts
+
+
+def new_batch(problems_, spare=2):
+    from swe_amd.batch import Batch
+    nodes = sum((p.np + 63) // 64 * 64 for p in problems_) * spare
+    entries = int(sum(p.nnz for p in problems_) * 1.3) * spare + 4096
+    bnd = sum(sum(x.size for x in p.dirichlet()) for p in problems_) * spare + 64
+    return Batch(nodes, entries, bnd, 16)
+
+
+def test_batch_instances_equal_runs_alone():
+    from swe_amd.batch import INSTANCE_DONE
+    ps = fct_problems()
+    n = 40
+    alone = [run_alone(p, n) for p in ps]
+    with new_batch(ps) as b:
+        ids = [b.add_problem(p, max_steps=n) for p in ps]
+        print("batch of", [p.np for p in ps], flush=True)
+        b.step(n + 3)                      # the extra steps find every instance done
+        assert sorted(b.poll()) == sorted(ids)
+        for k, p, (u, t, _) in zip(ids, ps, alone):
+            st, tb, steps, err = b.status(k)
+            assert st == INSTANCE_DONE and steps == n and err == 0
+            assert tb == t
+            assert same_bits(b.get_state(k), u), p.name
+
+
+def test_batch_reproduces_reference_fixture():
+    """The 65x65 bumps run of the reference (200 steps, inflow) as one instance among others."""
+    from swe_amd import problems
+    from swe_amd.batch import INSTANCE_FAILED
+    from swe_amd.solver import SWE_ENEG_H
+    z = golden("bumps_run.npz")
+    p = problems.bumps(n_side=65, L=1000.0, seed=11)
+    assert same_bits(p.u0, z["u0"])
+    others = [problems.dam_break(61, 13, seed=3), problems.bumps(n_side=33, L=500.0, seed=5)]
+    with new_batch([p] + others) as b:
+        o1 = b.add_problem(others[0], max_steps=150)
+        k = b.add_problem(p, max_steps=z["dts"].size)
+        o2 = b.add_problem(others[1], max_steps=400)
+        b.step(int(z["dts"].size))
+        st, t, steps, err = b.status(k)
+        assert steps == z["dts"].size and err == 0
+        assert same_bits(b.get_state(k), z["final"]) and t == float(z["time"])
+        st, t, steps, err = b.status(o1)
+        assert err == 0 and steps == 150
+        # the reference stops this one with 'Min h<0' during step 24 (the oracle agrees): the
+        # batch marks it failed there and the other instances are not affected
+        st, t, steps, err = b.status(o2)
+        assert st == INSTANCE_FAILED and err == SWE_ENEG_H and steps == 24
+
+
+def test_admission_mid_run_and_removal():
+    """An instance admitted while others are mid-run, into rows freed by a removed one, gives the
+    same result as alone; so do the instances it joined."""
+    ps = fct_problems()
+    n = 30
+    alone = {i: run_alone(p, n) for i, p in enumerate(ps)}
+    with new_batch(ps[:3], spare=1) as b:
+        k0 = b.add_problem(ps[0], max_steps=n)
+        k1 = b.add_problem(ps[1], max_steps=n)
+        k2 = b.add_problem(ps[2], max_steps=n)
+        b.step(10)
+        b.remove(k1)                                   # frees the rows in the middle of the arena
+        k3 = b.add_problem(ps[3], max_steps=n)         # the batch is full but for those rows
+        assert k3 is not None
+        b.step(n)
+        for k, i in ((k0, 0), (k2, 2), (k3, 3)):
+            assert same_bits(b.get_state(k), alone[i][0]), i
+            assert b.status(k)[2] == n
+
+
+def test_final_time_stops_like_the_reference_loop():
+    """DO WHILE (time < Tfinal) (SOURCES/main.f90:61): the instance takes steps while its time is
+    below tfinal, so its last step ends at or beyond tfinal."""
+    from swe_amd.batch import INSTANCE_DONE
+    from swe_amd import problems
+    p = problems.dam_break(101, 21, seed=7)
+    u, t40, dts = run_alone(p, 40)
+    tfinal = float(np.cumsum(dts)[24]) - 1e-9           # reached during step 25
+    with new_batch([p]) as b:
+        k = b.add_problem(p, tfinal=tfinal, max_steps=10 ** 9)
+        b.step(60)
+        st, t, steps, err = b.status(k)
+    assert st == INSTANCE_DONE and steps == 25 and t >= tfinal
+    u25, t25, _ = run_alone(p, 25)
+    assert t == t25
+
+
+def test_graph_replay_equals_eager_launches():
+    ps = fct_problems()[:2]
+    res = []
+    for graph in (True, False):
+        with new_batch(ps) as b:
+            b.use_graph(graph)
+            ids = [b.add_problem(p, max_steps=25) for p in ps]
+            for _ in range(5):                         # replays separated by host work and syncs
+                b.step(5)
+                b.sync()
+                _ = b.status(ids[0])
+            res.append([b.get_state(k) for k in ids])
+    for a, c in zip(*res):
+        assert same_bits(a, c)
+
+
+def test_schedules_do_not_change_results():
+    from swe_amd import problems
+    p = problems.bumps(n_side=65, L=1000.0, seed=11)
+    ref, _, _ = run_alone(p, 20)
+    with p.solver() as s:
+        s.set_state(p.u0)
+        sched = s.tune(steps=2)
+        print("tuned schedule", sched, flush=True)
+        assert s.time == 0.0                           # tuning restored the time and the state
+        assert same_bits(s.get_state(), p.u0)
+        s.run(20, time=0.0)
+        assert same_bits(s.get_state(), ref)
+        for blocks, remap in ((128, 0), (512, 1), (256, 0)):
+            s.schedule = {k: (blocks, remap) for k in sched}
+            s.set_state(p.u0)
+            s.run(20, time=0.0)
+            assert same_bits(s.get_state(), ref), (blocks, remap)
+
+
+def test_full_batch_reports_no_room():
+    from swe_amd import problems
+    from swe_amd.batch import Batch
+    p = problems.dam_break(101, 21, seed=7)
+    with Batch((p.np + 63) // 64 * 64, int(p.nnz * 1.3), 1024, 4) as b:
+        assert b.add_problem(p, max_steps=5) is not None
+        assert b.add_problem(p, max_steps=5) is None      # no room for a second copy
+        b.step(5)
+        assert len(b.poll()) == 1
+
+
+def test_mixed_batch_against_oracle(oracle):
+    """Three dam breaks of different sizes, Manning friction, 20 steps, against the oracle."""
+    from swe_amd import problems
+    ps = [problems.dam_break(n, m, seed=s) for n, m, s in ((81, 17, 1), (161, 33, 2), (41, 9, 3))]
+    with new_batch(ps) as b:
+        ids = [b.add_problem(p, max_steps=20) for p in ps]
+        b.step(20)
+        for k, p in zip(ids, ps):
+            hd, uxd, uyd = p.dirichlet()
+            orc = oracle.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=9.81, htiny=1e-4, cfl=0.25,
+                                max_water_h=1.0, viscosity_type=oracle.FCT, alpha_limit=True, type_test=10,
+                                mannings=0.02)
+            ref, _, _, err = orc.run(p.u0, 20, bc_kind=1, nodes_ux=uxd, nodes_uy=uyd)
+            assert err == 0
+            l1, linf = rel_diff(b.get_state(k), ref)
+            assert l1 <= 1e-11 and linf <= 1e-11, (p.np, l1, linf)

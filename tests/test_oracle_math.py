@@ -251,3 +251,45 @@ def test_fct_limiter_bounds(square, seed):
 def test_planted_unlimited_fct_fails_bound(square):
     ia, ja, r, f, us = fct_case(square, 0)
     assert fct_violation(ia, ja, r, f, us, np.ones_like(f)) > 1e-3
+
+
+# --- sin rounded to nearest (inflow boundary value) -------------------------------------------
+
+def sin_exact(x):
+    """sin(x) for a double x, rounded to the nearest double from an 80-digit decimal evaluation."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 80
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459230781640628620899863")
+    X = Decimal(x)
+    k = (X / (pi / 2)).to_integral_value()
+    r = X - k * pi / 2
+    out = []
+    for n0 in (1, 0):                      # sin series, cos series
+        s, t, n = Decimal(0), (r if n0 else Decimal(1)), n0
+        for _ in range(40):                # |r| <= pi/4: 40 terms reach 1e-100
+            s += t
+            t = -t * r * r / ((n + 1) * (n + 2))
+            n += 2
+        out.append(s)
+    s, c = out
+    return float([s, c, -s, -c][int(k) % 4])
+
+
+HARD_SIN = [-3627.9681361144821, 2.3342538793503689, 13.17291631866291, 82.114506176080312,
+            112.9365273696114, 22.529236817273095, 3.3494088846916159]   # libm is 1 ulp off here
+
+
+def test_sin_rn_is_correctly_rounded(oracle):
+    import ctypes as C
+    f = O.lib().orc_sin_rn
+    f.restype, f.argtypes = C.c_double, [C.c_double]
+    rng = np.random.default_rng(5)
+    xs = list(HARD_SIN) + list(2 * np.pi * rng.uniform(0, 400, 600) / 20) + list(rng.uniform(-1e4, 1e4, 300))
+    xs += [k * (np.pi / 2) for k in range(-40, 41)] + [np.nextafter(k * np.pi / 2, 9e9) for k in range(1, 40)]
+    xs += [0.0, -0.0, 1e-300, 5e-324, 1e-8]
+    for x in xs:
+        assert f(float(x)) == sin_exact(float(x)), x
+
+
+def test_planted_libm_sin_fails_rounding_check():
+    assert any(float(np.sin(x)) != sin_exact(x) for x in HARD_SIN)
