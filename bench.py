@@ -129,12 +129,153 @@ def copy_calibration(gib=1, reps=3):
     print("copied %d x %d GiB device to device" % (reps, gib))
 
 
+def _batch_caps(ps, spare_rows=0):
+    rows = sum((p.np + 63) // 64 * 64 for p in ps) + spare_rows
+    entries = int(sum(p.nnz for p in ps) * 1.4) + 64 * 64
+    bnd = sum(sum(x.size for x in p.dirichlet()) for p in ps) + 64
+    return rows, entries, bnd
+
+
+def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=5000, nmax=500000):
+    """Batched form: `count` meshes of the 16.8M-node workload's family, sizes log-uniform in
+    [nmin, nmax], dam-break or still-water-with-inflow initial data, all resident in one batch
+    and advanced together (one captured graph per step, autotuned schedules); against the plain
+    per-instance loop: the same meshes one at a time, each alone on the device."""
+    from swe_amd import problems
+    from swe_amd.batch import Batch
+    t0 = time.perf_counter()
+    ps = problems.build_family(problems.batch_family_specs(count, nmin, nmax, seed=seed), workers)
+    t_gen = time.perf_counter() - t0
+    nodes = sum(p.np for p in ps)
+    rows, entries, bnd = _batch_caps(ps)
+    print("# batch: %d meshes, %d nodes (%d..%d), generated in %.1f s" % (count, nodes, min(p.np for p in ps),
+          max(p.np for p in ps), t_gen), file=sys.stderr, flush=True)
+    total = warmup + steps
+    b = Batch(rows, entries, bnd, count)
+    t0 = time.perf_counter()
+    ids = [b.add_problem(p, max_steps=total) for p in ps]
+    t_admit = time.perf_counter() - t0
+    if tune:
+        b.tune(2)
+    b.step(warmup)
+    b.sync()
+    before = [b.status(k)[2] for k in ids]
+    b.enable_kernel_timing("k_fct_lo")
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    nl, kms = b.kernel_timing("k_fct_lo")
+    b.enable_kernel_timing(None)
+    # the timed steps again without per-kernel events, as one captured graph per step
+    stat = [b.status(k) for k in ids]
+    work = sum(p.np * 3 * (st[2] - n0) for p, st, n0 in zip(ps, stat, before))
+    failed = sum(1 for st in stat if st[3] != 0)
+    b.close()
+    b = Batch(rows, entries, bnd, count)
+    ids = [b.add_problem(p, max_steps=total) for p in ps]
+    if tune:
+        b.tune(2)
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    el_graph = time.perf_counter() - t0
+    b.close()
+    # plain loop: one mesh at a time on a batch with room for the largest
+    big = max(ps, key=lambda p: p.nnz)
+    r1, e1, b1 = _batch_caps([big])
+    one = Batch(r1, e1, max(b1, max(sum(x.size for x in p.dirichlet()) for p in ps) + 64), 1)
+    el_plain, work_plain = 0.0, 0
+    chunk = 10
+    for p in ps:
+        k = one.add_problem(p, max_steps=total)
+        one.step(warmup)
+        one.sync()
+        n0 = one.status(k)[2]
+        t0 = time.perf_counter()
+        left = steps
+        while left > 0:                     # in chunks, so a mesh that stops early stops stepping
+            one.step(min(chunk, left))
+            left -= chunk
+            st = one.status(k)
+            if st[0] != 1:
+                break
+        el_plain += time.perf_counter() - t0
+        work_plain += p.np * 3 * (one.status(k)[2] - n0)
+        one.remove(k)
+    one.close()
+    kavg = sum(p.nnz for p in ps) / nodes
+    bpn = kernel_bytes_per_node("k_fct_lo", kavg, True)
+    launch_s = kms / 1e3 / max(nl, 1)
+    achieved = bpn * nodes / launch_s / 1e9
+    rate_b = work / el_graph
+    rate_p = work_plain / el_plain
+    return {"instances": count, "nodes": nodes, "sizes": [nmin, nmax], "steps": steps, "warmup": warmup,
+            "node_stage_per_s": rate_b, "ms_per_step": el_graph / steps * 1e3,
+            "node_stage_per_s_eager_timed": work / el, "plain_node_stage_per_s": rate_p,
+            "plain_ms_total": el_plain * 1e3, "speedup_vs_plain": rate_b / rate_p, "failed_instances": failed,
+            "admission_s": t_admit,
+            "roofline": {"bound": "hbm", "kernel": "k_fct_lo", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "launch_ms": launch_s * 1e3,
+                         "bytes_per_node": bpn}}
+
+
+def bench_serve(count, rate, workers, chunk=16, budget_gb=200.0, seed=20261202, median=100000, nmax=2000000):
+    """Served form: a Poisson stream of `count` requests at `rate` per second (sizes lognormal with
+    the given median, capped at nmax; final times of 1-20 wave periods; 20% dry-bed dam breaks)
+    admitted into one batch whose capacity is the memory budget, against the same stream served
+    one request at a time.  Both runs must give identical results per request."""
+    from swe_amd import problems
+    from swe_amd.batch import Batch
+    from swe_amd.serve import BatchEngine, Request, Server, run_plain, summarize
+    specs = problems.serve_request_specs(count, rate, seed=seed, median=median, nmax=nmax)
+    t0 = time.perf_counter()
+    ps = problems.build_family(specs, workers)
+    print("# serve: %d requests at %.0f/s, %d nodes in all (max %d), generated in %.1f s"
+          % (count, rate, sum(p.np for p in ps), max(p.np for p in ps), time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+
+    def reqs():
+        return [Request(rid=q, problem=p, tfinal=sp["tfinal"], arrival=sp["arrival"])
+                for q, (p, sp) in enumerate(zip(ps, specs))]
+    budget = budget_gb * (1 << 30)
+    need = Batch.bytes_for(*_batch_caps(ps), count)
+    b = Batch.for_budget(min(budget, 1.05 * need + (1 << 30)), max_instances=max(64, count))
+    served = Server(BatchEngine(b), chunk=chunk).run(reqs())
+    resident = b.caps.max_nodes
+    b.close()
+    big = max(ps, key=lambda p: p.nnz)
+    r1, e1, b1 = _batch_caps([big])
+    one = Batch(r1, e1, max(b1, max(sum(x.size for x in p.dirichlet()) for p in ps) + 64), 1)
+    plain = run_plain(BatchEngine(one), reqs(), chunk=chunk)
+    one.close()
+    same = all(a.steps == c.steps and a.time == c.time and np.array_equal(a.state, c.state, equal_nan=True)
+               for a, c in zip(served, plain))
+    s, pl = summarize(served), summarize(plain)
+    return {"requests": count, "rate_per_s": rate, "budget_gb": budget_gb, "batch_rows": resident,
+            "chunk_steps": chunk, "p50_s": s["p50_s"], "p99_s": s["p99_s"], "mean_s": s["mean_s"],
+            "node_stage_per_s": s["node_stage_per_s"], "makespan_s": s["makespan_s"],
+            "plain_p50_s": pl["p50_s"], "plain_p99_s": pl["p99_s"], "plain_node_stage_per_s": pl["node_stage_per_s"],
+            "speedup_vs_plain": s["node_stage_per_s"] / pl["node_stage_per_s"],
+            "results_identical_to_plain": bool(same), "errors": s["errors"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bumps", choices=["bumps", "dam_break", "paraboloid"])
+    ap.add_argument("--workload", default="bumps", choices=["bumps", "dam_break", "paraboloid", "batch", "serve"])
+    ap.add_argument("--instances", type=int, default=256, help="batch: meshes in the batch")
+    ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
+    ap.add_argument("--rates", default="20", help="serve: arrival rates (per s), comma separated")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget")
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="processes that generate the meshes of the batch and serve workloads")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default run: leave out the reduced batch and serve measurements")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -151,6 +292,24 @@ def main():
 
     from swe_amd.solver import load_library
     load_library()
+    if args.workload == "batch":
+        res = {"metric": "node-stage updates/s (FP64, SSP-RK3, FCT + alpha), batched", "unit": "node-stage updates/s",
+               "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic"}
+        r = bench_batch(args.instances, args.steps, args.warmup, args.workers)
+        res.update({"value": r["node_stage_per_s"], "steps": args.steps, "warmup": args.warmup,
+                    "ms_per_step": r["ms_per_step"], "roofline": r.pop("roofline"),
+                    "config": {"workload": "batch", **r}})
+        print(json.dumps(res))
+        return
+    if args.workload == "serve":
+        for rate in [float(x) for x in args.rates.split(",")]:
+            r = bench_serve(args.requests, rate, args.workers, budget_gb=args.budget_gb)
+            print(json.dumps({"metric": "time-to-solution p99 (served, FP64 FCT + alpha)", "value": r["p99_s"],
+                              "unit": "s", "n_gpus": 1, "higher_is_better": False, "scaling": "weak",
+                              "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                              "config": {"workload": "serve", **r}}), flush=True)
+        return
     t0 = time.perf_counter()
     p = build_problem(args.workload, args.size)
     t_setup = time.perf_counter() - t0
@@ -200,6 +359,10 @@ def main():
     s.close()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    if args.workload == "bumps" and args.size is None and not args.no_extra:
+        # the batched and served forms at reduced sizes, against the plain per-instance loop
+        res["batched"] = bench_batch(64, 50, 3, args.workers)
+        res["served"] = bench_serve(32, 20.0, args.workers)
     print(json.dumps(res))
 
 

@@ -173,7 +173,7 @@ typedef struct swe_batch swe_batch;
 
 typedef struct swe_batch_caps {
     long long max_nodes;      /* node rows (each mesh takes its np rounded up to 64) */
-    long long max_entries;    /* entry slots (a mesh takes about nnz * 1.05) */
+    long long max_entries;    /* entry slots (a mesh takes nnz plus the padding of its 64-row slices, up to ~1.3 nnz) */
     long long max_boundary;   /* Dirichlet list entries (h, ux and uy lists together) */
     int max_instances;        /* meshes resident at once */
 } swe_batch_caps;

@@ -501,15 +501,22 @@ int swe_batch_poll(swe_batch* b, int* ids, int cap, int* n) {
     *n = 0;
     int rc = pool_read_table(b->P);
     if (rc) return rc;
-    const int ni = b->P->ninst();
-    for (int k = 0; k < ni && *n < cap; ++k) {
-        if (!b->P->inst[(size_t)k].used) continue;
-        const InstDev& D = b->P->tab_host[k];
-        const bool over = D.state != INST_RUNNING || b->P->flags_host[k] != 0 ||
+    Pool* P = b->P;
+    const int ni = P->ninst();
+    bool unlisted = false;
+    for (int k = 0; k < ni; ++k) {
+        if (!P->inst[(size_t)k].used) continue;
+        const InstDev& D = P->tab_host[k];
+        const bool over = D.state != INST_RUNNING || P->flags_host[k] != 0 ||
                           !(D.time < D.tfinal && D.steps < D.max_steps);
-        if (over) ids[(*n)++] = k;
+        if (!over) continue;
+        if (P->listed[(size_t)k]) {          // it takes no more steps: its waves leave the launches
+            P->listed[(size_t)k] = 0;
+            unlisted = true;
+        }
+        if (*n < cap) ids[(*n)++] = k;
     }
-    return SWE_OK;
+    return unlisted ? pool_update_active(P) : SWE_OK;
 }
 
 int swe_batch_get_state(swe_batch* b, int id, double* un) {

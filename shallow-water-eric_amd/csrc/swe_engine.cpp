@@ -5,8 +5,9 @@
 // and then, on the compute stream, writes the instance's table entry, which is what makes the
 // step kernels see it.  Removal clears the slice and boundary owners on the compute stream and
 // records an event that the next upload waits for, so a range is never rewritten while queued
-// kernels of its previous owner may still run.  A step is a fixed launch sequence over every
-// slice below the high-water mark; instances that are done, failed or free skip it per wave.
+// kernels of its previous owner may still run.  A step is a fixed launch sequence over the list
+// of listed slices (every admitted instance until it is found done or failed); an instance that
+// stops between list updates skips the step per wave.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -284,6 +285,11 @@ void free_pool(Pool* P) {
                     P->flags, P->bc,    P->bcv};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (P->act) (void)hipFree(P->act);
+    for (int k = 0; k < 2; ++k) {
+        if (P->act_host[k]) (void)hipHostFree(P->act_host[k]);
+        if (P->act_ev[k]) (void)hipEventDestroy(P->act_ev[k]);
+    }
     if (P->tab_host) (void)hipHostFree(P->tab_host);
     if (P->flags_host) (void)hipHostFree(P->flags_host);
     if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
@@ -370,7 +376,18 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     ALLOC(P->flags, (size_t)caps.instances);
     ALLOC(P->bc, (size_t)std::max<int64_t>(caps.bc, 1));
     ALLOC(P->bcv, (size_t)std::max<int64_t>(caps.bc, 1));
+    ALLOC(P->act, (size_t)slices);
 #undef ALLOC
+    P->listed.assign((size_t)caps.instances, 0);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipHostMalloc((void**)&P->act_host[k], sizeof(int) * (size_t)slices, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&P->act_ev[k], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(P->act_ev[k], P->stream);
+    }
+    if (e != hipSuccess) {
+        free_pool(P);
+        return fail(SWE_EHIP, "pool lists: %s", hipGetErrorString(e));
+    }
     e = hipHostMalloc((void**)&P->tab_host, sizeof(InstDev) * (size_t)caps.instances, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&P->flags_host, sizeof(int) * (size_t)caps.instances, hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -520,7 +537,28 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     if (e == hipSuccess) e = hipMemcpyAsync(P->flags + k, &z32, sizeof z32, hipMemcpyHostToDevice, P->stream);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return fail(SWE_EHIP, "activation of an admitted mesh: %s", hipGetErrorString(e));
+    P->listed[(size_t)k] = 1;
+    if ((rc = pool_update_active(P))) return rc;
     *id = k;
+    return SWE_OK;
+}
+
+int pool_update_active(Pool* P) {
+    // wait until the staging buffer's previous upload has been consumed
+    const int b = P->act_buf;
+    P->act_buf ^= 1;
+    SWE_HIPCHK(hipEventSynchronize(P->act_ev[b]));
+    std::vector<std::pair<int64_t, int64_t>> runs;
+    for (int k = 0; k < P->inst_cap; ++k)
+        if (P->inst[(size_t)k].used && P->listed[(size_t)k])
+            runs.emplace_back(P->inst[(size_t)k].slice0, P->inst[(size_t)k].nslices);
+    std::sort(runs.begin(), runs.end());
+    int n = 0;
+    for (const auto& r : runs)
+        for (int64_t s = r.first; s < r.first + r.second; ++s) P->act_host[b][n++] = (int)s;
+    if (n) SWE_HIPCHK(hipMemcpyAsync(P->act, P->act_host[b], sizeof(int) * (size_t)n, hipMemcpyHostToDevice, P->stream));
+    SWE_HIPCHK(hipEventRecord(P->act_ev[b], P->stream));
+    P->nact = n;
     return SWE_OK;
 }
 
@@ -538,6 +576,9 @@ int pool_remove(Pool* P, int k) {
     P->a_elist.release(I.e0, I.ne);
     P->a_bc.release(I.bc0, I.nbc);
     I = InstHost{};
+    P->listed[(size_t)k] = 0;
+    int rc = pool_update_active(P);
+    if (rc) return rc;
     bool any = false;
     for (const auto& q : P->inst) any |= q.used;
     if (!any) {
@@ -621,7 +662,7 @@ int64_t tuning_hash(const Tuning& t) {
 
 // The captured step stays valid while the launch extents, the scheme and the schedules stay.
 int ensure_graph(Pool* P) {
-    const int64_t key[6] = {(int64_t)P->nrows(), (int64_t)P->ninst(), P->nbc(), (int64_t)P->wmax,
+    const int64_t key[6] = {(int64_t)P->nact, (int64_t)P->ninst(), P->nbc(), (int64_t)P->wmax,
                             (int64_t)(P->mode.viscosity_type * 64 + P->mode.viscous_type * 8 + P->mode.alpha * 2 +
                                       P->mode.roundoff),
                             tuning_hash(P->tune)};
@@ -652,7 +693,7 @@ int ensure_graph(Pool* P) {
 }  // namespace
 
 int pool_queue_steps(Pool* P, int nsteps) {
-    if (nsteps <= 0 || P->nrows() == 0) return SWE_OK;
+    if (nsteps <= 0 || P->nact == 0) return SWE_OK;
     int rc;
     if (P->use_graph && !P->timing) {
         if ((rc = ensure_graph(P))) return rc;
@@ -681,7 +722,7 @@ int pool_read_table(Pool* P) {
 // kernel's launches; the state and the table are restored afterwards, so tuning changes no
 // result (every schedule computes the same rows with the same arithmetic).
 int pool_tune(Pool* P, int steps, int trials) {
-    if (P->nrows() == 0) return SWE_OK;
+    if (P->nact == 0) return SWE_OK;
     steps = std::max(1, steps);
     trials = std::max(1, trials);
     const int n = std::max(P->ninst(), 1);

@@ -108,6 +108,13 @@ struct Pool {
     double2* nvec = nullptr;
     double* nrm = nullptr;
     uint16_t* elist = nullptr;
+    // the slices the step kernels cover: those of the instances that are (or may be) stepping
+    int* act = nullptr;
+    int nact = 0;
+    int* act_host[2] = {nullptr, nullptr};   // pinned staging, used alternately
+    hipEvent_t act_ev[2] = {nullptr, nullptr};
+    int act_buf = 0;
+    std::vector<char> listed;                // per instance: its slices are in the list
     // states and work arrays
     Node* S[3] = {nullptr, nullptr, nullptr};   // S[0] holds U^n between steps
     Node* scratch = nullptr;
@@ -155,6 +162,8 @@ int pool_set_field(Pool* P, int id, size_t offset, const void* src, size_t bytes
 int pool_upload_state(Pool* P, int id, const double* un, Node* dst);
 int pool_download_state(Pool* P, int id, const Node* src, double* un);
 int pool_queue_steps(Pool* P, int nsteps);
+// Rebuild the launch list from the instances marked listed (ordered on the compute stream).
+int pool_update_active(Pool* P);
 int pool_read_table(Pool* P);         // sync, then tab_host / flags_host hold the table
 int pool_tune(Pool* P, int steps, int trials);
 

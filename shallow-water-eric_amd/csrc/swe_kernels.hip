@@ -42,10 +42,15 @@ enum SmbKind : int { SMB_ROUNDOFF = 0, SMB_2 = 1, SMB_1 = 2, SMB_GALERKIN = 3 };
 
 constexpr double kPi = 3.141592653589793;   // ACOS(-1.d0)
 
+// Row of this lane: the waves of a launch take the listed slices in order (remap: XCD-aware
+// workgroup order), one slice each; -1 for waves past the end of the list.
 template <int BS>
-__device__ __forceinline__ int lane_row(int remap) {
+__device__ __forceinline__ int lane_row(const Mesh& M, int remap) {
     const int b = remap ? xcd_block() : (int)blockIdx.x;
-    return b * BS + (int)threadIdx.x;
+    const int w = b * (BS / 64) + (int)(threadIdx.x >> 6);
+    if (w >= M.nact) return -1;
+    const int slice = __builtin_amdgcn_readfirstlane(M.act[w]);
+    return slice * 64 + (int)(threadIdx.x & 63);
 }
 
 // Instance of the wave holding row i (rows i..i|63 share it); -1 for a free slice.
@@ -129,11 +134,11 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
              int remap) {
     extern __shared__ double smem[];
     double* lw = wave_lds(smem, M.wmax);
-    const int i = lane_row<BS>(remap);
+    const int i = lane_row<BS>(M, remap);
     const int lane = (int)(threadIdx.x & 63);
     bool wave_live = false;
     int inst = -1;
-    if (i < M.nrows) {
+    if (i >= 0) {
         inst = wave_inst(M, i);
         if (inst >= 0) wave_live = all ? (T[inst].state != INST_FREE) : takes_steps(T[inst], flags[inst]);
     }
@@ -255,8 +260,8 @@ template <int BS>
 __global__ void __launch_bounds__(BS)
 k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ dij,
            int* __restrict__ flags_out) {
-    const int i = (int)blockIdx.x * BS + (int)threadIdx.x;
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, 0);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || T[inst].state == INST_FREE) return;
     const int len = M.len[i];
@@ -288,8 +293,8 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
 template <int BS>
 __global__ void __launch_bounds__(BS)
 k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ alpha, int remap) {
-    const int i = lane_row<BS>(remap);
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, remap);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || !T[inst].go) return;
     const int len = M.len[i];
@@ -337,8 +342,8 @@ __global__ void __launch_bounds__(BS)
 k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
            Node* __restrict__ out, const double* __restrict__ d0, const double* __restrict__ alpha, int stage,
            int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(remap);
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, remap);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || !T[inst].go) return;
     const int len = M.len[i];
@@ -466,8 +471,8 @@ __global__ void __launch_bounds__(BS)
 k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node* __restrict__ ulo,
          double* __restrict__ dL, double* __restrict__ muL, double* __restrict__ res, const double* __restrict__ d0,
          const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(remap);
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, remap);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || !T[inst].go) return;
     const int len = M.len[i];
@@ -630,8 +635,8 @@ __global__ void __launch_bounds__(BS)
 k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ ulo,
              const double* __restrict__ dL, const double* __restrict__ muL, const double* __restrict__ res,
              double* __restrict__ Rm, int remap) {
-    const int i = lane_row<BS>(remap);
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, remap);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || !T[inst].go) return;
     const int len = M.len[i];
@@ -667,8 +672,8 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
             const Node* __restrict__ ulo, Node* __restrict__ out, const double* __restrict__ dL,
             const double* __restrict__ muL, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
             int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(remap);
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, remap);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || !T[inst].go) return;
     const int len = M.len[i];
@@ -723,8 +728,8 @@ __global__ void __launch_bounds__(BS)
 k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
             Node* __restrict__ out, const double* __restrict__ dL, const double* __restrict__ muL,
             const double* __restrict__ res, int roundoff, int stage, int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(remap);
-    if (i >= M.nrows) return;
+    const int i = lane_row<BS>(M, remap);
+    if (i < 0) return;
     const int inst = wave_inst(M, i);
     if (inst < 0 || !T[inst].go) return;
     const int len = M.len[i];
@@ -897,8 +902,8 @@ __global__ void k_fill_bc(BcEntry* __restrict__ bc, int64_t b0, int64_t n) {
 // FCT_positivity on the pool's sliced layout (fct.f90:15-41): R- per row, then l_ij.
 __global__ void k_fctpos_r(Mesh M, const double* __restrict__ f, const double* __restrict__ us, double htiny,
                            double* __restrict__ Rm) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= M.nrows || M.len[i] == 0) return;
+    const int i = lane_row<256>(M, 0);
+    if (i < 0 || M.len[i] == 0) return;
     const double Q = 0.0 - us[i];
     double P = -1.0e-15;
     const int64_t pb = row_base(M, i);
@@ -913,8 +918,8 @@ __global__ void k_fctpos_r(Mesh M, const double* __restrict__ f, const double* _
 }
 
 __global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* __restrict__ Rm, double* __restrict__ lij) {
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= M.nrows || M.len[i] == 0) return;
+    const int i = lane_row<256>(M, 0);
+    if (i < 0 || M.len[i] == 0) return;
     const int64_t pb = row_base(M, i);
     const int len = M.len[i];
     const double Ri = Rm[i];
@@ -990,7 +995,7 @@ template <int SMB, bool ALPHA>
 void launch_stage_lo(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_STAGE_LO];
-    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
                                               P.stream, M, (const InstDev*)P.tab, U, Uo, out, d0,
                                               (const double*)P.alpha, stage, P.flags, L.remap))
 }
@@ -1000,7 +1005,7 @@ void launch_fct_lo(Pool& P, const Node* U, Node* ulo, const double* d0) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_FCT_LO];
     const int ro = P.mode.roundoff;
-    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO>), dim3(blocks_of(M.nrows, BS)), dim3(BS),
+    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS),
                                               0, P.stream, M, (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
                                               (const double*)P.alpha, ro, P.flags, L.remap))
 }
@@ -1016,12 +1021,12 @@ bool block_supported(int kid, int block) {
 }
 
 Mesh mesh_view(const Pool& P) {
-    return Mesh{P.sl, P.len, P.col, P.c, P.nvec, P.nrm, P.m, P.elist, P.dpos, P.nrows(), P.wmax};
+    return Mesh{P.sl, P.len, P.col, P.c, P.nvec, P.nrm, P.m, P.elist, P.dpos, P.act, P.nact, P.wmax};
 }
 
 void launch_compute_dt(Pool& P, const Node* U, double* d0, int all) {
     const Mesh M = mesh_view(P);
-    if (M.nrows == 0) return;
+    if (M.nact == 0) return;
     const LaunchCfg& L = P.tune.k[KID_DT];
     // the dynamic LDS of a workgroup stays within 64 KiB: fall back to smaller workgroups
     int block = L.block;
@@ -1029,7 +1034,7 @@ void launch_compute_dt(Pool& P, const Node* U, double* d0, int all) {
     timed_begin(P, KID_DT);
     SWE_BS_SWITCH(block, {
         const size_t lds = (size_t)(2 + M.wmax) * 64 * sizeof(double) * (BS / 64);
-        hipLaunchKernelGGL((k_compute_dt<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), lds, P.stream, M,
+        hipLaunchKernelGGL((k_compute_dt<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), lds, P.stream, M,
                            (const InstDev*)P.tab, (const int*)P.flags, U, d0, P.vmax, P.flags, all, L.remap);
     })
     timed_end(P, KID_DT);
@@ -1048,13 +1053,13 @@ void launch_finalize_dt(Pool& P, int advance) {
 int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage) {
     const Mode& md = P.mode;
     const Mesh M = mesh_view(P);
-    if (M.nrows == 0) return SWE_OK;
+    if (M.nact == 0) return SWE_OK;
     const InstDev* T = P.tab;
     const bool alpha = md.alpha != 0 && md.viscosity_type != SWE_GALERKIN;
     if (alpha) {
         const LaunchCfg& L = P.tune.k[KID_ALPHA];
         timed_begin(P, KID_ALPHA);
-        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_alpha<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0, P.stream,
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_alpha<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream,
                                                   M, T, U, P.alpha, L.remap))
         timed_end(P, KID_ALPHA);
     }
@@ -1082,7 +1087,7 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         {
             const LaunchCfg& L = P.tune.k[KID_RMINUS];
             timed_begin(P, KID_RMINUS);
-            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_rminus<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_rminus<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
                                                       P.stream, M, T, U, (const Node*)P.ulo, (const double*)P.dL,
                                                       (const double*)P.muL, (const double*)P.res, P.Rm, L.remap))
             timed_end(P, KID_RMINUS);
@@ -1090,7 +1095,7 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         {
             const LaunchCfg& L = P.tune.k[KID_APPLY];
             timed_begin(P, KID_APPLY);
-            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_apply<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_apply<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
                                                       P.stream, M, T, U, Uo, (const Node*)P.ulo, out,
                                                       (const double*)P.dL, (const double*)P.muL, (const double*)P.res,
                                                       (const double*)P.Rm, stage, P.flags, L.remap))
@@ -1101,7 +1106,7 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         else launch_fct_lo<false, false>(P, U, out, d0);
         const LaunchCfg& L = P.tune.k[KID_EV];
         timed_begin(P, KID_EV);
-        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_ev_update<BS>), dim3(blocks_of(M.nrows, BS)), dim3(BS), 0,
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_ev_update<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
                                                   P.stream, M, T, U, Uo, out, (const double*)P.dL,
                                                   (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
                                                   P.flags, L.remap))
@@ -1121,8 +1126,8 @@ void launch_bdy(Pool& P, Node* u, int stage) {
 
 void launch_dij_full(Pool& P, const Node* U, double* dij) {
     const Mesh M = mesh_view(P);
-    if (M.nrows == 0) return;
-    hipLaunchKernelGGL((k_dij_full<256>), dim3(blocks_of(M.nrows, 256)), dim3(256), 0, P.stream, M,
+    if (M.nact == 0) return;
+    hipLaunchKernelGGL((k_dij_full<256>), dim3(blocks_of((int64_t)M.nact * 64, 256)), dim3(256), 0, P.stream, M,
                        (const InstDev*)P.tab, U, dij, P.flags);
 }
 
@@ -1154,9 +1159,9 @@ void launch_sell_to_csr(Pool& P, const double* src, const int64_t* map, double* 
 
 void launch_fctpos(Pool& P, const double* f, const double* us, double htiny, double* lij) {
     const Mesh M = mesh_view(P);
-    if (M.nrows == 0) return;
-    hipLaunchKernelGGL(k_fctpos_r, dim3(blocks_of(M.nrows, 256)), dim3(256), 0, P.stream, M, f, us, htiny, P.Rm);
-    hipLaunchKernelGGL(k_fctpos_l, dim3(blocks_of(M.nrows, 256)), dim3(256), 0, P.stream, M, f, (const double*)P.Rm,
+    if (M.nact == 0) return;
+    hipLaunchKernelGGL(k_fctpos_r, dim3(blocks_of((int64_t)M.nact * 64, 256)), dim3(256), 0, P.stream, M, f, us, htiny, P.Rm);
+    hipLaunchKernelGGL(k_fctpos_l, dim3(blocks_of((int64_t)M.nact * 64, 256)), dim3(256), 0, P.stream, M, f, (const double*)P.Rm,
                        lij);
 }
 

@@ -5,8 +5,9 @@
 // wave reads its instance's constants, dt and state once, from the instance table, with scalar
 // loads.  Entries of a slice are stored sliced-ELL: row r of slice s owns the slots
 // sl[s].sbase + (r % 64) + 64*q for q < len[r], in the row's original CSR order.  Column indices
-// are global row numbers, so a kernel never needs an instance offset.  The single-mesh handle is
-// a pool with one instance.
+// are global row numbers, so a kernel never needs an instance offset.  A launch covers the list
+// of slices of the instances that are stepping (rebuilt when instances join or leave), so free
+// rows and finished instances cost no waves.  The single-mesh handle is a pool with one instance.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -88,7 +89,8 @@ struct Mesh {
     const double* __restrict__ m;       // lumped mass of each row
     const uint16_t* __restrict__ elist; // per slice: slot offsets (q*64 + lane) of off-diagonal entries
     const int* __restrict__ dpos;       // position q of the diagonal in each row
-    int nrows;                          // rows covered by a launch (whole slices)
+    const int* __restrict__ act;        // the slices a launch covers, ascending (one wave each)
+    int nact;                           // number of them
     int wmax;                           // longest row of the resident instances
 };
 

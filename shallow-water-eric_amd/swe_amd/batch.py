@@ -37,7 +37,7 @@ class Batch:
         return int(load_library().swe_batch_bytes(C.byref(caps)))
 
     @classmethod
-    def for_budget(cls, budget_bytes, max_instances=1024, entries_per_node=7.6, boundary_per_node=0.02):
+    def for_budget(cls, budget_bytes, max_instances=1024, entries_per_node=9.0, boundary_per_node=0.02):
         """The largest batch whose device arrays fit in budget_bytes, for meshes with about
         entries_per_node stored entries per node."""
         per_node = cls.bytes_for(1 << 20, int(entries_per_node * (1 << 20)), int(boundary_per_node * (1 << 20)),

@@ -40,6 +40,7 @@ class Problem:
     bc: tuple = (BC_NONE, 0.0, 1.0, 0.0)   # (kind, amp, period, level)
     nsteps: int = 1
     reference_init: bool = False           # True when the reference's own init covers the case
+    tfinal: float | None = None            # final time of a served request
 
     @property
     def np(self):
@@ -150,3 +151,103 @@ def bumps(n_side=4097, L=1000.0, nbumps=64, seed=20261202, cfl=0.25, amp=0.05, p
     return _finish("bumps", M.jittered_delaunay_mesh(n_side, L, jitter=0.25, seed=seed),
                    lambda rr: bump_field(rr, L, nbumps, seed), u0, params,
                    h_sides=(4,), ux_sides=(2,), uy_sides=(1, 3), bc=(BC_INFLOW, amp, period, 1.0), nsteps=200)
+
+
+# ----------------------------------------------------------------------------------------------
+def family_member(n_nodes, seed, kind="bumps", L=1000.0, tfinal=None, amp=0.05, period=20.0, zmax=0.7,
+                  cfl=0.25) -> Problem:
+    """One mesh of the 16.8M-node workload's family at another size: a jittered-lattice Delaunay
+    mesh of about n_nodes nodes on [0,L]^2 with its own 64 Gaussian bumps (seeded), scaled down
+    where needed so that max z <= zmax: the bumps stay submerged (the reference scheme stops with
+    'Min h<0' within a few hundred steps on many coarse meshes with emerging islands, so the long
+    batched and served runs keep the islands out).  Kinds:
+      'bumps'   still water of level 1 with the sinusoidal west inflow, walls elsewhere;
+      'dam'     dam break over the bumps: level 1 on x < L/2, level 0.8 beyond, q = 0, walls;
+      'drydam'  the dry-bed dam break of the 1M-node workload on this mesh: flat bed, h = 1 on
+                x < L/2 and 0 beyond, Manning n = 0.02 (type_test 10), walls.
+    FCT + alpha limiter; type_test 4 (no source term) except for 'drydam'."""
+    n_side = max(8, int(round(math.sqrt(n_nodes))))
+    params = dict(gravity=9.81, htiny=1.0e-4 * 1.0, cfl=cfl, max_water_h=1.0, viscosity="fct",
+                  viscous_type=2, alpha_limit=True, type_test=4, mannings=0.0)
+
+    def bath(rr):
+        if kind == "drydam":
+            return np.zeros(rr.shape[0])
+        z = bump_field(rr, L, 64, seed)
+        return z * (zmax / z.max()) if z.max() > zmax else z
+
+    walls = dict(ux_sides=(2, 4), uy_sides=(1, 3), bc=(BC_WALLS, 0.0, 1.0, 0.0))
+    if kind == "bumps":
+        def u0(rr, z):
+            u = np.zeros((rr.shape[0], 3))
+            u[:, 0] = np.maximum(1.0 - z, 0.0)
+            return u
+        bc = dict(h_sides=(4,), ux_sides=(2,), uy_sides=(1, 3), bc=(BC_INFLOW, amp, period, 1.0))
+    elif kind == "basin":
+        cx, cy = np.random.default_rng(seed + 7).uniform(0.2 * L, 0.8 * L, 2)
+
+        def u0(rr, z):
+            u = np.zeros((rr.shape[0], 3))
+            eta = 1.0 + 0.05 * np.exp(-((rr[:, 0] - cx) ** 2 + (rr[:, 1] - cy) ** 2) / (2.0 * (0.05 * L) ** 2))
+            u[:, 0] = np.maximum(eta - z, 0.0)
+            return u
+        bc = walls
+    elif kind == "dam":
+        def u0(rr, z):
+            u = np.zeros((rr.shape[0], 3))
+            u[:, 0] = np.where(rr[:, 0] < 0.5 * L, np.maximum(1.0 - z, 0.0), np.maximum(0.8 - z, 0.0))
+            return u
+        bc = walls
+    elif kind == "drydam":
+        params.update(type_test=10, mannings=0.02)
+
+        def u0(rr, z):
+            u = np.zeros((rr.shape[0], 3))
+            u[:, 0] = np.where(rr[:, 0] < 0.5 * L, 1.0, 0.0)
+            return u
+        bc = walls
+    else:
+        raise ValueError("unknown kind %r" % kind)
+    return _finish("%s_%d" % (kind, n_side * n_side), M.jittered_delaunay_mesh(n_side, L, jitter=0.25, seed=seed),
+                   bath, u0, params, nsteps=1000, tfinal=tfinal, **bc)
+
+
+def batch_family_specs(count=256, nmin=5000, nmax=500000, seed=20261202, dam_fraction=0.5):
+    """Sizes (log-uniform in [nmin, nmax]), seeds and kinds of the batched workload's instances:
+    dam breaks ('dam') with probability dam_fraction, still water ('basin') otherwise."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), count))
+    kinds = np.where(rng.uniform(size=count) < dam_fraction, "dam", "basin")
+    seeds = rng.integers(1, 2 ** 31 - 1, count)
+    return [dict(n_nodes=int(n), seed=int(s), kind=str(k)) for n, s, k in zip(sizes, seeds, kinds)]
+
+
+def serve_request_specs(count, rate, seed=20261202, median=100000, sigma=1.0, nmax=2000000, nmin=2000,
+                        dry_fraction=0.2, period=20.0, L=1000.0):
+    """A Poisson arrival stream: arrival times (s), sizes lognormal (median, sigma, clipped to
+    [nmin, nmax]), final times U[1,20] wave periods, dry-bed dam breaks ('drydam') with probability
+    dry_fraction, the rest a wave in a still-water basin over their own bumps ('basin')."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    sizes = np.clip(np.exp(math.log(median) + sigma * rng.standard_normal(count)), nmin, nmax)
+    tfinal = rng.uniform(1.0, 20.0, count) * period
+    kinds = np.where(rng.uniform(size=count) < dry_fraction, "drydam", "basin")
+    seeds = rng.integers(1, 2 ** 31 - 1, count)
+    return [dict(arrival=float(a), n_nodes=int(n), tfinal=float(t), kind=str(k), seed=int(s), L=L, period=period)
+            for a, n, t, k, s in zip(arrivals, sizes, tfinal, kinds, seeds)]
+
+
+def build_family(specs, workers=1):
+    """Problems of a list of specs (family_member keyword sets), in parallel processes if workers > 1."""
+    keys = ("n_nodes", "seed", "kind", "L", "tfinal", "period")
+    args = [{k: s[k] for k in keys if k in s} for s in specs]
+    if workers <= 1 or len(args) < 2:
+        return [family_member(**a) for a in args]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_family_member_kw, args, chunksize=1)
+
+
+def _family_member_kw(kw):
+    return family_member(**kw)

@@ -1,0 +1,145 @@
+"""Serving: shallow-water solves of different sizes and final times, admitted as they arrive into
+one device-resident batch and advanced together (continuous batching), with time-to-solution
+recorded per request.
+
+:class:`Server` runs the loop: finished instances are collected (state read back, rows freed),
+the next chunk of steps is queued for everything resident, and while the GPU runs it the host
+admits waiting requests (FIFO) into the rows that are free, against the batch's capacity, which
+is set from a memory budget.  A request joins the steps queued after its admission and leaves
+when its time reaches its final time (the loop condition of SOURCES/main.f90:61), so its result
+is the one it would get alone.  :func:`run_plain` serves the same stream one request at a time
+on the same engine: the plain per-instance loop the server is measured against.
+
+The engine interface (add / step / finished / collect) is what the loop needs from a batch;
+:class:`BatchEngine` binds it to :class:`swe_amd.batch.Batch`.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+MAX_STEPS = 2 ** 62
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    problem: object              # swe_amd.problems.Problem
+    tfinal: float
+    arrival: float               # seconds after the start of the stream
+    admitted: float | None = None
+    finished: float | None = None
+    time: float = 0.0
+    steps: int = 0
+    error: int = 0
+    state: np.ndarray | None = None
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+    @property
+    def node_stages(self):
+        return self.problem.np * 3 * self.steps
+
+
+class BatchEngine:
+    """The serving loop's view of a :class:`swe_amd.batch.Batch`."""
+
+    def __init__(self, batch):
+        self.batch = batch
+
+    def add(self, req: Request):
+        return self.batch.add_problem(req.problem, tfinal=req.tfinal, max_steps=MAX_STEPS)
+
+    def step(self, n):
+        self.batch.step(n)
+
+    def finished(self):
+        return self.batch.poll()
+
+    def collect(self, k):
+        st, t, steps, err = self.batch.status(k)
+        u = self.batch.get_state(k)
+        self.batch.remove(k)
+        return u, t, steps, err
+
+
+class Server:
+    """Continuous batching: requests join and leave a running batch between chunks of steps."""
+
+    def __init__(self, engine, chunk=16, clock=time.perf_counter, sleep=time.sleep):
+        self.engine = engine
+        self.chunk = int(chunk)
+        self.clock = clock
+        self.sleep = sleep
+        self.max_resident = 0
+
+    def run(self, requests):
+        reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        pending = collections.deque(reqs)
+        waiting = collections.deque()
+        active = {}
+        t0 = self.clock()
+        while pending or waiting or active:
+            now = self.clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            if active:
+                for k in self.engine.finished():
+                    r = active.pop(k)
+                    r.state, r.time, r.steps, r.error = self.engine.collect(k)
+                    r.finished = self.clock() - t0
+            if active:
+                self.engine.step(self.chunk)          # the GPU runs this while the host admits
+            admitted = 0
+            while waiting:
+                r = waiting[0]
+                k = self.engine.add(r)
+                if k is None:
+                    if not active:
+                        raise RuntimeError("request %d (%d nodes) does not fit an empty batch" % (r.rid, r.problem.np))
+                    break
+                waiting.popleft()
+                active[k] = r
+                r.admitted = self.clock() - t0
+                admitted += 1
+            self.max_resident = max(self.max_resident, len(active))
+            if not active and not waiting and pending:
+                self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
+        return reqs
+
+
+def run_plain(engine, requests, chunk=16, clock=time.perf_counter, sleep=time.sleep):
+    """The same stream served one request at a time, in arrival order."""
+    reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
+    t0 = clock()
+    for r in reqs:
+        now = clock() - t0
+        if r.arrival > now:
+            sleep(r.arrival - now)
+        r.admitted = clock() - t0
+        k = engine.add(r)
+        if k is None:
+            raise RuntimeError("request %d (%d nodes) does not fit the engine" % (r.rid, r.problem.np))
+        while True:
+            engine.step(chunk)
+            if k in engine.finished():
+                break
+        r.state, r.time, r.steps, r.error = engine.collect(k)
+        r.finished = clock() - t0
+    return reqs
+
+
+def summarize(reqs):
+    """p50/p99/mean time-to-solution (s), aggregate node-stage updates/s over the makespan."""
+    lat = np.array([r.latency for r in reqs])
+    work = float(sum(r.node_stages for r in reqs))
+    span = max(r.finished for r in reqs) - min(r.arrival for r in reqs)
+    return {"requests": len(reqs), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "makespan_s": float(span), "node_stages": work,
+            "node_stage_per_s": work / span if span > 0 else float("nan"),
+            "errors": int(sum(1 for r in reqs if r.error))}
