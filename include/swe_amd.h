@@ -90,7 +90,9 @@ int swe_assemble(int np, int me, const int *jj, const double *rr, int *ia, int *
                  double *mass, double *lumped, double *cx, double *cy);
 
 /* Create a solver over one mesh: copies the mesh arrays to the device.  Replaces the module
- * state built by construct_matrices (SOURCES/update.f90:18-132) and construct_mesh. */
+ * state built by construct_matrices (SOURCES/update.f90:18-132) and construct_mesh.
+ * The environment variable SWE_SORT_WINDOW (default 64) sets the window of rows inside which
+ * the internal numbering sorts rows by length (a layout choice; results do not depend on it). */
 int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **out);
 int swe_destroy(swe_handle *h);
 int swe_set_params(swe_handle *h, const swe_params *prm);

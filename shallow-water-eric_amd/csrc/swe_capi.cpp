@@ -157,7 +157,7 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
     HostLayout L;
     Mode md;
     if ((rc = mode_of(*prm, md))) return rc;
-    if ((rc = build_layout(*d, 64, L))) return rc;
+    if ((rc = build_layout(*d, sort_window_default(), L))) return rc;
     PoolCaps caps;
     caps.rows = L.np_pad;
     caps.slots = L.nslot;

@@ -14,6 +14,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -314,12 +315,19 @@ size_t pool_bytes(const PoolCaps& c) {
            (size_t)c.elist * sizeof(uint16_t) + (size_t)c.instances * kInstBytes + (size_t)c.bc * kBcBytes;
 }
 
+int sort_window_default() {
+    const char* v = std::getenv("SWE_SORT_WINDOW");
+    const int w = v ? std::atoi(v) : 0;
+    return (w >= 1 && w <= (1 << 20)) ? w : 64;
+}
+
 int pool_create(const PoolCaps& caps, Pool** out) {
     *out = nullptr;
     if (caps.rows <= 0 || caps.slots <= 0 || caps.instances <= 0 || caps.elist < 0 || caps.bc < 0)
         return fail(SWE_EINVAL, "pool capacities must be positive");
     if (caps.rows > (int64_t)INT_MAX - 64) return fail(SWE_EINVAL, "at most 2^31 rows per pool");
     Pool* P = new Pool();
+    P->sort_window = sort_window_default();
     const int64_t slices = (caps.rows + 63) / 64;
     P->rows_cap = slices * 64;
     P->slots_cap = caps.slots;
@@ -549,9 +557,13 @@ int pool_update_active(Pool* P) {
     P->act_buf ^= 1;
     SWE_HIPCHK(hipEventSynchronize(P->act_ev[b]));
     std::vector<std::pair<int64_t, int64_t>> runs;
+    P->any_src = false;
     for (int k = 0; k < P->inst_cap; ++k)
-        if (P->inst[(size_t)k].used && P->listed[(size_t)k])
+        if (P->inst[(size_t)k].used && P->listed[(size_t)k]) {
             runs.emplace_back(P->inst[(size_t)k].slice0, P->inst[(size_t)k].nslices);
+            Consts K;
+            if (make_consts(P->inst[(size_t)k].prm, K) == SWE_OK && K.src != SRC_NONE) P->any_src = true;
+        }
     std::sort(runs.begin(), runs.end());
     int n = 0;
     for (const auto& r : runs)
@@ -605,7 +617,10 @@ int pool_set_params(Pool* P, int k, const swe_params& prm) {
     P->mode = md;
     P->mode_set = true;
     P->inst[(size_t)k].prm = prm;
-    return pool_set_field(P, k, offsetof(InstDev, K), &K, sizeof K);
+    rc = pool_set_field(P, k, offsetof(InstDev, K), &K, sizeof K);
+    if (rc) return rc;
+    if (K.src != SRC_NONE) P->any_src = true;    // (a list update recomputes it exactly)
+    return SWE_OK;
 }
 
 int pool_set_bc(Pool* P, int k, int kind, double amp, double period, double level) {
@@ -663,8 +678,8 @@ int64_t tuning_hash(const Tuning& t) {
 // The captured step stays valid while the launch extents, the scheme and the schedules stay.
 int ensure_graph(Pool* P) {
     const int64_t key[6] = {(int64_t)P->nact, (int64_t)P->ninst(), P->nbc(), (int64_t)P->wmax,
-                            (int64_t)(P->mode.viscosity_type * 64 + P->mode.viscous_type * 8 + P->mode.alpha * 2 +
-                                      P->mode.roundoff),
+                            (int64_t)(P->mode.viscosity_type * 128 + P->mode.viscous_type * 16 + P->mode.alpha * 4 +
+                                      P->mode.roundoff * 2 + (P->any_src ? 1 : 0)),
                             tuning_hash(P->tune)};
     if (P->gexec && std::equal(key, key + 6, P->graph_key)) return SWE_OK;
     if (P->gexec) {

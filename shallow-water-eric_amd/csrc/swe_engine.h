@@ -61,6 +61,8 @@ struct HostLayout {
 };
 int validate_mesh(const swe_mesh_desc& d);
 int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L);
+// Rows sorted by length inside windows of this many rows: SWE_SORT_WINDOW, else 64.
+int sort_window_default();
 int make_consts(const swe_params& p, Consts& K);
 
 // ---- the pool ---------------------------------------------------------------------------------
@@ -115,6 +117,7 @@ struct Pool {
     hipEvent_t act_ev[2] = {nullptr, nullptr};
     int act_buf = 0;
     std::vector<char> listed;                // per instance: its slices are in the list
+    bool any_src = false;                    // a listed instance has a source term (type_test)
     // states and work arrays
     Node* S[3] = {nullptr, nullptr, nullptr};   // S[0] holds U^n between steps
     Node* scratch = nullptr;

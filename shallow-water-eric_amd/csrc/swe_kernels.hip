@@ -42,21 +42,54 @@ enum SmbKind : int { SMB_ROUNDOFF = 0, SMB_2 = 1, SMB_1 = 2, SMB_GALERKIN = 3 };
 
 constexpr double kPi = 3.141592653589793;   // ACOS(-1.d0)
 
-// Row of this lane: the waves of a launch take the listed slices in order (remap: XCD-aware
-// workgroup order), one slice each; -1 for waves past the end of the list.
+#define SWE_PRAGMA(x) _Pragma(#x)
+#define SWE_UNROLL(n) SWE_PRAGMA(unroll n)
+#ifndef SWE_UNROLL_LO
+#define SWE_UNROLL_LO 1
+#endif
+#ifndef SWE_UNROLL_RM
+#define SWE_UNROLL_RM 1
+#endif
+#ifndef SWE_UNROLL_AP
+#define SWE_UNROLL_AP 1
+#endif
+// minimum waves per SIMD requested from the register allocator (0: the compiler's choice).  The
+// dt pass (VALU-bound, LDS-staged) runs best at 7 waves; k_fct_lo spills below 124 registers.
+#ifndef SWE_WAVES_LO
+#define SWE_WAVES_LO 0
+#endif
+#ifndef SWE_WAVES_DT
+#define SWE_WAVES_DT 7
+#endif
+#if SWE_WAVES_LO > 0
+#define SWE_LB_LO(BS) __launch_bounds__(BS, SWE_WAVES_LO)
+#else
+#define SWE_LB_LO(BS) __launch_bounds__(BS)
+#endif
+#if SWE_WAVES_DT > 0
+#define SWE_LB_DT(BS) __launch_bounds__(BS, SWE_WAVES_DT)
+#else
+#define SWE_LB_DT(BS) __launch_bounds__(BS)
+#endif
+
+// Slice of this wave: the waves of a launch take the listed slices in order (remap: XCD-aware
+// workgroup order), one slice each; -1 past the end of the list.  The value (and everything
+// derived from it: the instance, its constants, dt) is wave-uniform and kept in scalar registers;
+// the lane's row is slice*64 + lane.
 template <int BS>
-__device__ __forceinline__ int lane_row(const Mesh& M, int remap) {
+__device__ __forceinline__ int wave_slice(const Mesh& M, int remap) {
     const int b = remap ? xcd_block() : (int)blockIdx.x;
-    const int w = b * (BS / 64) + (int)(threadIdx.x >> 6);
+    const int w = __builtin_amdgcn_readfirstlane(b * (BS / 64) + (int)(threadIdx.x >> 6));
     if (w >= M.nact) return -1;
-    const int slice = __builtin_amdgcn_readfirstlane(M.act[w]);
-    return slice * 64 + (int)(threadIdx.x & 63);
+    return __builtin_amdgcn_readfirstlane(M.act[w]);
 }
 
-// Instance of the wave holding row i (rows i..i|63 share it); -1 for a free slice.
-__device__ __forceinline__ int wave_inst(const Mesh& M, int i) {
-    return __builtin_amdgcn_readfirstlane(M.sl[i >> 6].inst);
+// Instance owning a slice; -1 for a free slice.
+__device__ __forceinline__ int slice_inst(const Mesh& M, int slice) {
+    return __builtin_amdgcn_readfirstlane(M.sl[slice].inst);
 }
+
+__device__ __forceinline__ int lane_of() { return (int)(threadIdx.x & 63); }
 
 __device__ __forceinline__ bool takes_steps(const InstDev& D, int flags) {
     return D.state == INST_RUNNING && flags == 0 && D.time < D.tfinal && D.steps < D.max_steps;
@@ -85,8 +118,11 @@ __device__ __forceinline__ double rk_combine(int stage, double uo, double ui) {
 
 // Source terms of one node, dispatched on type_test (update.f90:614-623): friction (425-452),
 // my_friction (454-469), coriolis (471-486).  v1, v2 = velocity at U.  src is uniform per wave.
+// Launches whose instances have no source term use the HAS_SRC = false kernels, which leave it out.
+template <bool HAS_SRC>
 __device__ __forceinline__ void add_sources(int src, const Node& I, double mi, double v1, double v2, double dt,
                                             const Consts& K, double& r2, double& r3) {
+    if constexpr (!HAS_SRC) return;
     if (src == SRC_MANNING) {
         const double hs = pow43(fmaxF(I.h, K.htiny));
         const double vel = K.fric * sqrt(v1 * v1 + v2 * v2);
@@ -102,6 +138,16 @@ __device__ __forceinline__ void add_sources(int src, const Node& I, double mi, d
         r2 = r2 + mi * f * I.qy;
         r3 = r3 - mi * f * I.qx;
     }
+}
+
+// compute_dij of one off-diagonal entry from c_ij itself: the norm and the unit vector are
+// evaluated with compute_dij's expressions (SOURCES/update.f90:306-310; correctly rounded sqrt and
+// division give the bits the host computes for the stored normals), which costs a square root and
+// two divisions instead of 24 bytes of reads per entry in the passes that already load c_ij.
+__device__ __forceinline__ double dij_from_c(const double2 c, const Node& I, const Node& J, double ovh_i,
+                                             double ovh_j, double sql, double sqr, const Consts& K, int& err) {
+    const double norm = sqrt(0.0 + c.x * c.x + c.y * c.y);
+    return dij_of(make_double2(c.x / norm, c.y / norm), norm, I, J, ovh_i, ovh_j, sql, sqr, K, err);
 }
 
 // dlam is compute_dij's value for this entry (from the dt pass or computed inline).
@@ -128,20 +174,21 @@ __device__ __forceinline__ void first_order_dmu(const double2 c, double v1i, dou
 // instance; otherwise only the instances that take the step.
 // =============================================================================================
 template <int BS>
-__global__ void __launch_bounds__(BS)
+__global__ void SWE_LB_DT(BS)
 k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flags, const Node* __restrict__ U,
              double* __restrict__ d0, unsigned long long* __restrict__ vmax, int* __restrict__ flags_out, int all,
              int remap) {
     extern __shared__ double smem[];
     double* lw = wave_lds(smem, M.wmax);
-    const int i = lane_row<BS>(M, remap);
-    const int lane = (int)(threadIdx.x & 63);
+    const int slice = wave_slice<BS>(M, remap);
+    const int lane = lane_of();
     bool wave_live = false;
     int inst = -1;
-    if (i >= 0) {
-        inst = wave_inst(M, i);
+    if (slice >= 0) {
+        inst = slice_inst(M, slice);
         if (inst >= 0) wave_live = all ? (T[inst].state != INST_FREE) : takes_steps(T[inst], flags[inst]);
     }
+    const int i = slice * 64 + lane;
     const bool valid = wave_live && M.len[i] > 0;
     int err = 0;
     Consts K{};
@@ -155,7 +202,6 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
     }
     __syncthreads();
     if (wave_live) {
-        const int slice = i >> 6;
         const SliceInfo S = M.sl[slice];
         const int64_t e1 = S.ebase + S.ecount;
         for (int64_t e = S.ebase + lane; e < e1; e += 64) {
@@ -260,10 +306,11 @@ template <int BS>
 __global__ void __launch_bounds__(BS)
 k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ dij,
            int* __restrict__ flags_out) {
-    const int i = lane_row<BS>(M, 0);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, 0);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || T[inst].state == INST_FREE) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const Consts K = T[inst].K;
@@ -293,10 +340,11 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
 template <int BS>
 __global__ void __launch_bounds__(BS)
 k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ alpha, int remap) {
-    const int i = lane_row<BS>(M, remap);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, remap);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const double htiny = T[inst].K.htiny;
@@ -337,15 +385,16 @@ k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doubl
 // =============================================================================================
 // First-order stage ('viscous', or 'galerkin'): one pass per stage.
 // =============================================================================================
-template <int BS, int SMB, bool ALPHA>
+template <int BS, int SMB, bool ALPHA, bool HAS_SRC>
 __global__ void __launch_bounds__(BS)
 k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
            Node* __restrict__ out, const double* __restrict__ d0, const double* __restrict__ alpha, int stage,
            int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(M, remap);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, remap);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const Consts K = T[inst].K;
@@ -376,7 +425,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
         const bool off = (j != i);
         if (SMB != SMB_GALERKIN && off)
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
-                                   d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
                                    ai, alpha, j, d, mu);
         if constexpr (SMB == SMB_1) {
             // smb_1 (update.f90:378-409)
@@ -438,7 +487,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
             r3 = r3 - topo * c.y;
         }
     }
-    add_sources(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
+    add_sources<HAS_SRC>(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
     r1 = r1 / mi;
     r2 = r2 / mi;
     r3 = r3 / mi;
@@ -466,15 +515,16 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // FCT / entropy-viscosity stage, pass 1: first-order viscosities (kept in dL, muL), the
 // low-order solution (smb_2_roundoff, update) and the entropy residual res_i.
 // =============================================================================================
-template <int BS, bool ALPHA, bool WRITE_LO>
-__global__ void __launch_bounds__(BS)
+template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
+__global__ void SWE_LB_LO(BS)
 k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node* __restrict__ ulo,
          double* __restrict__ dL, double* __restrict__ muL, double* __restrict__ res, const double* __restrict__ d0,
          const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(M, remap);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, remap);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const Consts K = T[inst].K;
@@ -491,6 +541,7 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node
     double e1 = 0.0, e2 = 0.0, e3 = 0.0;              // smb_2 of entropy_residual
     double emax = 0.0, emin = 0.0;
     const int64_t pb = row_base(M, i);
+    SWE_UNROLL(SWE_UNROLL_LO)
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
@@ -510,7 +561,7 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node
         double d = 0.0, mu = 0.0;
         if (off) {
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
-                                   d0 ? d0[p] : dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
                                    ai, alpha, j, d, mu);
             dL[p] = d;
             muL[p] = mu;
@@ -569,9 +620,9 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node
             emin = (Ej < emin) ? Ej : emin;
         }
     }
-    add_sources(K.src, I, mi, v1i, v2i, dt, K, e2, e3);
+    add_sources<HAS_SRC>(K.src, I, mi, v1i, v2i, dt, K, e2, e3);
     if (WRITE_LO) {
-        add_sources(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
+        add_sources<HAS_SRC>(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
         r1 = r1 / mi;
         r2 = r2 / mi;
         r3 = r3 / mi;
@@ -635,10 +686,11 @@ __global__ void __launch_bounds__(BS)
 k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ ulo,
              const double* __restrict__ dL, const double* __restrict__ muL, const double* __restrict__ res,
              double* __restrict__ Rm, int remap) {
-    const int i = lane_row<BS>(M, remap);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, remap);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const double dt = T[inst].dt;
@@ -649,6 +701,7 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
     const double Q = 0.0 - scaled;
     double P = -1.0e-15;
     const int64_t pb = row_base(M, i);
+    SWE_UNROLL(SWE_UNROLL_RM)
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
@@ -672,10 +725,11 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
             const Node* __restrict__ ulo, Node* __restrict__ out, const double* __restrict__ dL,
             const double* __restrict__ muL, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
             int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(M, remap);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, remap);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const double dt = T[inst].dt;
@@ -685,6 +739,7 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
     const double Ri = Rm[i];
     double x1 = 0.0, x2 = 0.0, x3 = 0.0;
     const int64_t pb = row_base(M, i);
+    SWE_UNROLL(SWE_UNROLL_AP)
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = M.col[p];
@@ -723,15 +778,16 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
 // =============================================================================================
 // Entropy-viscosity-only stage, pass 2 (update.f90:241-262).
 // =============================================================================================
-template <int BS>
+template <int BS, bool HAS_SRC>
 __global__ void __launch_bounds__(BS)
 k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
             Node* __restrict__ out, const double* __restrict__ dL, const double* __restrict__ muL,
             const double* __restrict__ res, int roundoff, int stage, int* __restrict__ flags_out, int remap) {
-    const int i = lane_row<BS>(M, remap);
-    if (i < 0) return;
-    const int inst = wave_inst(M, i);
+    const int slice = wave_slice<BS>(M, remap);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
     const Consts K = T[inst].K;
@@ -795,7 +851,7 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         r2 = r2 - topo * c.x;
         r3 = r3 - topo * c.y;
     }
-    add_sources(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
+    add_sources<HAS_SRC>(K.src, I, mi, v1i, v2i, dt, K, r2, r3);
     r1 = r1 / mi;
     r2 = r2 / mi;
     r3 = r3 / mi;
@@ -902,8 +958,10 @@ __global__ void k_fill_bc(BcEntry* __restrict__ bc, int64_t b0, int64_t n) {
 // FCT_positivity on the pool's sliced layout (fct.f90:15-41): R- per row, then l_ij.
 __global__ void k_fctpos_r(Mesh M, const double* __restrict__ f, const double* __restrict__ us, double htiny,
                            double* __restrict__ Rm) {
-    const int i = lane_row<256>(M, 0);
-    if (i < 0 || M.len[i] == 0) return;
+    const int slice = wave_slice<256>(M, 0);
+    if (slice < 0) return;
+    const int i = slice * 64 + lane_of();
+    if (M.len[i] == 0) return;
     const double Q = 0.0 - us[i];
     double P = -1.0e-15;
     const int64_t pb = row_base(M, i);
@@ -918,8 +976,10 @@ __global__ void k_fctpos_r(Mesh M, const double* __restrict__ f, const double* _
 }
 
 __global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* __restrict__ Rm, double* __restrict__ lij) {
-    const int i = lane_row<256>(M, 0);
-    if (i < 0 || M.len[i] == 0) return;
+    const int slice = wave_slice<256>(M, 0);
+    if (slice < 0) return;
+    const int i = slice * 64 + lane_of();
+    if (M.len[i] == 0) return;
     const int64_t pb = row_base(M, i);
     const int len = M.len[i];
     const double Ri = Rm[i];
@@ -995,9 +1055,14 @@ template <int SMB, bool ALPHA>
 void launch_stage_lo(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_STAGE_LO];
-    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
-                                              P.stream, M, (const InstDev*)P.tab, U, Uo, out, d0,
-                                              (const double*)P.alpha, stage, P.flags, L.remap))
+    constexpr int BS = 256;
+    const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
+    if (P.any_src)
+        hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA, true>), g, dim3(BS), 0, P.stream, M, (const InstDev*)P.tab, U,
+                           Uo, out, d0, (const double*)P.alpha, stage, P.flags, L.remap);
+    else
+        hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA, false>), g, dim3(BS), 0, P.stream, M, (const InstDev*)P.tab, U,
+                           Uo, out, d0, (const double*)P.alpha, stage, P.flags, L.remap);
 }
 
 template <bool ALPHA, bool WRITE_LO>
@@ -1005,9 +1070,17 @@ void launch_fct_lo(Pool& P, const Node* U, Node* ulo, const double* d0) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_FCT_LO];
     const int ro = P.mode.roundoff;
-    SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS),
-                                              0, P.stream, M, (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
-                                              (const double*)P.alpha, ro, P.flags, L.remap))
+    if (P.any_src) {
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, true>),
+                                                  dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
+                                                  (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
+                                                  (const double*)P.alpha, ro, P.flags, L.remap))
+    } else {
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, false>),
+                                                  dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
+                                                  (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
+                                                  (const double*)P.alpha, ro, P.flags, L.remap))
+    }
 }
 
 }  // namespace
@@ -1016,7 +1089,7 @@ const char* const kKernelNames[KID_COUNT] = {"k_compute_dt", "k_alpha", "k_stage
                                              "k_fct_rminus", "k_fct_apply", "k_ev_update"};
 
 bool block_supported(int kid, int block) {
-    (void)kid;
+    if (kid == KID_STAGE_LO || kid == KID_EV) return block == 256;
     return block == 128 || block == 256 || block == 512;
 }
 
@@ -1106,10 +1179,16 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         else launch_fct_lo<false, false>(P, U, out, d0);
         const LaunchCfg& L = P.tune.k[KID_EV];
         timed_begin(P, KID_EV);
-        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_ev_update<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
-                                                  P.stream, M, T, U, Uo, out, (const double*)P.dL,
-                                                  (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
-                                                  P.flags, L.remap))
+        constexpr int BS = 256;
+        const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
+        if (P.any_src)
+            hipLaunchKernelGGL((k_ev_update<BS, true>), g, dim3(BS), 0, P.stream, M, T, U, Uo, out,
+                               (const double*)P.dL, (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
+                               P.flags, L.remap);
+        else
+            hipLaunchKernelGGL((k_ev_update<BS, false>), g, dim3(BS), 0, P.stream, M, T, U, Uo, out,
+                               (const double*)P.dL, (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
+                               P.flags, L.remap);
         timed_end(P, KID_EV);
     }
     const hipError_t e = hipGetLastError();
