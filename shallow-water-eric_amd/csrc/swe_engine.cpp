@@ -271,8 +271,8 @@ int dalloc(T** p, size_t n) {
 }
 
 // Device bytes per unit of each capacity.
-constexpr size_t kRowBytes = sizeof(int) * 3 + sizeof(double) * 2 + sizeof(Node) * 5 + sizeof(double) * 4 +
-                             sizeof(double) * 3;   // len dpos perm, m z, S[3] scratch ulo, alpha res Rm nwork, stage3
+constexpr size_t kRowBytes = sizeof(int) * 3 + sizeof(double) * 2 + sizeof(Node) * 4 + sizeof(double) * 3 +
+                             sizeof(double) * 4 + sizeof(double) * 3;   // len dpos perm, m z, S[3] scratch, ulo, alpha res Rm nwork, stage3
 constexpr size_t kSlotBytes = sizeof(int) + sizeof(double2) * 2 + sizeof(double) * 5;  // col c nvec nrm d0 dL muL ework
 constexpr size_t kSliceBytes = sizeof(SliceInfo);
 constexpr size_t kInstBytes = sizeof(InstDev) + sizeof(unsigned long long) + sizeof(int);
@@ -281,7 +281,7 @@ constexpr size_t kBcBytes = sizeof(BcEntry) + sizeof(double);
 void free_pool(Pool* P) {
     if (!P) return;
     void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->c,     P->nvec,
-                    P->nrm, P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo, P->d0,    P->dL,
+                    P->nrm, P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dL,
                     P->muL, P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
                     P->flags, P->bc,    P->bcv};
     for (void* b : bufs)
@@ -369,7 +369,8 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     ALLOC(P->elist, (size_t)std::max<int64_t>(caps.elist, 1));
     for (int k = 0; k < 3; ++k) ALLOC(P->S[k], R);
     ALLOC(P->scratch, R);
-    ALLOC(P->ulo, R);
+    ALLOC(P->ulo_h, R);
+    ALLOC(P->ulo_q, R);
     ALLOC(P->d0, S);
     ALLOC(P->dL, S);
     ALLOC(P->muL, S);

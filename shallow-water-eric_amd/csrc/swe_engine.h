@@ -121,7 +121,8 @@ struct Pool {
     // states and work arrays
     Node* S[3] = {nullptr, nullptr, nullptr};   // S[0] holds U^n between steps
     Node* scratch = nullptr;
-    Node* ulo = nullptr;
+    double* ulo_h = nullptr;          // FCT low-order solution: depth
+    double2* ulo_q = nullptr;         // and discharge
     double *d0 = nullptr, *dL = nullptr, *muL = nullptr, *ework = nullptr;
     double *alpha = nullptr, *res = nullptr, *Rm = nullptr, *nwork = nullptr, *stage3 = nullptr;
     // instances and boundaries

@@ -165,6 +165,44 @@ __device__ __forceinline__ void first_order_dmu(const double2 c, double v1i, dou
     }
 }
 
+// alpha_limit, first loop (update.f90:758-781) for row i: alpha_i = (|sum(h_j-h_i)| /
+// sum|h_j-h_i|)**8, 1 if h_i <= htiny or the denominator does not exceed the numerator.  All index
+// loads, then all gathers, then the sums in entry order.  (Evaluating it inside the dt pass for
+// stage 1 was measured slower than this separate sweep: the gathers after the LDS phase stall.)
+__device__ __forceinline__ double alpha_of(const Mesh& M, const Node* __restrict__ U, int i, double hi, double htiny) {
+    double a = 1.0;
+    if (!(hi <= htiny)) {
+        const int64_t pb = row_base(M, i);
+        const int len = M.len[i];
+        constexpr int kUnroll = 12;
+        int js[kUnroll];
+        double hs[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) js[q] = (q < len) ? M.col[pb + 64 * q] : i;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) hs[q] = U[js[q]].h;
+        double num = 0.0, denom = 0.0;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            if (q < len) {
+                num = num + hs[q] - hi;
+                denom = denom + fabs(hs[q] - hi);
+            }
+        }
+        for (int q = kUnroll; q < len; ++q) {
+            const double hj = U[M.col[pb + 64 * (int64_t)q]].h;
+            num = num + hj - hi;
+            denom = denom + fabs(hj - hi);
+        }
+        if (denom <= fabs(num)) a = 1.0;
+        else a = fabs(num) / (denom);
+        a = a * a;
+        a = a * a;
+        a = a * a;
+    }
+    return a;
+}
+
 // =============================================================================================
 // COMPUTE_DT: d_ij for every off-diagonal entry, row sums, max |d_ii|/m_i per instance.
 // Phase 0 (one lane per row) puts the row's one_over_h and SQRT(g*ABS(h)) in LDS and an exact 0.0
@@ -347,39 +385,7 @@ k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doubl
     const int i = slice * 64 + lane_of();
     const int len = M.len[i];
     if (len == 0) return;
-    const double htiny = T[inst].K.htiny;
-    const double hi = U[i].h;
-    double a = 1.0;
-    if (!(hi <= htiny)) {
-        const int64_t pb = row_base(M, i);
-        // all index loads, then all gathers, then the sums in entry order
-        constexpr int kUnroll = 12;
-        int js[kUnroll];
-        double hs[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) js[q] = (q < len) ? M.col[pb + 64 * q] : i;
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) hs[q] = U[js[q]].h;
-        double num = 0.0, denom = 0.0;
-#pragma unroll
-        for (int q = 0; q < kUnroll; ++q) {
-            if (q < len) {
-                num = num + hs[q] - hi;
-                denom = denom + fabs(hs[q] - hi);
-            }
-        }
-        for (int q = kUnroll; q < len; ++q) {
-            const double hj = U[M.col[pb + 64 * (int64_t)q]].h;
-            num = num + hj - hi;
-            denom = denom + fabs(hj - hi);
-        }
-        if (denom <= fabs(num)) a = 1.0;
-        else a = fabs(num) / (denom);
-        a = a * a;
-        a = a * a;
-        a = a * a;
-    }
-    alpha[i] = a;
+    alpha[i] = alpha_of(M, U, i, U[i].h, T[inst].K.htiny);
 }
 
 // =============================================================================================
@@ -517,7 +523,8 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // =============================================================================================
 template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
 __global__ void SWE_LB_LO(BS)
-k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node* __restrict__ ulo,
+k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
+         double2* __restrict__ ulo_q,
          double* __restrict__ dL, double* __restrict__ muL, double* __restrict__ res, const double* __restrict__ d0,
          const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
@@ -626,14 +633,13 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, Node
         r1 = r1 / mi;
         r2 = r2 / mi;
         r3 = r3 / mi;
-        Node o;
-        if (roundoff) o.h = I.h * (1.0 + dt * fix / mi) + dt * r1;
-        else o.h = fmaxF(I.h + dt * r1, 0.0);
-        o.qx = I.qx + dt * r2;
-        o.qy = I.qy + dt * r3;
-        o.z = I.z;
-        ulo[i] = o;
-        if (K.check_h && o.h < 0.0) err |= FLAG_NEG_H;
+        // the low-order solution, depth and discharge apart: the R- pass reads the depth only
+        double hL;
+        if (roundoff) hL = I.h * (1.0 + dt * fix / mi) + dt * r1;
+        else hL = fmaxF(I.h + dt * r1, 0.0);
+        ulo_h[i] = hL;
+        ulo_q[i] = make_double2(I.qx + dt * r2, I.qy + dt * r3);
+        if (K.check_h && hL < 0.0) err |= FLAG_NEG_H;
     }
     // entropy_residual (update.f90:838-868)
     e1 = e1 / mi;
@@ -683,7 +689,7 @@ __device__ __forceinline__ double fct_flux(const Node& I, const Node& J, double 
 // =============================================================================================
 template <int BS>
 __global__ void __launch_bounds__(BS)
-k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ ulo,
+k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const double* __restrict__ ulo_h,
              const double* __restrict__ dL, const double* __restrict__ muL, const double* __restrict__ res,
              double* __restrict__ Rm, int remap) {
     const int slice = wave_slice<BS>(M, remap);
@@ -697,7 +703,7 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
     const double htiny = T[inst].K.htiny;
     const Node I = U[i];
     const double resi = res[i];
-    const double scaled = (M.m[i] / dt) * ulo[i].h;
+    const double scaled = (M.m[i] / dt) * ulo_h[i];
     const double Q = 0.0 - scaled;
     double P = -1.0e-15;
     const int64_t pb = row_base(M, i);
@@ -722,7 +728,8 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
 template <int BS>
 __global__ void __launch_bounds__(BS)
 k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
-            const Node* __restrict__ ulo, Node* __restrict__ out, const double* __restrict__ dL,
+            const double* __restrict__ ulo_h, const double2* __restrict__ ulo_q, Node* __restrict__ out,
+            const double* __restrict__ dL,
             const double* __restrict__ muL, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
             int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
@@ -756,10 +763,11 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         x3 = x3 + d * (J.qy * ratji - I.qy * ratij) + mu * (J.qy * (1.0 - ratji) - I.qy * (1.0 - ratij));
     }
     const double mi = M.m[i];
-    const Node L = ulo[i];      // low-order solution of k_fct_lo
-    const double h = L.h + dt * (x1 / mi);
-    const double qx = L.qx + dt * (x2 / mi);
-    const double qy = L.qy + dt * (x3 / mi);
+    const double hL = ulo_h[i];       // low-order solution of k_fct_lo
+    const double2 qL = ulo_q[i];
+    const double h = hL + dt * (x1 / mi);
+    const double qx = qL.x + dt * (x2 / mi);
+    const double qy = qL.y + dt * (x3 / mi);
     const int err = (check_h && h < 0.0) ? FLAG_NEG_H : 0;
     Node o;
     if (stage == 0) {
@@ -1066,19 +1074,19 @@ void launch_stage_lo(Pool& P, const Node* U, const Node* Uo, Node* out, const do
 }
 
 template <bool ALPHA, bool WRITE_LO>
-void launch_fct_lo(Pool& P, const Node* U, Node* ulo, const double* d0) {
+void launch_fct_lo(Pool& P, const Node* U, const double* d0) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_FCT_LO];
     const int ro = P.mode.roundoff;
     if (P.any_src) {
         SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, true>),
                                                   dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
+                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dL, P.muL, P.res, d0,
                                                   (const double*)P.alpha, ro, P.flags, L.remap))
     } else {
         SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, false>),
                                                   dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, ulo, P.dL, P.muL, P.res, d0,
+                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dL, P.muL, P.res, d0,
                                                   (const double*)P.alpha, ro, P.flags, L.remap))
     }
 }
@@ -1154,14 +1162,14 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         timed_end(P, KID_STAGE_LO);
     } else if (md.viscosity_type == SWE_FCT) {
         timed_begin(P, KID_FCT_LO);
-        if (alpha) launch_fct_lo<true, true>(P, U, P.ulo, d0);
-        else launch_fct_lo<false, true>(P, U, P.ulo, d0);
+        if (alpha) launch_fct_lo<true, true>(P, U, d0);
+        else launch_fct_lo<false, true>(P, U, d0);
         timed_end(P, KID_FCT_LO);
         {
             const LaunchCfg& L = P.tune.k[KID_RMINUS];
             timed_begin(P, KID_RMINUS);
             SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_rminus<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
-                                                      P.stream, M, T, U, (const Node*)P.ulo, (const double*)P.dL,
+                                                      P.stream, M, T, U, (const double*)P.ulo_h, (const double*)P.dL,
                                                       (const double*)P.muL, (const double*)P.res, P.Rm, L.remap))
             timed_end(P, KID_RMINUS);
         }
@@ -1169,14 +1177,14 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
             const LaunchCfg& L = P.tune.k[KID_APPLY];
             timed_begin(P, KID_APPLY);
             SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_apply<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
-                                                      P.stream, M, T, U, Uo, (const Node*)P.ulo, out,
+                                                      P.stream, M, T, U, Uo, (const double*)P.ulo_h, (const double2*)P.ulo_q, out,
                                                       (const double*)P.dL, (const double*)P.muL, (const double*)P.res,
                                                       (const double*)P.Rm, stage, P.flags, L.remap))
             timed_end(P, KID_APPLY);
         }
     } else {  // entropy_visc
-        if (alpha) launch_fct_lo<true, false>(P, U, out, d0);
-        else launch_fct_lo<false, false>(P, U, out, d0);
+        if (alpha) launch_fct_lo<true, false>(P, U, d0);
+        else launch_fct_lo<false, false>(P, U, d0);
         const LaunchCfg& L = P.tune.k[KID_EV];
         timed_begin(P, KID_EV);
         constexpr int BS = 256;
