@@ -230,8 +230,8 @@ int swe_compute_dij(swe_handle* h, const double* un, double* dij) {
     Pool* P = h->P;
     int rc = pool_upload_state(P, kId, un, P->scratch);
     if (rc) return rc;
-    launch_dij_full(*P, P->scratch, P->dL);
-    launch_sell_to_csr(*P, P->dL, h->csr2slot, P->ework, h->nnz);
+    launch_dij_full(*P, P->scratch, P->dL());
+    launch_sell_to_csr(*P, P->dL(), h->csr2slot, P->ework, h->nnz);
     SWE_HIPCHK(hipMemcpyAsync(dij, P->ework, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost, P->stream));
     return check_flags(P, kId);
 }
@@ -278,8 +278,8 @@ int swe_fct_positivity(swe_handle* h, const double* us, const double* f, double*
     SWE_HIPCHK(hipMemcpyAsync(P->ework, f, sizeof(double) * (size_t)h->nnz, hipMemcpyHostToDevice, P->stream));
     launch_perm_nodes(*P, I, P->stage3, P->nwork);
     launch_csr_to_sell(*P, P->ework, h->csr2slot, P->d0, h->nnz);
-    launch_fctpos(*P, P->d0, P->nwork, htiny, P->dL);
-    launch_sell_to_csr(*P, P->dL, h->csr2slot, P->ework, h->nnz);
+    launch_fctpos(*P, P->d0, P->nwork, htiny, P->dL());
+    launch_sell_to_csr(*P, P->dL(), h->csr2slot, P->ework, h->nnz);
     SWE_HIPCHK(hipMemcpyAsync(lij, P->ework, sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost, P->stream));
     SWE_HIPCHK(hipStreamSynchronize(P->stream));
     return SWE_OK;

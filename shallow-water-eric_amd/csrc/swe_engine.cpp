@@ -281,8 +281,8 @@ constexpr size_t kBcBytes = sizeof(BcEntry) + sizeof(double);
 void free_pool(Pool* P) {
     if (!P) return;
     void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->c,     P->nvec,
-                    P->nrm, P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dL,
-                    P->muL, P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
+                    P->nrm, P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dmu,
+                    P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
                     P->flags, P->bc,    P->bcv};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -372,8 +372,7 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     ALLOC(P->ulo_h, R);
     ALLOC(P->ulo_q, R);
     ALLOC(P->d0, S);
-    ALLOC(P->dL, S);
-    ALLOC(P->muL, S);
+    ALLOC(P->dmu, S);
     ALLOC(P->ework, S);
     ALLOC(P->alpha, R);
     ALLOC(P->res, R);

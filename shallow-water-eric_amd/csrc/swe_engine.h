@@ -123,7 +123,9 @@ struct Pool {
     Node* scratch = nullptr;
     double* ulo_h = nullptr;          // FCT low-order solution: depth
     double2* ulo_q = nullptr;         // and discharge
-    double *d0 = nullptr, *dL = nullptr, *muL = nullptr, *ework = nullptr;
+    double *d0 = nullptr, *ework = nullptr;
+    double2* dmu = nullptr;           // first-order viscosities (dL, muL) of the FCT passes, per slot
+    double* dL() const { return (double*)dmu; }   // the same memory as a per-slot scratch array
     double *alpha = nullptr, *res = nullptr, *Rm = nullptr, *nwork = nullptr, *stage3 = nullptr;
     // instances and boundaries
     InstDev* tab = nullptr;

@@ -525,7 +525,7 @@ template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
 __global__ void SWE_LB_LO(BS)
 k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
          double2* __restrict__ ulo_q,
-         double* __restrict__ dL, double* __restrict__ muL, double* __restrict__ res, const double* __restrict__ d0,
+         double2* __restrict__ dmu, double* __restrict__ res, const double* __restrict__ d0,
          const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
@@ -570,8 +570,7 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doub
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
                                    d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
                                    ai, alpha, j, d, mu);
-            dL[p] = d;
-            muL[p] = mu;
+            dmu[p] = make_double2(d, mu);
         }
         double xm = 0.0;
         xm = xm - c.x * (v1j);
@@ -690,7 +689,7 @@ __device__ __forceinline__ double fct_flux(const Node& I, const Node& J, double 
 template <int BS>
 __global__ void __launch_bounds__(BS)
 k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const double* __restrict__ ulo_h,
-             const double* __restrict__ dL, const double* __restrict__ muL, const double* __restrict__ res,
+             const double2* __restrict__ dmu, const double* __restrict__ res,
              double* __restrict__ Rm, int remap) {
     const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
@@ -714,7 +713,8 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
         if (j == i) continue;    // f_ii is a zero, never < 0
         const Node J = U[j];
         double dH, muH, rij_, rji_;
-        const double f = fct_flux(I, J, dL[p], muL[p], fmaxF(resi, res[j]), dH, muH, rij_, rji_);
+        const double2 lo = dmu[p];
+        const double f = fct_flux(I, J, lo.x, lo.y, fmaxF(resi, res[j]), dH, muH, rij_, rji_);
         if (f < 0.0) P = P + f;
     }
     double R = fminF(Q / P, 1.0);
@@ -729,8 +729,7 @@ template <int BS>
 __global__ void __launch_bounds__(BS)
 k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
             const double* __restrict__ ulo_h, const double2* __restrict__ ulo_q, Node* __restrict__ out,
-            const double* __restrict__ dL,
-            const double* __restrict__ muL, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
+            const double2* __restrict__ dmu, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
             int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
@@ -752,7 +751,8 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         const int j = M.col[p];
         if (j == i) continue;    // contributes d*(0) + mu*(0)
         const Node J = U[j];
-        const double dLp = dL[p], muLp = muL[p];
+        const double2 lo = dmu[p];
+        const double dLp = lo.x, muLp = lo.y;
         double dH, muH, ratij, ratji;
         const double f = fct_flux(I, J, dLp, muLp, fmaxF(resi, res[j]), dH, muH, ratij, ratji);
         const double l = (f >= 0.0) ? Rm[j] : Ri;
@@ -789,7 +789,7 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
 template <int BS, bool HAS_SRC>
 __global__ void __launch_bounds__(BS)
 k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
-            Node* __restrict__ out, const double* __restrict__ dL, const double* __restrict__ muL,
+            Node* __restrict__ out, const double2* __restrict__ dmu,
             const double* __restrict__ res, int roundoff, int stage, int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
@@ -824,8 +824,9 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         double d = 0.0, mu = 0.0;
         if (off) {
             const double rij = fmaxF(resi, res[j]);
-            d = fminF(dL[p], 1.0 * rij);
-            mu = fminF(muL[p], 1.0 * rij);
+            const double2 lo = dmu[p];
+            d = fminF(lo.x, 1.0 * rij);
+            mu = fminF(lo.y, 1.0 * rij);
         }
         if (roundoff) {
             double xm = 0.0;
@@ -1081,12 +1082,12 @@ void launch_fct_lo(Pool& P, const Node* U, const double* d0) {
     if (P.any_src) {
         SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, true>),
                                                   dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dL, P.muL, P.res, d0,
+                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
                                                   (const double*)P.alpha, ro, P.flags, L.remap))
     } else {
         SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, false>),
                                                   dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dL, P.muL, P.res, d0,
+                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
                                                   (const double*)P.alpha, ro, P.flags, L.remap))
     }
 }
@@ -1169,8 +1170,8 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
             const LaunchCfg& L = P.tune.k[KID_RMINUS];
             timed_begin(P, KID_RMINUS);
             SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_rminus<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
-                                                      P.stream, M, T, U, (const double*)P.ulo_h, (const double*)P.dL,
-                                                      (const double*)P.muL, (const double*)P.res, P.Rm, L.remap))
+                                                      P.stream, M, T, U, (const double*)P.ulo_h, (const double2*)P.dmu,
+                                                      (const double*)P.res, P.Rm, L.remap))
             timed_end(P, KID_RMINUS);
         }
         {
@@ -1178,7 +1179,7 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
             timed_begin(P, KID_APPLY);
             SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_apply<BS>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0,
                                                       P.stream, M, T, U, Uo, (const double*)P.ulo_h, (const double2*)P.ulo_q, out,
-                                                      (const double*)P.dL, (const double*)P.muL, (const double*)P.res,
+                                                      (const double2*)P.dmu, (const double*)P.res,
                                                       (const double*)P.Rm, stage, P.flags, L.remap))
             timed_end(P, KID_APPLY);
         }
@@ -1191,11 +1192,11 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
         if (P.any_src)
             hipLaunchKernelGGL((k_ev_update<BS, true>), g, dim3(BS), 0, P.stream, M, T, U, Uo, out,
-                               (const double*)P.dL, (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
+                               (const double2*)P.dmu, (const double*)P.res, md.roundoff, stage,
                                P.flags, L.remap);
         else
             hipLaunchKernelGGL((k_ev_update<BS, false>), g, dim3(BS), 0, P.stream, M, T, U, Uo, out,
-                               (const double*)P.dL, (const double*)P.muL, (const double*)P.res, md.roundoff, stage,
+                               (const double2*)P.dmu, (const double*)P.res, md.roundoff, stage,
                                P.flags, L.remap);
         timed_end(P, KID_EV);
     }
