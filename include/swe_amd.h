@@ -213,6 +213,18 @@ int swe_batch_status(swe_batch *b, int id, int *state, double *time, long long *
 int swe_batch_poll(swe_batch *b, int *ids, int cap, int *n);
 int swe_batch_get_state(swe_batch *b, int id, double *un);
 
+/* Pipelined serving: the host never waits for the step it just queued.
+ * swe_batch_step_async queues nsteps steps, then a copy of the instance table, and returns that
+ * copy's ticket.  swe_batch_finished waits for the copy of one ticket only (steps queued after it
+ * keep running) and lists the instances it shows done or failed; they take no later launches.
+ * swe_batch_result then reads such an instance's final state, time, step count and error on the
+ * copy stream, again without waiting for the queued steps.  An instance admitted after a ticket
+ * was taken is not reported by that ticket. */
+int swe_batch_step_async(swe_batch *b, int nsteps, long long *ticket);
+int swe_batch_finished(swe_batch *b, long long ticket, int *ids, int cap, int *n);
+int swe_batch_result(swe_batch *b, int id, double *un, int *state, double *time, long long *steps,
+                     int *error);
+
 /* Free capacity: largest free run of node rows and entry slots, free instance records. */
 int swe_batch_free(swe_batch *b, long long *rows, long long *entries, int *instances);
 

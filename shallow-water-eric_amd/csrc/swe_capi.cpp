@@ -43,8 +43,9 @@ int check_flags(Pool* P, int k) {
     if (f) {
         // report once; the instance may step again afterwards, as the reference run would not
         const int z = 0, st = INST_RUNNING;
-        SWE_HIPCHK(hipMemcpyAsync(P->flags + k, &z, sizeof z, hipMemcpyHostToDevice, P->stream));
-        SWE_HIPCHK(hipMemcpyAsync(&P->tab[k].state, &st, sizeof st, hipMemcpyHostToDevice, P->stream));
+        int rc = launch_poke(*P, P->flags + k, &z, sizeof z);
+        if (!rc) rc = launch_poke(*P, &P->tab[k].state, &st, sizeof st);
+        if (rc) return rc;
         SWE_HIPCHK(hipStreamSynchronize(P->stream));
         if (f & FLAG_LAMBDA) return fail(SWE_ELAMBDA, "BUG in compute_lambda_vacc (Delta < 0)");
         if (f & FLAG_NEG_H) return fail(SWE_ENEG_H, "Min h<0, STOP");
@@ -471,6 +472,39 @@ int swe_batch_step(swe_batch* b, int nsteps) {
     if (!b) return fail(SWE_EINVAL, "null batch");
     if (nsteps < 0) return fail(SWE_EINVAL, "nsteps < 0");
     return pool_queue_steps(b->P, nsteps);
+}
+
+int swe_batch_step_async(swe_batch* b, int nsteps, long long* ticket) {
+    if (!b || !ticket) return fail(SWE_EINVAL, "null argument");
+    if (nsteps < 0) return fail(SWE_EINVAL, "nsteps < 0");
+    int rc = pool_queue_steps(b->P, nsteps);
+    if (rc) return rc;
+    return pool_snapshot(b->P, ticket);
+}
+
+int swe_batch_finished(swe_batch* b, long long ticket, int* ids, int cap, int* n) {
+    if (!b || !n || (cap > 0 && !ids)) return fail(SWE_EINVAL, "null argument");
+    return pool_finished(b->P, ticket, ids, cap, n);
+}
+
+int swe_batch_result(swe_batch* b, int id, double* un, int* state, double* time, long long* steps, int* error) {
+    if (!b || !live(b->P, id)) return fail(SWE_EINVAL, "no instance %d", id);
+    const InstHost& I = b->P->inst[(size_t)id];
+    if (!I.final_known) return fail(SWE_EINVAL, "instance %d has not been reported finished", id);
+    if (un) {
+        int rc = pool_result(b->P, id, un);
+        if (rc) return rc;
+    }
+    const InstDev& D = I.final_rec;
+    const int f = I.final_flags;
+    int st = D.state;
+    if (st == INST_RUNNING && f) st = INST_FAILED;
+    else if (st == INST_RUNNING) st = INST_DONE;
+    if (state) *state = st;
+    if (time) *time = D.time;
+    if (steps) *steps = D.steps;
+    if (error) *error = (f & FLAG_LAMBDA) ? SWE_ELAMBDA : (f & FLAG_NEG_H) ? SWE_ENEG_H : SWE_OK;
+    return SWE_OK;
 }
 
 int swe_batch_sync(swe_batch* b) {

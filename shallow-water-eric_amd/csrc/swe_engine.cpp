@@ -291,6 +291,11 @@ void free_pool(Pool* P) {
         if (P->act_host[k]) (void)hipHostFree(P->act_host[k]);
         if (P->act_ev[k]) (void)hipEventDestroy(P->act_ev[k]);
     }
+    for (auto& sn : P->snaps) {
+        if (sn.tab) (void)hipHostFree(sn.tab);
+        if (sn.flags) (void)hipHostFree(sn.flags);
+        if (sn.ev) (void)hipEventDestroy(sn.ev);
+    }
     if (P->tab_host) (void)hipHostFree(P->tab_host);
     if (P->flags_host) (void)hipHostFree(P->flags_host);
     if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
@@ -396,7 +401,12 @@ int pool_create(const PoolCaps& caps, Pool** out) {
         free_pool(P);
         return fail(SWE_EHIP, "pool lists: %s", hipGetErrorString(e));
     }
-    e = hipHostMalloc((void**)&P->tab_host, sizeof(InstDev) * (size_t)caps.instances, hipHostMallocDefault);
+    for (auto& sn : P->snaps) {
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sn.tab, sizeof(InstDev) * (size_t)caps.instances, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&sn.flags, sizeof(int) * (size_t)caps.instances, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sn.ev, hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipHostMalloc((void**)&P->tab_host, sizeof(InstDev) * (size_t)caps.instances, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&P->flags_host, sizeof(int) * (size_t)caps.instances, hipHostMallocDefault);
     if (e != hipSuccess) {
         free_pool(P);
@@ -506,6 +516,7 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     InstHost& I = P->inst[(size_t)k];
     I = InstHost{};
     I.used = true;
+    I.since = P->next_ticket;
     I.np = d.np;
     I.nnz = d.nnz;
     I.np_pad = L.np_pad;
@@ -538,13 +549,12 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     D.max_steps = max_steps;
     D.state = INST_RUNNING;
     D.bc_kind = SWE_BC_NONE;
+    static_assert(sizeof(InstDev) <= sizeof(PokeWords), "an instance record is written in one edit");
     const unsigned long long z64 = 0ull;
     const int z32 = 0;
-    e = hipMemcpyAsync(P->tab + k, &D, sizeof D, hipMemcpyHostToDevice, P->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(P->vmax + k, &z64, sizeof z64, hipMemcpyHostToDevice, P->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(P->flags + k, &z32, sizeof z32, hipMemcpyHostToDevice, P->stream);
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e != hipSuccess) return fail(SWE_EHIP, "activation of an admitted mesh: %s", hipGetErrorString(e));
+    if ((rc = launch_poke(*P, P->tab + k, &D, sizeof D))) return rc;
+    if ((rc = launch_poke(*P, P->vmax + k, &z64, sizeof z64))) return rc;
+    if ((rc = launch_poke(*P, P->flags + k, &z32, sizeof z32))) return rc;
     P->listed[(size_t)k] = 1;
     if ((rc = pool_update_active(P))) return rc;
     *id = k;
@@ -580,8 +590,9 @@ int pool_remove(Pool* P, int k) {
     launch_fill_slices(*P, I.slice0, I.nslices, -1);
     launch_fill_bc(*P, I.bc0, I.nbc);
     const int st = INST_FREE, go = 0;
-    SWE_HIPCHK(hipMemcpyAsync(&P->tab[k].state, &st, sizeof st, hipMemcpyHostToDevice, P->stream));
-    SWE_HIPCHK(hipMemcpyAsync(&P->tab[k].go, &go, sizeof go, hipMemcpyHostToDevice, P->stream));
+    int rc = launch_poke(*P, &P->tab[k].state, &st, sizeof st);
+    if (!rc) rc = launch_poke(*P, &P->tab[k].go, &go, sizeof go);
+    if (rc) return rc;
     SWE_HIPCHK(hipEventRecord(P->released, P->stream));
     P->a_slices.release(I.slice0, I.nslices);
     P->a_slots.release(I.slot0, I.nslots);
@@ -589,8 +600,7 @@ int pool_remove(Pool* P, int k) {
     P->a_bc.release(I.bc0, I.nbc);
     I = InstHost{};
     P->listed[(size_t)k] = 0;
-    int rc = pool_update_active(P);
-    if (rc) return rc;
+    if ((rc = pool_update_active(P))) return rc;
     bool any = false;
     for (const auto& q : P->inst) any |= q.used;
     if (!any) {
@@ -601,8 +611,7 @@ int pool_remove(Pool* P, int k) {
 }
 
 int pool_set_field(Pool* P, int k, size_t offset, const void* src, size_t bytes) {
-    SWE_HIPCHK(hipMemcpyAsync((char*)(P->tab + k) + offset, src, bytes, hipMemcpyHostToDevice, P->stream));
-    return SWE_OK;
+    return launch_poke(*P, (char*)(P->tab + k) + offset, src, bytes);
 }
 
 int pool_set_params(Pool* P, int k, const swe_params& prm) {
@@ -719,6 +728,56 @@ int pool_queue_steps(Pool* P, int nsteps) {
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(SWE_EHIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return SWE_OK;
+}
+
+int pool_snapshot(Pool* P, long long* ticket) {
+    Snapshot& sn = P->snaps[P->next_ticket % 4];
+    if (sn.ticket >= 0) SWE_HIPCHK(hipEventSynchronize(sn.ev));   // its buffers are about to be reused
+    const int n = std::max(P->ninst(), 1);
+    SWE_HIPCHK(hipMemcpyAsync(sn.tab, P->tab, sizeof(InstDev) * (size_t)n, hipMemcpyDeviceToHost, P->stream));
+    SWE_HIPCHK(hipMemcpyAsync(sn.flags, P->flags, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, P->stream));
+    SWE_HIPCHK(hipEventRecord(sn.ev, P->stream));
+    sn.n = n;
+    sn.ticket = P->next_ticket++;
+    *ticket = sn.ticket;
+    return SWE_OK;
+}
+
+int pool_finished(Pool* P, long long ticket, int* ids, int cap, int* n) {
+    *n = 0;
+    Snapshot& sn = P->snaps[ticket % 4];
+    if (ticket < 0 || sn.ticket != ticket) return fail(SWE_EINVAL, "snapshot %lld is not available", ticket);
+    SWE_HIPCHK(hipEventSynchronize(sn.ev));
+    const int ni = std::min(P->ninst(), sn.n);
+    bool unlisted = false;
+    for (int k = 0; k < ni; ++k) {
+        InstHost& I = P->inst[(size_t)k];
+        // an instance admitted after the snapshot was queued is not in it (the slot may show the
+        // record of an instance removed since)
+        if (!I.used || I.final_known || ticket < I.since) continue;
+        const InstDev& D = sn.tab[k];
+        const bool over = D.state != INST_RUNNING || sn.flags[k] != 0 || !(D.time < D.tfinal && D.steps < D.max_steps);
+        if (!over) continue;
+        I.final_known = true;
+        I.final_rec = D;
+        I.final_flags = sn.flags[k];
+        if (P->listed[(size_t)k]) {
+            P->listed[(size_t)k] = 0;
+            unlisted = true;
+        }
+        if (*n < cap) ids[(*n)++] = k;
+    }
+    return unlisted ? pool_update_active(P) : SWE_OK;
+}
+
+int pool_result(Pool* P, int k, double* un) {
+    const InstHost& I = P->inst[(size_t)k];
+    if (!I.final_known) return fail(SWE_EINVAL, "instance %d has not been reported finished", k);
+    double* st = P->stage3 + 3 * I.slice0 * 64;
+    launch_unpack(*P, I, P->S[0], st, P->copy);
+    SWE_HIPCHK(hipMemcpyAsync(un, st, sizeof(double) * 3 * (size_t)I.np, hipMemcpyDeviceToHost, P->copy));
+    SWE_HIPCHK(hipStreamSynchronize(P->copy));
     return SWE_OK;
 }
 

@@ -68,6 +68,10 @@ int make_consts(const swe_params& p, Consts& K);
 // ---- the pool ---------------------------------------------------------------------------------
 struct InstHost {
     bool used = false;
+    bool final_known = false;         // reported finished by a snapshot: its rows no longer change
+    InstDev final_rec{};              // that snapshot's record
+    int final_flags = 0;
+    long long since = 0;              // first table snapshot ticket that shows this instance
     int np = 0, nnz = 0, np_pad = 0, wmax = 0;
     int64_t slice0 = 0, nslices = 0, slot0 = 0, nslots = 0, e0 = 0, ne = 0, bc0 = 0, nbc = 0;
     int nh = 0, nux = 0, nuy = 0;
@@ -78,6 +82,15 @@ struct Mode {
     int viscosity_type = SWE_VISCOUS, viscous_type = 2, alpha = 0, roundoff = 1;
 };
 int mode_of(const swe_params& p, Mode& m);
+
+// Table snapshots taken after queued steps (pinned copies, completion event, ticket).
+struct Snapshot {
+    InstDev* tab = nullptr;
+    int* flags = nullptr;
+    hipEvent_t ev = nullptr;
+    long long ticket = -1;
+    int n = 0;                        // records copied
+};
 
 struct TimedKernel {
     std::vector<hipEvent_t> ev;   // start/stop pairs
@@ -135,6 +148,8 @@ struct Pool {
     double* bcv = nullptr;
     InstDev* tab_host = nullptr;      // pinned read-back of the table
     int* flags_host = nullptr;        // pinned read-back of the flags
+    Snapshot snaps[4];
+    long long next_ticket = 0;
     // schedules, graph, timing
     Tuning tune;
     bool use_graph = false;
@@ -171,6 +186,13 @@ int pool_queue_steps(Pool* P, int nsteps);
 // Rebuild the launch list from the instances marked listed (ordered on the compute stream).
 int pool_update_active(Pool* P);
 int pool_read_table(Pool* P);         // sync, then tab_host / flags_host hold the table
+// Queue a copy of the table after the work queued so far; returns its ticket.
+int pool_snapshot(Pool* P, long long* ticket);
+// Wait for a snapshot (not for later work) and list the instances it shows done or failed; they
+// leave the launch list and their final records are kept for pool_result.
+int pool_finished(Pool* P, long long ticket, int* ids, int cap, int* n);
+// Final state of an instance reported by pool_finished, read on the copy stream while steps run.
+int pool_result(Pool* P, int id, double* un);
 int pool_tune(Pool* P, int steps, int trials);
 
 // ---- launch wrappers (swe_kernels.hip) --------------------------------------------------------
@@ -181,13 +203,19 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
 void launch_bdy(Pool& P, Node* u, int stage);
 void launch_dij_full(Pool& P, const Node* U, double* dij);
 void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst);
-void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3);
+void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3, hipStream_t stream = nullptr);
 void launch_perm_nodes(Pool& P, const InstHost& I, const double* src, double* dst);
 void launch_csr_to_sell(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz);
 void launch_sell_to_csr(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz);
 void launch_fctpos(Pool& P, const double* f, const double* us, double htiny, double* lij);
 void launch_fill_slices(Pool& P, int64_t s0, int64_t n, int inst);
 void launch_fill_bc(Pool& P, int64_t b0, int64_t n);
+// Queue a write of up to sizeof(PokeWords) bytes (whole 4-byte words) on the compute stream.
+constexpr int kPokeWords = 128;
+struct PokeWords {
+    uint32_t w[kPokeWords];
+};
+int launch_poke(Pool& P, void* dst, const void* src, size_t bytes);
 int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, const double* us, double htiny,
                       double* lij);
 

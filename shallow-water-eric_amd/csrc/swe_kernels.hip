@@ -31,6 +31,8 @@
 // sign of a zero.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "swe_engine.h"
 #include "swe_kernels.h"
 
@@ -959,6 +961,12 @@ __global__ void k_fill_slices(SliceInfo* __restrict__ sl, int64_t s0, int64_t n,
     if (k < n) sl[s0 + k].inst = inst;
 }
 
+// Small table edits ride in the kernel arguments, so queuing one never waits for the stream.
+__global__ void k_poke(uint32_t* __restrict__ dst, PokeWords w, int nwords) {
+    const int k = threadIdx.x;
+    if (k < nwords) dst[k] = w.w[k];
+}
+
 __global__ void k_fill_bc(BcEntry* __restrict__ bc, int64_t b0, int64_t n) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) bc[b0 + k].inst = -1;
@@ -1225,9 +1233,9 @@ void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst) {
                        (const int*)(P.perm + r0), dst + r0, I.np);
 }
 
-void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3) {
+void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3, hipStream_t stream) {
     const int64_t r0 = I.slice0 * 64;
-    hipLaunchKernelGGL(k_unpack, dim3(blocks_of(I.np, 256)), dim3(256), 0, P.stream, src + r0,
+    hipLaunchKernelGGL(k_unpack, dim3(blocks_of(I.np, 256)), dim3(256), 0, stream ? stream : P.stream, src + r0,
                        (const int*)(P.perm + r0), un3, I.np);
 }
 
@@ -1256,6 +1264,17 @@ void launch_fctpos(Pool& P, const double* f, const double* us, double htiny, dou
 void launch_fill_slices(Pool& P, int64_t s0, int64_t n, int inst) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_slices, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, P.sl, s0, n, inst);
+}
+
+int launch_poke(Pool& P, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return SWE_OK;
+    if (bytes % 4 || bytes > sizeof(PokeWords) || ((uintptr_t)dst & 3))
+        return fail(SWE_EINVAL, "table edit of %zu bytes is not a whole number of aligned words", bytes);
+    PokeWords w;
+    std::memcpy(w.w, src, bytes);
+    hipLaunchKernelGGL(k_poke, dim3(1), dim3(kPokeWords), 0, P.stream, (uint32_t*)dst, w, (int)(bytes / 4));
+    SWE_HIPCHK(hipGetLastError());
+    return SWE_OK;
 }
 
 void launch_fill_bc(Pool& P, int64_t b0, int64_t n) {

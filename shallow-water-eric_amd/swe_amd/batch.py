@@ -126,6 +126,32 @@ class Batch:
         _check(load_library().swe_batch_get_state(self.h, int(k), _ptr(out)))
         return out
 
+    # -- pipelined serving ----------------------------------------------------------------------------
+    def step_async(self, nsteps=1):
+        """Queue nsteps steps and a copy of the instance table; returns the copy's ticket."""
+        t = C.c_longlong()
+        _check(load_library().swe_batch_step_async(self.h, int(nsteps), C.byref(t)))
+        return t.value
+
+    def finished(self, ticket, cap=4096):
+        """Ids of the instances that the table copy of `ticket` shows done or failed (waits for
+        that copy only; they take no later launches)."""
+        ids = np.zeros(cap, np.int32)
+        n = C.c_int()
+        _check(load_library().swe_batch_finished(self.h, int(ticket), _ptr(ids), cap, C.byref(n)))
+        return [int(x) for x in ids[:n.value]]
+
+    def result(self, k):
+        """(state, (np, 3) final state, time, steps, error) of an instance reported by
+        :meth:`finished`, read while later steps run."""
+        out = np.zeros((self._np[k], 3))
+        st, err = C.c_int(), C.c_int()
+        t = C.c_double()
+        n = C.c_longlong()
+        _check(load_library().swe_batch_result(self.h, int(k), _ptr(out), C.byref(st), C.byref(t), C.byref(n),
+                                               C.byref(err)))
+        return st.value, out, t.value, n.value, err.value
+
     def free(self):
         """(largest free run of node rows, of entry slots, free instance records)."""
         r, e = C.c_longlong(), C.c_longlong()

@@ -2,16 +2,20 @@
 one device-resident batch and advanced together (continuous batching), with time-to-solution
 recorded per request.
 
-:class:`Server` runs the loop: finished instances are collected (state read back, rows freed),
-the next chunk of steps is queued for everything resident, and while the GPU runs it the host
-admits waiting requests (FIFO) into the rows that are free, against the batch's capacity, which
-is set from a memory budget.  A request joins the steps queued after its admission and leaves
-when its time reaches its final time (the loop condition of SOURCES/main.f90:61), so its result
-is the one it would get alone.  :func:`run_plain` serves the same stream one request at a time
-on the same engine: the plain per-instance loop the server is measured against.
+:class:`Server` runs a pipelined loop that never lets the GPU drain while requests are resident:
+it queues the next chunk of steps (which ends with a copy of the instance table, its ticket),
+then waits only for the previous chunk's table copy, reads the results of the instances that copy
+shows finished (on the copy stream, while the queued chunk runs), frees their rows, and admits
+waiting requests (FIFO) into the free rows, against the batch's capacity, which is set from a
+memory budget.  A request joins the steps queued after its admission and leaves when its time
+reaches its final time (the loop condition of SOURCES/main.f90:61), so its result is the one it
+would get alone.  :func:`run_plain` serves the same stream one request at a time on the same
+engine: the plain per-instance loop the server is measured against.
 
-The engine interface (add / step / finished / collect) is what the loop needs from a batch;
-:class:`BatchEngine` binds it to :class:`swe_amd.batch.Batch`.
+The engine interface (add / step / finished / collect) is what the loop needs from a batch:
+``step(n)`` queues n steps and returns a ticket, ``finished(ticket)`` lists the instances that
+ticket's table copy shows done or failed, ``collect(k)`` returns (state, time, steps, error) and
+frees the instance.  :class:`BatchEngine` binds it to :class:`swe_amd.batch.Batch`.
 """
 from __future__ import annotations
 
@@ -56,20 +60,20 @@ class BatchEngine:
         return self.batch.add_problem(req.problem, tfinal=req.tfinal, max_steps=MAX_STEPS)
 
     def step(self, n):
-        self.batch.step(n)
+        return self.batch.step_async(n)
 
-    def finished(self):
-        return self.batch.poll()
+    def finished(self, ticket):
+        return self.batch.finished(ticket)
 
     def collect(self, k):
-        st, t, steps, err = self.batch.status(k)
-        u = self.batch.get_state(k)
+        st, u, t, steps, err = self.batch.result(k)
         self.batch.remove(k)
         return u, t, steps, err
 
 
 class Server:
-    """Continuous batching: requests join and leave a running batch between chunks of steps."""
+    """Continuous batching: requests join and leave a running batch between chunks of steps, with
+    one chunk always queued ahead of the one the host is reading."""
 
     def __init__(self, engine, chunk=16, clock=time.perf_counter, sleep=time.sleep):
         self.engine = engine
@@ -83,19 +87,19 @@ class Server:
         pending = collections.deque(reqs)
         waiting = collections.deque()
         active = {}
+        prev = None
         t0 = self.clock()
         while pending or waiting or active:
             now = self.clock() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            if active:
-                for k in self.engine.finished():
+            # queue the next chunk before reading the last one, so the GPU stays busy meanwhile
+            ticket = self.engine.step(self.chunk) if active else None
+            if prev is not None:
+                for k in self.engine.finished(prev):
                     r = active.pop(k)
                     r.state, r.time, r.steps, r.error = self.engine.collect(k)
                     r.finished = self.clock() - t0
-            if active:
-                self.engine.step(self.chunk)          # the GPU runs this while the host admits
-            admitted = 0
             while waiting:
                 r = waiting[0]
                 k = self.engine.add(r)
@@ -106,7 +110,7 @@ class Server:
                 waiting.popleft()
                 active[k] = r
                 r.admitted = self.clock() - t0
-                admitted += 1
+            prev = ticket
             self.max_resident = max(self.max_resident, len(active))
             if not active and not waiting and pending:
                 self.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
@@ -114,7 +118,8 @@ class Server:
 
 
 def run_plain(engine, requests, chunk=16, clock=time.perf_counter, sleep=time.sleep):
-    """The same stream served one request at a time, in arrival order."""
+    """The same stream served one request at a time, in arrival order: queue a chunk, wait for
+    it, look whether the request is done."""
     reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
     t0 = clock()
     for r in reqs:
@@ -125,10 +130,8 @@ def run_plain(engine, requests, chunk=16, clock=time.perf_counter, sleep=time.sl
         k = engine.add(r)
         if k is None:
             raise RuntimeError("request %d (%d nodes) does not fit the engine" % (r.rid, r.problem.np))
-        while True:
-            engine.step(chunk)
-            if k in engine.finished():
-                break
+        while k not in engine.finished(engine.step(chunk)):
+            pass
         r.state, r.time, r.steps, r.error = engine.collect(k)
         r.finished = clock() - t0
     return reqs

@@ -94,6 +94,9 @@ SIGNATURES = {
     "swe_batch_status": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_D), C.POINTER(C.c_longlong), C.POINTER(_I)]),
     "swe_batch_poll": (_I, [_P, _P, _I, C.POINTER(_I)]),
     "swe_batch_get_state": (_I, [_P, _I, _P]),
+    "swe_batch_step_async": (_I, [_P, _I, C.POINTER(C.c_longlong)]),
+    "swe_batch_finished": (_I, [_P, C.c_longlong, _P, _I, C.POINTER(_I)]),
+    "swe_batch_result": (_I, [_P, _I, _P, C.POINTER(_I), C.POINTER(_D), C.POINTER(C.c_longlong), C.POINTER(_I)]),
     "swe_batch_free": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(_I)]),
     "swe_batch_tune": (_I, [_P, _I]),
     "swe_batch_use_graph": (_I, [_P, _I]),

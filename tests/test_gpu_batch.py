@@ -205,3 +205,37 @@ def test_served_results_equal_alone_and_in_the_mix():
         s.set_state(r.problem.u0)
         t, _ = s.run(r.steps, time=0.0)
         assert t == r.time and same_bits(s.get_state(), r.state)
+
+
+def test_tickets_report_finished_instances_once():
+    """Pipelined reads: a ticket's table copy reports an instance once it is done, the result read
+    while later steps run equals the state a full synchronisation reads, and an instance admitted
+    into a reused record after a ticket was taken is not reported by that ticket."""
+    from swe_amd.batch import INSTANCE_DONE
+    ps = fct_problems()[:2]
+    with new_batch(ps) as b:
+        a = b.add_problem(ps[0], max_steps=10)
+        c = b.add_problem(ps[1], max_steps=30)
+        t1 = b.step_async(12)
+        t2 = b.step_async(4)                      # queued before t1 is read
+        assert b.finished(t1) == [a]
+        st, ua, ta, na, ea = b.result(a)
+        assert st == INSTANCE_DONE and na == 10 and ea == 0
+        assert b.finished(t2) == []               # reported once
+        b.sync()
+        st2, tb, nb, eb = b.status(a)
+        assert (st2, tb, nb, eb) == (st, ta, na, ea) and same_bits(b.get_state(a), ua)
+        u_alone, t_alone, _ = run_alone(ps[0], 10)
+        assert same_bits(ua, u_alone) and ta == t_alone
+        b.remove(a)
+        t3 = b.step_async(1)
+        a2 = b.add_problem(ps[0], max_steps=2)    # takes the freed record
+        assert a2 == a
+        b.step(3)
+        assert b.finished(t3) == []               # t3 was taken before a2 was admitted
+        t4 = b.step_async(20)
+        assert sorted(b.finished(t4)) == sorted([a2, c])
+        st, u2, t2_, n2, _ = b.result(a2)
+        assert n2 == 2 and same_bits(u2, run_alone(ps[0], 2)[0])
+        st, uc, tc, nc, _ = b.result(c)
+        assert nc == 30 and same_bits(uc, run_alone(ps[1], 30)[0])

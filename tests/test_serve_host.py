@@ -28,7 +28,9 @@ class FakeProblem:
 
 class FakeEngine:
     """Instances of `np` rows in a pool of `capacity` rows; each step advances an instance's time by
-    its dt while time < tfinal; a step costs time proportional to the resident rows."""
+    its dt while time < tfinal; a step costs time proportional to the resident rows.  step()
+    returns a ticket whose record lists the instances finished when that chunk ended; an
+    instance admitted after a ticket is not in its record."""
 
     def __init__(self, capacity, clock):
         self.capacity = capacity
@@ -37,6 +39,9 @@ class FakeEngine:
         self.next = 0
         self.max_rows = 0
         self.order = []
+        self.tickets = []
+        self.reported = set()
+        self.drained = 0          # reads of the newest ticket while instances were still running
 
     def add(self, r):
         used = sum(i["np"] for i in self.inst.values())
@@ -56,11 +61,18 @@ class FakeEngine:
                     i["time"] = i["time"] + i["dt"]
                     i["steps"] += 1
             self.clock.t += 1e-6 * sum(i["np"] for i in self.inst.values())
+        self.tickets.append({k for k, i in self.inst.items() if not i["time"] < i["tfinal"]})
+        return len(self.tickets) - 1
 
-    def finished(self):
-        return [k for k, i in self.inst.items() if not i["time"] < i["tfinal"]]
+    def finished(self, ticket):
+        if ticket == len(self.tickets) - 1 and any(i["time"] < i["tfinal"] for i in self.inst.values()):
+            self.drained += 1
+        out = sorted(k for k in self.tickets[ticket] if k in self.inst and k not in self.reported)
+        self.reported.update(out)
+        return out
 
     def collect(self, k):
+        assert k in self.reported                     # only reported instances are collected
         i = self.inst.pop(k)
         return None, i["time"], i["steps"], 0
 
@@ -81,6 +93,7 @@ def test_server_completes_every_request_within_capacity(chunk):
     assert all(r.finished is not None for r in reqs)
     assert eng.max_rows <= 1000
     assert eng.order == sorted(eng.order)                       # FIFO admission
+    assert eng.drained == 0                                     # a chunk is always queued ahead
     for r in reqs:
         assert r.admitted >= r.arrival and r.finished >= r.admitted
         # DO WHILE (time < tfinal): the first step count whose time reaches tfinal
