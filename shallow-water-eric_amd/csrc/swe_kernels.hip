@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <type_traits>
 
 #include "swe_engine.h"
 #include "swe_kernels.h"
@@ -523,7 +524,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // FCT / entropy-viscosity stage, pass 1: first-order viscosities (kept in dL, muL), the
 // low-order solution (smb_2_roundoff, update) and the entropy residual res_i.
 // =============================================================================================
-template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
+template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC, bool D0>
 __global__ void SWE_LB_LO(BS)
 k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
          double2* __restrict__ ulo_q,
@@ -569,9 +570,12 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doub
         const bool off = (j != i);
         double d = 0.0, mu = 0.0;
         if (off) {
-            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
-                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
-                                   ai, alpha, j, d, mu);
+            // D0: compute_dij at U comes from the dt pass (stage 1), so this kernel carries no
+            // lambda code and needs fewer registers (5 waves per SIMD instead of 4)
+            double dlam;
+            if constexpr (D0) dlam = d0[p];
+            else dlam = dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
+            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j, dlam, ai, alpha, j, d, mu);
             dmu[p] = make_double2(d, mu);
         }
         double xm = 0.0;
@@ -1087,16 +1091,21 @@ void launch_fct_lo(Pool& P, const Node* U, const double* d0) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_FCT_LO];
     const int ro = P.mode.roundoff;
+    auto go = [&](auto src, auto has_d0) {
+        constexpr bool S = decltype(src)::value, D = decltype(has_d0)::value;
+        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, S, D>),
+                                                  dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
+                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
+                                                  (const double*)P.alpha, ro, P.flags, L.remap))
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
     if (P.any_src) {
-        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, true>),
-                                                  dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
-                                                  (const double*)P.alpha, ro, P.flags, L.remap))
+        if (d0) go(T_{}, T_{});
+        else go(T_{}, F_{});
     } else {
-        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, false>),
-                                                  dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
-                                                  (const double*)P.alpha, ro, P.flags, L.remap))
+        if (d0) go(F_{}, T_{});
+        else go(F_{}, F_{});
     }
 }
 
