@@ -2,7 +2,7 @@
  *
  * The library replaces the explicit time step of t-bltg/shallow-water-eric: MODULE update
  * (SOURCES/update.f90) and MODULE fct (SOURCES/fct.f90), with the pointwise physics they call in
- * SOURCES/condlim.f90, and the per-step work of the driver's time loop (SOURCES/main.f90:105-136:
+ * SOURCES/condlim.f90, and the per-step work of the driver's time loop (SOURCES/main.f90:61-92:
  * COMPUTE_DT, the SSP-RK3 combinations and bdy).  Every entry point takes plain pointers and
  * sizes; arrays keep the reference's layout: 1-based CSR (ia(np+1), ja(nnz), columns ascending,
  * diagonal included, as st_csr builds it), state un(3,np) in Fortran order (h, qx, qy per node),
@@ -97,14 +97,14 @@ int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **ou
 int swe_destroy(swe_handle *h);
 int swe_set_params(swe_handle *h, const swe_params *prm);
 
-/* Dirichlet program applied by the device-resident steps (bdy, SOURCES/main.f90:220-232). */
+/* Dirichlet program applied by the device-resident steps (bdy, SOURCES/main.f90:176-188). */
 int swe_set_bc(swe_handle *h, int kind, double amp, double period, double level);
 int swe_set_bc_values(swe_handle *h, const double *h_vals, const double *ux_vals, const double *uy_vals);
 
 /* compute_dij (SOURCES/update.f90:291-326): dij%aa at state un(3,np) (host arrays). */
 int swe_compute_dij(swe_handle *h, const double *un, double *dij);
 
-/* COMPUTE_DT (SOURCES/main.f90:210-218): dt = CFL/MAXVAL(ABS(dij%aa(diag))/lumped).
+/* COMPUTE_DT (SOURCES/main.f90:166-174): dt = CFL/MAXVAL(ABS(dij%aa(diag))/lumped).
  * un == NULL uses the device-resident state. */
 int swe_compute_dt(swe_handle *h, const double *un, double *dt);
 
@@ -129,7 +129,7 @@ int swe_get_state(swe_handle *h, double *un);
 int swe_set_time(swe_handle *h, double t);
 int swe_get_time(swe_handle *h, double *t);
 
-/* nsteps SSP-RK3 steps on the device-resident state, as in SOURCES/main.f90:111-135 with a
+/* nsteps SSP-RK3 steps on the device-resident state, as in SOURCES/main.f90:71-91 with a
  * fixed step count: COMPUTE_DT, three euler stages, the combinations (3*uo+ui)/4 and
  * (uo+2*ui)/3, and bdy after each stage.  If time is not NULL the run starts at *time and
  * *time receives the end time; dts (may be NULL) receives dt of each step.  Returns

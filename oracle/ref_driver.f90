@@ -9,7 +9,7 @@
 !   ref_driver fct <in> <out>
 !       FCT_positivity (SOURCES/fct.f90:5-42) on a given CSR matrix of antidiffusive fluxes.
 !   ref_driver setup <data> <init|none> <outdir>
-!       read_my_data + construct_mesh + construct_matrices + init (SOURCES/main.f90:63-75) and
+!       read_my_data + construct_mesh + construct_matrices + init (SOURCES/main.f90:19-40) and
 !       dump the mesh, CSR pattern, mass, lumped mass, diag index, c_ij, bathymetry, U0 and the
 !       Dirichlet node lists.
 !   ref_driver dij <data> <init|none> <state> <out>
@@ -18,7 +18,7 @@
 !       one forward-Euler stage, euler(un,unext) (SOURCES/update.f90:134-289); writes unext
 !       and dij%aa as left by euler.
 !   ref_driver run <data> <init|none> <nsteps> <outdir> [dump_every]
-!       nsteps SSP-RK3 steps as in the time loop of SOURCES/main.f90:105-136 (fixed step count
+!       nsteps SSP-RK3 steps as in the time loop of SOURCES/main.f90:61-92 (fixed step count
 !       instead of the Tfinal loop); writes dt per step, U after each stage of step 1, the final
 !       U and optional snapshots.
 !
@@ -136,7 +136,7 @@ CONTAINS
   END SUBROUTINE do_fct
 
   !---------------------------------------------------------------------------------------------
-  ! Problem set-up as in SOURCES/main.f90:63-75.
+  ! Problem set-up as in SOURCES/main.f90:19-40.
   SUBROUTINE prepare(datafile, initfile)
     CHARACTER(LEN=*), INTENT(IN) :: datafile, initfile
     CALL read_my_data(TRIM(datafile))
@@ -175,7 +175,7 @@ CONTAINS
     inputs%htiny = inputs%epsilon*max_water_h
   END SUBROUTINE custom_init
 
-  ! Dirichlet overwrite.  bc_kind 0 is the reference bdy (SOURCES/main.f90:220-232); kinds 1 and
+  ! Dirichlet overwrite.  bc_kind 0 is the reference bdy (SOURCES/main.f90:176-188); kinds 1 and
   ! 2 give values for the problems without a sol_anal case: walls (q.n = 0 on the listed nodes)
   ! and walls plus a sinusoidal free-surface inflow on the h list (sin rounded to nearest).
   SUBROUTINE bdy_ref(uu, t)
@@ -204,7 +204,7 @@ CONTAINS
     END DO
   END SUBROUTINE bdy_ref
 
-  ! COMPUTE_DT of SOURCES/main.f90:210-218.
+  ! COMPUTE_DT of SOURCES/main.f90:166-174.
   SUBROUTINE compute_dt_ref(u0)
     REAL(KIND=8), DIMENSION(:,:), INTENT(IN) :: u0
     CALL compute_dij(u0)
@@ -318,7 +318,7 @@ CONTAINS
   END SUBROUTINE do_euler
 
   !---------------------------------------------------------------------------------------------
-  ! Fixed-step form of the SSP-RK3 loop of SOURCES/main.f90:105-136.
+  ! Fixed-step form of the SSP-RK3 loop of SOURCES/main.f90:61-92.
   SUBROUTINE do_run
     REAL(KIND=8), DIMENSION(:,:), ALLOCATABLE :: ui, uo
     REAL(KIND=8), DIMENSION(:), ALLOCATABLE :: dts

@@ -6,7 +6,7 @@
  *            SFEMaNS/FEMSUB_OBJET/fem_sparsekit.f90:6-53 (qs_00_M mass)
  *            SOURCES/update.f90:18-132 (construct_matrices: lumped, diag, c_ij)
  *   step     SOURCES/update.f90:134-935, SOURCES/fct.f90:5-42,
- *            SOURCES/condlim.f90:21-90,425-486,879-981, SOURCES/main.f90:105-136,210-232
+ *            SOURCES/condlim.f90:21-90,425-486,879-981, SOURCES/main.f90:61-92,166-188
  *
  * Fortran's MAX(a,b) is evaluated as (a > b ? a : b) and MIN(a,b) as (a < b ? a : b), which is
  * how flang lowers them; SUM over an array section is a sequential left-to-right loop from 0.
@@ -867,7 +867,7 @@ int orc_compute_dij(orc_ctx *c, const double *un, double *dij_out)
     return err;
 }
 
-/* COMPUTE_DT (main.f90:210-218): dt = CFL*1/MAXVAL(ABS(dij%aa(diag))/lumped) */
+/* COMPUTE_DT (main.f90:166-174): dt = CFL*1/MAXVAL(ABS(dij%aa(diag))/lumped) */
 double orc_compute_dt(orc_ctx *c, const double *un, int *err)
 {
     int e = compute_dij(c, un);
@@ -957,7 +957,7 @@ double orc_sin_rn(double x)
     return (q & 2) ? -v : v;
 }
 
-/* Dirichlet overwrite of bdy (main.f90:220-232) for the problems driven by this checker */
+/* Dirichlet overwrite of bdy (main.f90:176-188) for the problems driven by this checker */
 static void bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double period, double level,
                 int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
 {
@@ -974,7 +974,7 @@ static void bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double
     }
 }
 
-/* SSP-RK3 time loop body of main.f90:111-135, repeated nsteps times */
+/* SSP-RK3 time loop body of main.f90:71-91, repeated nsteps times */
 int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
             int bc_kind, double amp, double period, double level,
             int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)

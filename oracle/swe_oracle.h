@@ -56,7 +56,7 @@ int orc_compute_dij(orc_ctx *c, const double *un, double *dij_out);
 double orc_compute_dt(orc_ctx *c, const double *un, int *err);
 int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *dij_out);
 
-/* Fixed-step SSP-RK3 loop of SOURCES/main.f90:105-136.  bc_kind 0: no Dirichlet lists allowed;
+/* Fixed-step SSP-RK3 loop of SOURCES/main.f90:61-92.  bc_kind 0: no Dirichlet lists allowed;
  * 1: walls (qx = 0 on nodes_ux, qy = 0 on nodes_uy); 2: walls plus the inflow
  * h = max(level - z + amp*orc_sin_rn(2*pi*t/period), 0) on nodes_h.  dts receives dt per step. */
 int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,

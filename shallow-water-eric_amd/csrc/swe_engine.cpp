@@ -660,7 +660,7 @@ int pool_download_state(Pool* P, int k, const Node* src, double* un) {
 
 namespace {
 
-// One SSP-RK3 step of every live instance (SOURCES/main.f90:111-135): A = U^n -> B -> C -> A.
+// One SSP-RK3 step of every live instance (SOURCES/main.f90:71-91): A = U^n -> B -> C -> A.
 int record_step(Pool* P) {
     const bool lam = P->mode.viscosity_type != SWE_GALERKIN;
     Node* A = P->S[0];

@@ -1,18 +1,18 @@
 // swe_kernels.hip -- MI355X (gfx950) kernels of the explicit shallow-water step and their launch
 // wrappers.
 //
-// One SSP-RK3 step (SOURCES/main.f90:111-135) runs as a short chain of row kernels over every
+// One SSP-RK3 step (SOURCES/main.f90:71-91) runs as a short chain of row kernels over every
 // instance of a pool at once.  A kernel gives each CSR row to one lane, walks the row's entries
 // in their stored order and accumulates every sum sequentially, so each node's result is bitwise
 // the reference's:
 //
-//   k_compute_dt    COMPUTE_DT + compute_dij  (main.f90:210-218, update.f90:291-326); keeps d_ij
+//   k_compute_dt    COMPUTE_DT + compute_dij  (main.f90:166-174, update.f90:291-326); keeps d_ij
 //                   for the first stage and reduces max |d_ii|/m_i per instance
 //   k_finalize_dt   dt = CFL/max per instance, the stage times of bdy, time = to + dt
 //   k_alpha         alpha_limit, first loop   (update.f90:758-781)
 //   k_stage_lo      first-order stage: compute_dij/compute_muij/MAX/alpha scaling, smb_2_roundoff
 //                   (or smb_2, smb_1), sources, divide_by_lumped, update, RK combination
-//                   (update.f90:180-223 and main.f90:121-133)
+//                   (update.f90:180-223 and main.f90:83-89)
 //   k_fct_lo        FCT stage, pass 1: low-order solution as above plus entropy_residual
 //                   (update.f90:825-878) for the high-order viscosity
 //   k_fct_rminus    FCT stage, pass 2: compute_fct_matrix + first loop of FCT_positivity
@@ -20,7 +20,7 @@
 //   k_fct_apply     FCT stage, pass 3: l_ij (fct.f90:31-41), limited viscosities
 //                   (update.f90:278-279), apply_viscosity (684-718), update, RK combination
 //   k_ev_update     entropy-viscosity-only stage, pass 2 (update.f90:241-262)
-//   k_bdy           bdy (main.f90:220-232)
+//   k_bdy           bdy (main.f90:176-188)
 //
 // Every row kernel starts the same way: the lane's row, the wave's instance (one per 64-row
 // slice, read once and kept in scalar registers), and an exit for free slices, instances that do
@@ -112,7 +112,7 @@ __device__ __forceinline__ double* wave_lds(double* smem, int wmax) {
     return smem + (size_t)(threadIdx.x >> 6) * (size_t)(2 + wmax) * 64;
 }
 
-// RK combination of SOURCES/main.f90:127,133 (stage 0 keeps the Euler output)
+// RK combination of SOURCES/main.f90:83,89 (stage 0 keeps the Euler output)
 __device__ __forceinline__ double rk_combine(int stage, double uo, double ui) {
     if (stage == 1) return (3.0 * uo + ui) / 4.0;
     if (stage == 2) return (uo + 2.0 * ui) / 3.0;
@@ -297,7 +297,7 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
     }
 }
 
-// Per-instance step bookkeeping after the dt pass (SOURCES/main.f90:61-72,111-135):
+// Per-instance step bookkeeping after the dt pass (SOURCES/main.f90:61-91):
 // an instance takes the step while time < tfinal (and its step limit and error flags allow);
 // dt = CFL*1/MAXVAL(ABS(d_ii)/m_i); the boundary values of the three stages are evaluated at
 // t+dt, (t+dt)+dt/2 and (t+dt/2)+dt; time = to + dt.  advance = 0 only computes dt.

@@ -1,7 +1,7 @@
 """Batches: many meshes resident in one device arena and advanced together (swe_batch_* of
 include/swe_amd.h).
 
-Each instance runs the reference's time loop (SOURCES/main.f90:61-136) with its own parameters,
+Each instance runs the reference's time loop (SOURCES/main.f90:61-92) with its own parameters,
 boundary program, dt, time and stopping rule; the results of an instance are bitwise those of
 the same mesh run alone with :class:`swe_amd.solver.Solver`.  Instances share the scheme
 (viscosity type, viscous type, alpha limiter, roundoff fix).

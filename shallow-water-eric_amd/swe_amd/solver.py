@@ -202,8 +202,8 @@ class Solver:
     """One mesh, one parameter set, one device-resident state.
 
     Methods follow the reference: :meth:`compute_dij` (update.f90:291), :meth:`compute_dt`
-    (COMPUTE_DT, main.f90:210), :meth:`euler` (update.f90:134), :meth:`fct_positivity`
-    (fct.f90:5) and the SSP-RK3 time loop :meth:`run` (main.f90:105-136).
+    (COMPUTE_DT, main.f90:166), :meth:`euler` (update.f90:134), :meth:`fct_positivity`
+    (fct.f90:5) and the SSP-RK3 time loop :meth:`run` (main.f90:61-92).
     """
 
     def __init__(self, ia, ja, cij, lumped, bath, *, gravity, htiny, cfl, max_water_h,
