@@ -136,11 +136,13 @@ def _batch_caps(ps, spare_rows=0):
     return rows, entries, bnd
 
 
-def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=5000, nmax=500000):
+def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=5000, nmax=500000, dropin_steps=3):
     """Batched form: `count` meshes of the 16.8M-node workload's family, sizes log-uniform in
-    [nmin, nmax], dam-break or still-water-with-inflow initial data, all resident in one batch
-    and advanced together (one captured graph per step, autotuned schedules); against the plain
-    per-instance loop: the same meshes one at a time, each alone on the device."""
+    [nmin, nmax], dam-break or still-water initial data, all resident in one batch and advanced
+    together (one captured graph per step, autotuned schedules); against two plain per-instance
+    loops over the same meshes one at a time: the device-resident loop (state kept on the device,
+    one captured graph per step) and the drop-in loop (the reference's host time loop calling
+    compute_dij and euler per stage, `dropin_steps` steps per mesh)."""
     from swe_amd import problems
     from swe_amd.batch import Batch
     t0 = time.perf_counter()
@@ -206,16 +208,31 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
         work_plain += p.np * 3 * (one.status(k)[2] - n0)
         one.remove(k)
     one.close()
+    # plain drop-in loop: the reference's time loop on the host calling the per-stage entry points
+    # (compute_dij, euler) of each mesh, one mesh at a time, state over PCIe every stage
+    from swe_amd.dropin import DropinLoop
+    el_drop, work_drop = 0.0, 0
+    for p in ps:
+        with p.solver() as s:
+            loop = DropinLoop(s, p)
+            loop.run(p.u0, 1)
+            t0 = time.perf_counter()
+            loop.run(p.u0, dropin_steps)
+            el_drop += time.perf_counter() - t0
+            work_drop += p.np * 3 * dropin_steps
     kavg = sum(p.nnz for p in ps) / nodes
     bpn = kernel_bytes_per_node("k_fct_lo", kavg, True)
     launch_s = kms / 1e3 / max(nl, 1)
     achieved = bpn * nodes / launch_s / 1e9
     rate_b = work / el_graph
     rate_p = work_plain / el_plain
+    rate_d = work_drop / el_drop
     return {"instances": count, "nodes": nodes, "sizes": [nmin, nmax], "steps": steps, "warmup": warmup,
             "node_stage_per_s": rate_b, "ms_per_step": el_graph / steps * 1e3,
             "node_stage_per_s_eager_timed": work / el, "plain_node_stage_per_s": rate_p,
-            "plain_ms_total": el_plain * 1e3, "speedup_vs_plain": rate_b / rate_p, "failed_instances": failed,
+            "plain_ms_total": el_plain * 1e3, "speedup_vs_plain": rate_b / rate_p,
+            "dropin_node_stage_per_s": rate_d, "dropin_steps": dropin_steps, "speedup_vs_dropin": rate_b / rate_d,
+            "failed_instances": failed,
             "admission_s": t_admit,
             "roofline": {"bound": "hbm", "kernel": "k_fct_lo", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "launch_ms": launch_s * 1e3,

@@ -239,3 +239,17 @@ def test_tickets_report_finished_instances_once():
         assert n2 == 2 and same_bits(u2, run_alone(ps[0], 2)[0])
         st, uc, tc, nc, _ = b.result(c)
         assert nc == 30 and same_bits(uc, run_alone(ps[1], 30)[0])
+
+
+def test_host_dropin_loop_equals_device_loop():
+    """The reference's host time loop calling the per-stage entry points (the plain drop-in form,
+    state over PCIe every stage) gives bitwise the device-resident loop's states, dt and time."""
+    from swe_amd import problems
+    from swe_amd.dropin import DropinLoop
+    p = problems.dam_break(61, 13, seed=3)
+    n = 12
+    u_dev, t_dev, dts_dev = run_alone(p, n)
+    with p.solver() as s:
+        u, t, dts = DropinLoop(s, p).run(p.u0, n)
+    assert t == t_dev and np.array_equal(dts, dts_dev)
+    assert same_bits(u, u_dev)
