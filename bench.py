@@ -263,6 +263,7 @@ def bench_serve(count, rate, workers, chunk=16, budget_gb=200.0, seed=20261202, 
     served = Server(BatchEngine(b), chunk=chunk).run(reqs())
     resident = b.caps.max_nodes
     b.close()
+    print("# served: makespan %.1f s" % summarize(served)["makespan_s"], file=sys.stderr, flush=True)
     big = max(ps, key=lambda p: p.nnz)
     r1, e1, b1 = _batch_caps([big])
     one = Batch(r1, e1, max(b1, max(sum(x.size for x in p.dirichlet()) for p in ps) + 64), 1)
