@@ -280,12 +280,40 @@ def bench_serve(count, rate, workers, chunk=16, budget_gb=200.0, seed=20261202, 
             "results_identical_to_plain": bool(same), "errors": s["errors"]}
 
 
+def bench_setup(n_side=4097, reps=2):
+    """P1 set-up (construct_matrices) of the 16.8M-node mesh: host assembly (all host threads)
+    against the device assembly (including its host<->device copies); both must give the same
+    bits."""
+    from swe_amd import mesh as M
+    from swe_amd.solver import assemble
+    t0 = time.perf_counter()
+    m = M.renumber_first_touch(M.jittered_delaunay_mesh(n_side, 1000.0, jitter=0.25, seed=20261202))
+    print("# setup: mesh of %d nodes, %d elements generated in %.1f s" % (m.np, m.jj.shape[0], time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    assemble(np.array([[0, 1, 2]]), np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]), device=True)   # warm-up
+    t_host, t_dev = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        host = assemble(m.jj, m.rr)
+        t_host.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        dev = assemble(m.jj, m.rr, device=True)
+        t_dev.append(time.perf_counter() - t0)
+        same = all(np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8)) for a, b in zip(host, dev))
+        del host, dev
+        print("# setup: host %.2f s, device %.2f s, identical %s" % (t_host[-1], t_dev[-1], same), file=sys.stderr,
+              flush=True)
+    return {"nodes": int(m.np), "elements": int(m.jj.shape[0]), "host_s": min(t_host), "device_s": min(t_dev),
+            "host_threads": os.cpu_count(), "identical": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bumps", choices=["bumps", "dam_break", "paraboloid", "batch", "serve"])
+    ap.add_argument("--workload", default="bumps",
+                    choices=["bumps", "dam_break", "paraboloid", "batch", "serve", "setup"])
     ap.add_argument("--instances", type=int, default=256, help="batch: meshes in the batch")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
     ap.add_argument("--rates", default="20", help="serve: arrival rates (per s), comma separated")
@@ -319,6 +347,12 @@ def main():
                     "ms_per_step": r["ms_per_step"], "roofline": r.pop("roofline"),
                     "config": {"workload": "batch", **r}})
         print(json.dumps(res))
+        return
+    if args.workload == "setup":
+        r = bench_setup()
+        print(json.dumps({"metric": "P1 set-up time (construct_matrices), device", "value": r["device_s"], "unit": "s",
+                          "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": "setup", **r}}), flush=True)
         return
     if args.workload == "serve":
         for rate in [float(x) for x in args.rates.split(",")]:

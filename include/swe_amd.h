@@ -89,6 +89,16 @@ long swe_assemble_nnz(int np, int me, const int *jj);
 int swe_assemble(int np, int me, const int *jj, const double *rr, int *ia, int *ja, int *diag,
                  double *mass, double *lumped, double *cx, double *cy);
 
+/* The same set-up on the GPU, bitwise equal to swe_assemble: swe_assemble_device builds the
+ * pattern and the matrices in device memory and returns their number of entries (SIZE(ja));
+ * swe_assembly_get copies them out in swe_assemble's layout (any pointer may be NULL);
+ * swe_assembly_destroy frees them.  A node may belong to at most 32 elements. */
+typedef struct swe_assembly swe_assembly;
+int swe_assemble_device(int np, int me, const int *jj, const double *rr, swe_assembly **out, long long *nnz);
+int swe_assembly_get(swe_assembly *a, int *ia, int *ja, int *diag, double *mass, double *lumped, double *cx,
+                     double *cy);
+int swe_assembly_destroy(swe_assembly *a);
+
 /* Create a solver over one mesh: copies the mesh arrays to the device.  Replaces the module
  * state built by construct_matrices (SOURCES/update.f90:18-132) and construct_mesh.
  * The environment variable SWE_SORT_WINDOW (default 64) sets the window of rows inside which

@@ -58,6 +58,9 @@ SIGNATURES = {
     "swe_last_error": (C.c_char_p, []),
     "swe_assemble_nnz": (C.c_long, [_I, _I, _P]),
     "swe_assemble": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "swe_assemble_device": (_I, [_I, _I, _P, _P, C.POINTER(_P), C.POINTER(C.c_longlong)]),
+    "swe_assembly_get": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "swe_assembly_destroy": (_I, [_P]),
     "swe_create": (_I, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(_P)]),
     "swe_destroy": (_I, [_P]),
     "swe_set_params": (_I, [_P, C.POINTER(Params)]),
@@ -136,12 +139,31 @@ def _check(rc):
         raise SweError(rc, msg.decode() if msg else "")
 
 
-def assemble(jj0: np.ndarray, rr: np.ndarray):
-    """P1 set-up (construct_matrices): returns 0-based (ia, ja, diag), mass, lumped, cij (nnz,2)."""
+def assemble(jj0: np.ndarray, rr: np.ndarray, device: bool = False):
+    """P1 set-up (construct_matrices): returns 0-based (ia, ja, diag), mass, lumped, cij (nnz,2).
+    device=True assembles on the GPU (swe_assemble_device; the same bits)."""
     lib = load_library()
     jj1 = np.ascontiguousarray(jj0, dtype=np.int32) + 1
     rr = np.ascontiguousarray(rr, dtype=np.float64)
     npn, me = rr.shape[0], jj1.shape[0]
+    if device:
+        h = C.c_void_p()
+        n = C.c_longlong()
+        _check(lib.swe_assemble_device(npn, me, _ptr(jj1), _ptr(rr), C.byref(h), C.byref(n)))
+        try:
+            nnz = n.value
+            ia = np.zeros(npn + 1, np.int32)
+            ja = np.zeros(nnz, np.int32)
+            diag = np.zeros(npn, np.int32)
+            mass = np.zeros(nnz)
+            lumped = np.zeros(npn)
+            cx = np.zeros(nnz)
+            cy = np.zeros(nnz)
+            _check(lib.swe_assembly_get(h, _ptr(ia), _ptr(ja), _ptr(diag), _ptr(mass), _ptr(lumped), _ptr(cx),
+                                        _ptr(cy)))
+        finally:
+            lib.swe_assembly_destroy(h)
+        return ia - 1, ja - 1, diag - 1, mass, lumped, np.stack([cx, cy], 1)
     nnz = lib.swe_assemble_nnz(npn, me, _ptr(jj1))
     if nnz < 0:
         raise SweError(SWE_EINVAL, "invalid element list")
