@@ -46,7 +46,8 @@ typedef enum swe_bc_kind {        /* Dirichlet values written by bdy after each 
     SWE_BC_NONE = 0,              /* no Dirichlet nodes */
     SWE_BC_WALLS = 1,             /* qx = 0 on the ux list, qy = 0 on the uy list */
     SWE_BC_INFLOW = 2,            /* walls, plus h = max(level - z + amp*sin(2*pi*t/period), 0) on the h list */
-    SWE_BC_VALUES = 3             /* fixed values given per listed node by swe_set_bc_values */
+    SWE_BC_VALUES = 3,            /* fixed values given per listed node by swe_set_bc_values */
+    SWE_BC_STAGES = 4             /* values per stage given with each step by swe_step_finish */
 } swe_bc_kind;
 
 typedef struct swe_params {
@@ -145,6 +146,16 @@ int swe_get_time(swe_handle *h, double *t);
  * *time receives the end time; dts (may be NULL) receives dt of each step.  Returns
  * SWE_ENEG_H / SWE_ELAMBDA if any stage tripped those checks. */
 int swe_run(swe_handle *h, int nsteps, double *time, double *dts);
+
+/* One step in two calls, for Dirichlet values the host computes per stage (bdy with sol_anal at
+ * t+dt, t+dt/2 and t+dt, SOURCES/main.f90:78,84,90), which need dt first.  swe_step_begin runs
+ * COMPUTE_DT on the device-resident state and returns dt (the handle's time becomes t+dt at the
+ * end of the step, as main.f90:91 sets it); swe_step_finish runs the three stages.  With the
+ * SWE_BC_STAGES program, vals holds the values of the three stages one after the other, each
+ * as swe_set_bc_values orders them (h list, ux list, uy list); with other programs vals may be
+ * NULL.  Returns like swe_run. */
+int swe_step_begin(swe_handle *h, double *dt);
+int swe_step_finish(swe_handle *h, const double *vals);
 
 /* The same steps queued without waiting for completion (dt, the time and the inflow values are
  * evaluated on the device).  swe_sync waits for the queued work and reports the checks. */

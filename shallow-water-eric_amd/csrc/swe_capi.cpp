@@ -26,6 +26,8 @@ struct swe_handle {
     int dt_cap = 0;
     long long queued = 0;           // steps queued so far (the instance's step counter)
     bool state_set = false;
+    bool step_open = false;         // swe_step_begin ran; swe_step_finish completes the step
+    int bc_kind = SWE_BC_NONE;
 };
 
 struct swe_batch {
@@ -203,7 +205,9 @@ int swe_set_params(swe_handle* h, const swe_params* prm) {
 
 int swe_set_bc(swe_handle* h, int kind, double amp, double period, double level) {
     if (!h) return fail(SWE_EINVAL, "null handle");
-    return pool_set_bc(h->P, kId, kind, amp, period, level);
+    const int rc = pool_set_bc(h->P, kId, kind, amp, period, level);
+    if (!rc) h->bc_kind = kind;
+    return rc;
 }
 
 int swe_set_bc_values(swe_handle* h, const double* hv, const double* uxv, const double* uyv) {
@@ -357,6 +361,46 @@ int swe_run(swe_handle* h, int nsteps, double* time, double* dts) {
     const int frc = check_flags(h->P, kId);
     if (time && (rc = swe_get_time(h, time))) return rc;
     return frc;
+}
+
+int swe_step_begin(swe_handle* h, double* dt) {
+    if (!h || !dt) return fail(SWE_EINVAL, "null argument");
+    if (!h->state_set) return fail(SWE_EINVAL, "no state has been set");
+    if (h->step_open) return fail(SWE_EINVAL, "the step begun before has not been finished");
+    Pool* P = h->P;
+    int rc = record_dt(P);
+    if (rc) return rc;
+    SWE_HIPCHK(hipGetLastError());
+    int go = 0;
+    if ((rc = read_field(P, kId, offsetof(InstDev, go), &go, sizeof go))) return rc;
+    if (!go) {
+        // the instance takes no step: a check tripped earlier
+        rc = check_flags(P, kId);
+        return rc ? rc : fail(SWE_EINVAL, "the device-resident state takes no step");
+    }
+    if ((rc = read_field(P, kId, offsetof(InstDev, dt), dt, sizeof *dt))) return rc;
+    h->step_open = true;
+    h->queued += 1;
+    return SWE_OK;
+}
+
+int swe_step_finish(swe_handle* h, const double* vals) {
+    if (!h) return fail(SWE_EINVAL, "null handle");
+    if (!h->step_open) return fail(SWE_EINVAL, "swe_step_begin has not been called");
+    h->step_open = false;
+    Pool* P = h->P;
+    const InstHost& I = P->inst[(size_t)kId];
+    if (h->bc_kind == SWE_BC_STAGES) {
+        const int64_t n = (int64_t)I.nh + I.nux + I.nuy;
+        if (n > 0 && !vals) return fail(SWE_EINVAL, "missing per-stage boundary values");
+        for (int st = 0; st < 3 && n > 0; ++st)
+            SWE_HIPCHK(hipMemcpyAsync(P->bcv + st * P->bcv_stride + I.bc0, vals + st * n, sizeof(double) * (size_t)n,
+                                      hipMemcpyHostToDevice, P->stream));
+    }
+    int rc = record_stages(P);
+    if (rc) return rc;
+    SWE_HIPCHK(hipGetLastError());
+    return check_flags(P, kId);
 }
 
 int swe_run_async(swe_handle* h, int nsteps) {

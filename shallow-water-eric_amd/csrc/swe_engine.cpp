@@ -276,7 +276,7 @@ constexpr size_t kRowBytes = sizeof(int) * 3 + sizeof(double) * 2 + sizeof(Node)
 constexpr size_t kSlotBytes = sizeof(int) + sizeof(double2) * 2 + sizeof(double) * 5;  // col c nvec nrm d0 dL muL ework
 constexpr size_t kSliceBytes = sizeof(SliceInfo);
 constexpr size_t kInstBytes = sizeof(InstDev) + sizeof(unsigned long long) + sizeof(int);
-constexpr size_t kBcBytes = sizeof(BcEntry) + sizeof(double);
+constexpr size_t kBcBytes = sizeof(BcEntry) + 3 * sizeof(double);   // entry, value of each stage
 
 void free_pool(Pool* P) {
     if (!P) return;
@@ -388,7 +388,8 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     ALLOC(P->vmax, (size_t)caps.instances);
     ALLOC(P->flags, (size_t)caps.instances);
     ALLOC(P->bc, (size_t)std::max<int64_t>(caps.bc, 1));
-    ALLOC(P->bcv, (size_t)std::max<int64_t>(caps.bc, 1));
+    P->bcv_stride = std::max<int64_t>(caps.bc, 1);
+    ALLOC(P->bcv, 3 * (size_t)P->bcv_stride);
     ALLOC(P->act, (size_t)slices);
 #undef ALLOC
     P->listed.assign((size_t)caps.instances, 0);
@@ -633,7 +634,7 @@ int pool_set_params(Pool* P, int k, const swe_params& prm) {
 }
 
 int pool_set_bc(Pool* P, int k, int kind, double amp, double period, double level) {
-    if (kind < SWE_BC_NONE || kind > SWE_BC_VALUES) return fail(SWE_EINVAL, "unknown bc kind %d", kind);
+    if (kind < SWE_BC_NONE || kind > SWE_BC_STAGES) return fail(SWE_EINVAL, "unknown bc kind %d", kind);
     if (kind == SWE_BC_INFLOW && !(period != 0.0)) return fail(SWE_EINVAL, "inflow period must be non-zero");
     const double v[3] = {amp, period, level};
     int rc = pool_set_field(P, k, offsetof(InstDev, bc_amp), v, sizeof v);
@@ -658,17 +659,20 @@ int pool_download_state(Pool* P, int k, const Node* src, double* un) {
     return SWE_OK;
 }
 
-namespace {
-
 // One SSP-RK3 step of every live instance (SOURCES/main.f90:71-91): A = U^n -> B -> C -> A.
-int record_step(Pool* P) {
+int record_dt(Pool* P) {
+    const bool lam = P->mode.viscosity_type != SWE_GALERKIN;
+    launch_compute_dt(*P, P->S[0], lam ? P->d0 : nullptr, 0);
+    launch_finalize_dt(*P, 1);
+    return SWE_OK;
+}
+
+int record_stages(Pool* P) {
     const bool lam = P->mode.viscosity_type != SWE_GALERKIN;
     Node* A = P->S[0];
     Node* B = P->S[1];
     Node* C = P->S[2];
     int rc;
-    launch_compute_dt(*P, A, lam ? P->d0 : nullptr, 0);
-    launch_finalize_dt(*P, 1);
     if ((rc = launch_euler_stage(*P, A, A, B, lam ? P->d0 : nullptr, 0))) return rc;
     launch_bdy(*P, B, 0);
     if ((rc = launch_euler_stage(*P, B, A, C, nullptr, 1))) return rc;
@@ -676,6 +680,13 @@ int record_step(Pool* P) {
     if ((rc = launch_euler_stage(*P, C, A, A, nullptr, 2))) return rc;
     launch_bdy(*P, A, 2);
     return SWE_OK;
+}
+
+namespace {
+
+int record_step(Pool* P) {
+    const int rc = record_dt(P);
+    return rc ? rc : record_stages(P);
 }
 
 int64_t tuning_hash(const Tuning& t) {

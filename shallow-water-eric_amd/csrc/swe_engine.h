@@ -145,7 +145,8 @@ struct Pool {
     unsigned long long* vmax = nullptr;
     int* flags = nullptr;
     BcEntry* bc = nullptr;
-    double* bcv = nullptr;
+    double* bcv = nullptr;            // Dirichlet values: stage s of entry v at bcv[s * bcv_stride + v]
+    int64_t bcv_stride = 1;
     InstDev* tab_host = nullptr;      // pinned read-back of the table
     int* flags_host = nullptr;        // pinned read-back of the flags
     Snapshot snaps[4];
@@ -183,6 +184,10 @@ int pool_set_field(Pool* P, int id, size_t offset, const void* src, size_t bytes
 int pool_upload_state(Pool* P, int id, const double* un, Node* dst);
 int pool_download_state(Pool* P, int id, const Node* src, double* un);
 int pool_queue_steps(Pool* P, int nsteps);
+// The parts of one step, launched on the compute stream: the dt pass and its bookkeeping, then the
+// three stages with bdy.
+int record_dt(Pool* P);
+int record_stages(Pool* P);
 // Rebuild the launch list from the instances marked listed (ordered on the compute stream).
 int pool_update_active(Pool* P);
 int pool_read_table(Pool* P);         // sync, then tab_host / flags_host hold the table

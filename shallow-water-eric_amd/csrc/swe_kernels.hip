@@ -894,7 +894,7 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
 // bdy: Dirichlet overwrites after a stage, for every instance that takes the step.
 // =============================================================================================
 __global__ void k_bdy(Node* __restrict__ u, const BcEntry* __restrict__ E, int n, const double* __restrict__ vals,
-                      const InstDev* __restrict__ T, int stage) {
+                      int64_t stride, const InstDev* __restrict__ T, int stage) {
     const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= n) return;
     const BcEntry e = E[q];
@@ -904,13 +904,16 @@ __global__ void k_bdy(Node* __restrict__ u, const BcEntry* __restrict__ E, int n
     const int kind = D.bc_kind;
     if (kind == SWE_BC_NONE) return;
     Node& x = u[e.node];
+    // SWE_BC_VALUES: one value per entry; SWE_BC_STAGES: one per entry and stage
+    const bool given = kind == SWE_BC_VALUES || kind == SWE_BC_STAGES;
+    const double v = given ? vals[(kind == SWE_BC_STAGES ? stage * stride : 0) + e.vidx] : 0.0;
     if (e.comp == 0) {
         if (kind == SWE_BC_INFLOW) x.h = fmaxF(D.bc_level - x.z + D.bc_val[stage], 0.0);
-        else if (kind == SWE_BC_VALUES) x.h = vals[e.vidx];
+        else if (given) x.h = v;
     } else if (e.comp == 1) {
-        x.qx = (kind == SWE_BC_VALUES) ? vals[e.vidx] : 0.0;
+        x.qx = v;
     } else {
-        x.qy = (kind == SWE_BC_VALUES) ? vals[e.vidx] : 0.0;
+        x.qy = v;
     }
 }
 
@@ -1226,7 +1229,7 @@ void launch_bdy(Pool& P, Node* u, int stage) {
     const int64_t n = P.nbc();
     if (n == 0) return;
     hipLaunchKernelGGL(k_bdy, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, u, (const BcEntry*)P.bc, (int)n,
-                       (const double*)P.bcv, (const InstDev*)P.tab, stage);
+                       (const double*)P.bcv, P.bcv_stride, (const InstDev*)P.tab, stage);
 }
 
 void launch_dij_full(Pool& P, const Node* U, double* dij) {

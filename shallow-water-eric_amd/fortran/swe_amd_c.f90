@@ -10,7 +10,8 @@ MODULE swe_amd_c
   INTEGER(c_int), PARAMETER :: SWE_OK = 0, SWE_ENEG_H = 1, SWE_ELAMBDA = 2, SWE_EBADOPT = 3, &
        SWE_EHIP = 4, SWE_EINVAL = 5, SWE_ENOMEM = 6
   INTEGER(c_int), PARAMETER :: SWE_GALERKIN = 0, SWE_VISCOUS = 1, SWE_ENTROPY_VISC = 2, SWE_FCT = 3
-  INTEGER(c_int), PARAMETER :: SWE_BC_NONE = 0, SWE_BC_WALLS = 1, SWE_BC_INFLOW = 2, SWE_BC_VALUES = 3
+  INTEGER(c_int), PARAMETER :: SWE_BC_NONE = 0, SWE_BC_WALLS = 1, SWE_BC_INFLOW = 2, SWE_BC_VALUES = 3, &
+       SWE_BC_STAGES = 4
 
   TYPE, BIND(C) :: swe_params
      REAL(c_double)    :: gravity, htiny, cfl, max_water_h, mannings
@@ -93,6 +94,66 @@ MODULE swe_amd_c
        REAL(c_double), VALUE :: dt
        INTEGER(c_int) :: rc
      END FUNCTION swe_euler
+
+     ! ---- device-resident time loop (main.f90:71-91 on the GPU) ------------------------------
+     FUNCTION swe_set_state(handle, un) BIND(C, NAME='swe_set_state') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: un
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_set_state
+
+     FUNCTION swe_get_state(handle, un) BIND(C, NAME='swe_get_state') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), DIMENSION(*) :: un
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_get_state
+
+     FUNCTION swe_set_time(handle, t) BIND(C, NAME='swe_set_time') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), VALUE :: t
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_set_time
+
+     FUNCTION swe_get_time(handle, t) BIND(C, NAME='swe_get_time') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double) :: t
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_get_time
+
+     FUNCTION swe_set_bc(handle, kind, amp, period, level) BIND(C, NAME='swe_set_bc') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       INTEGER(c_int), VALUE :: kind
+       REAL(c_double), VALUE :: amp, period, level
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_set_bc
+
+     FUNCTION swe_run(handle, nsteps, time, dts) BIND(C, NAME='swe_run') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       INTEGER(c_int), VALUE :: nsteps
+       REAL(c_double) :: time
+       REAL(c_double), DIMENSION(*) :: dts
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_run
+
+     FUNCTION swe_step_begin(handle, dt) BIND(C, NAME='swe_step_begin') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double) :: dt
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_step_begin
+
+     FUNCTION swe_step_finish(handle, vals) BIND(C, NAME='swe_step_finish') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: handle
+       REAL(c_double), DIMENSION(*), INTENT(IN) :: vals
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_step_finish
 
      FUNCTION swe_fct_positivity_csr(np, nnz, ia, ja, unext_scaled, f, htiny, lij) &
           BIND(C, NAME='swe_fct_positivity_csr') RESULT(rc)

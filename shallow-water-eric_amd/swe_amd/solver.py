@@ -21,7 +21,7 @@ INSTANCE_RUNNING, INSTANCE_DONE, INSTANCE_FAILED = 1, 2, 3
 NUM_KERNELS = 7
 KERNELS = ("k_compute_dt", "k_alpha", "k_stage_lo", "k_fct_lo", "k_fct_rminus", "k_fct_apply", "k_ev_update")
 GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
-BC_NONE, BC_WALLS, BC_INFLOW, BC_VALUES = 0, 1, 2, 3
+BC_NONE, BC_WALLS, BC_INFLOW, BC_VALUES, BC_STAGES = 0, 1, 2, 3, 4
 VISCOSITY = {"galerkin": GALERKIN, "viscous": VISCOUS, "entropy_visc": ENTROPY_VISC, "fct": FCT}
 
 
@@ -85,6 +85,8 @@ SIGNATURES = {
     "swe_get_schedule": (_I, [_P, _P, _P]),
     "swe_set_schedule": (_I, [_P, _P, _P]),
     "swe_use_graph": (_I, [_P, _I]),
+    "swe_step_begin": (_I, [_P, C.POINTER(_D)]),
+    "swe_step_finish": (_I, [_P, _P]),
     "swe_batch_bytes": (C.c_ulonglong, [C.POINTER(BatchCaps)]),
     "swe_batch_create": (_I, [C.POINTER(BatchCaps), C.POINTER(_P)]),
     "swe_batch_destroy": (_I, [_P]),
@@ -333,6 +335,18 @@ class Solver:
         t = C.c_double(self.time if time is None else float(time))
         _check(load_library().swe_run(self.h, int(nsteps), C.byref(t), _ptr(dts)))
         return t.value, dts[:nsteps]
+
+    def step_begin(self):
+        """COMPUTE_DT on the device-resident state; returns dt of the step it begins."""
+        dt = C.c_double()
+        _check(load_library().swe_step_begin(self.h, C.byref(dt)))
+        return dt.value
+
+    def step_finish(self, vals=None):
+        """The three stages of the begun step; vals: (3, nh+nux+nuy) Dirichlet values per stage for
+        the BC_STAGES program (h list, ux list, uy list)."""
+        v = None if vals is None else np.ascontiguousarray(vals, np.float64)
+        _check(load_library().swe_step_finish(self.h, _ptr(v)))
 
     def run_async(self, nsteps):
         _check(load_library().swe_run_async(self.h, int(nsteps)))

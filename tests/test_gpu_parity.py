@@ -182,3 +182,59 @@ def test_negative_depth_is_reported():
         with pytest.raises(SweError) as e:
             s.run(1, time=0.0)
         assert e.value.status == SWE_ENEG_H
+
+
+def test_per_stage_boundary_values_against_oracle(oracle):
+    """Device-resident steps whose Dirichlet values the host evaluates per stage once dt is known
+    (swe_step_begin / swe_step_finish; bdy of SOURCES/main.f90:176-188 with a time-dependent
+    boundary function), against the CPU checker driven through the same time loop."""
+    from swe_amd import problems
+    from swe_amd.solver import BC_STAGES
+    p = problems.bumps(n_side=65, L=500.0, seed=5)
+    hd, uxd, uyd = (np.asarray(x, np.int64) for x in p.dirichlet())
+    z = p.bath
+    nh, nx = hd.size, uxd.size
+
+    def values(t):
+        h = np.maximum(1.0 - z[hd] + 0.05 * np.sin(2 * np.pi * t / 7.0), 0.0)
+        return np.concatenate([h, np.zeros(uxd.size), np.zeros(uyd.size)])
+
+    def bdy(u, v):
+        u[hd, 0] = v[:nh]
+        u[uxd, 1] = v[nh:nh + nx]
+        u[uyd, 2] = v[nh + nx:]
+        return u
+
+    def stage_times(to, dt):
+        return [to + dt, (to + dt) + dt / 2, (to + dt / 2) + dt]
+
+    nsteps = 30
+    orc = oracle.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0,
+                        viscosity_type=oracle.FCT, alpha_limit=True, type_test=4)
+    u, t, dts = p.u0.copy(), 0.0, []
+    for _ in range(nsteps):
+        dt, err = orc.compute_dt(u)
+        assert err == 0
+        ts = stage_times(t, dt)
+        uo = u
+        u1, _, e1 = orc.euler(uo, dt)
+        u1 = bdy(u1, values(ts[0]))
+        ui, _, e2 = orc.euler(u1, dt)
+        u2 = bdy((3 * uo + ui) / 4, values(ts[1]))
+        ui, _, e3 = orc.euler(u2, dt)
+        u = bdy((uo + 2 * ui) / 3, values(ts[2]))
+        assert e1 == e2 == e3 == 0
+        t = t + dt
+        dts.append(dt)
+    with p.solver() as s:
+        s.set_state(p.u0)
+        s.set_bc(BC_STAGES)
+        s.time = 0.0
+        td = 0.0
+        for k in range(nsteps):
+            dt = s.step_begin()
+            assert dt == dts[k], k
+            s.step_finish(np.stack([values(x) for x in stage_times(td, dt)]))
+            td = td + dt
+        assert s.time == t == td
+        assert same_bits(s.get_state(), u)
