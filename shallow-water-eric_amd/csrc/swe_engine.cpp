@@ -209,6 +209,7 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
     if (L.wmax > kMaxRow) return fail(SWE_EINVAL, "a row has %d entries; at most %d are supported", L.wmax, kMaxRow);
     L.nslot = L.sptr[(size_t)L.nsl];
     L.col.assign((size_t)L.nslot, 0);
+    L.dcol.assign((size_t)L.nslot, 0);
     L.c.assign((size_t)L.nslot, make_double2(0.0, 0.0));
     L.nv.assign((size_t)L.nslot, make_double2(0.0, 0.0));
     L.nr.assign((size_t)L.nslot, 0.0);
@@ -226,6 +227,8 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
             if (j == i) L.dpos[(size_t)r] = q;
             const int64_t slot = base + 64 * (int64_t)q;
             L.col[(size_t)slot] = L.iperm[(size_t)j];
+            const int off = L.iperm[(size_t)j] - r;
+            L.dcol[(size_t)slot] = (int16_t)((off > kColFar && off <= 32767) ? off : kColFar);
             L.c[(size_t)slot] = make_double2(d.cx[p], d.cy[p]);
             // compute_dij's normalisation (SOURCES/update.f90:306-310), same expressions and order
             const double norm = std::sqrt(0.0 + d.cx[p] * d.cx[p] + d.cy[p] * d.cy[p]);
@@ -273,14 +276,14 @@ int dalloc(T** p, size_t n) {
 // Device bytes per unit of each capacity.
 constexpr size_t kRowBytes = sizeof(int) * 3 + sizeof(double) * 2 + sizeof(Node) * 4 + sizeof(double) * 3 +
                              sizeof(double) * 4 + sizeof(double) * 3;   // len dpos perm, m z, S[3] scratch, ulo, alpha res Rm nwork, stage3
-constexpr size_t kSlotBytes = sizeof(int) + sizeof(double2) * 2 + sizeof(double) * 5;  // col c nvec nrm d0 dL muL ework
+constexpr size_t kSlotBytes = sizeof(int) + sizeof(int16_t) + sizeof(double2) * 2 + sizeof(double) * 5;  // col dcol c nvec nrm d0 dL muL ework
 constexpr size_t kSliceBytes = sizeof(SliceInfo);
 constexpr size_t kInstBytes = sizeof(InstDev) + sizeof(unsigned long long) + sizeof(int);
 constexpr size_t kBcBytes = sizeof(BcEntry) + 3 * sizeof(double);   // entry, value of each stage
 
 void free_pool(Pool* P) {
     if (!P) return;
-    void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->c,     P->nvec,
+    void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->dcol,  P->c,     P->nvec,
                     P->nrm, P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dmu,
                     P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
                     P->flags, P->bc,    P->bcv};
@@ -368,6 +371,7 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     ALLOC(P->m, R);
     ALLOC(P->z, R);
     ALLOC(P->col, S);
+    ALLOC(P->dcol, S);
     ALLOC(P->c, S);
     ALLOC(P->nvec, S);
     ALLOC(P->nrm, S);
@@ -499,6 +503,7 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     if (e == hipSuccess) e = up(P->m + r0, L.m.data(), sizeof(double) * L.m.size());
     if (e == hipSuccess) e = up(P->z + r0, L.z.data(), sizeof(double) * L.z.size());
     if (e == hipSuccess) e = up(P->col + p0, colg.data(), sizeof(int) * colg.size());
+    if (e == hipSuccess) e = up(P->dcol + p0, L.dcol.data(), sizeof(int16_t) * L.dcol.size());
     if (e == hipSuccess) e = up(P->c + p0, L.c.data(), sizeof(double2) * L.c.size());
     if (e == hipSuccess) e = up(P->nvec + p0, L.nv.data(), sizeof(double2) * L.nv.size());
     if (e == hipSuccess) e = up(P->nrm + p0, L.nr.data(), sizeof(double) * L.nr.size());

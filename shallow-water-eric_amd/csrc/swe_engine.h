@@ -53,6 +53,7 @@ struct HostLayout {
     int np = 0, nnz = 0, np_pad = 0, nsl = 0, wmax = 0;
     int64_t nslot = 0;
     std::vector<int> perm, iperm, len, dpos, col;     // col: local internal numbering
+    std::vector<int16_t> dcol;                        // col - row, or kColFar
     std::vector<int64_t> sptr, eoff, csr2slot;        // local slots / list offsets
     std::vector<double2> c, nv;
     std::vector<double> nr, m, z;
@@ -119,6 +120,7 @@ struct Pool {
     double* m = nullptr;
     double* z = nullptr;
     int* col = nullptr;
+    int16_t* dcol = nullptr;          // column offsets (read instead of col where they fit)
     double2* c = nullptr;
     double2* nvec = nullptr;
     double* nrm = nullptr;

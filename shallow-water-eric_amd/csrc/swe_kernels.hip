@@ -69,6 +69,14 @@ constexpr double kPi = 3.141592653589793;   // ACOS(-1.d0)
 #else
 #define SWE_LB_LO(BS) __launch_bounds__(BS)
 #endif
+#ifndef SWE_WAVES_LO1
+#define SWE_WAVES_LO1 5
+#endif
+#if SWE_WAVES_LO1 > 0
+#define SWE_LB_LO1(BS) __launch_bounds__(BS, SWE_WAVES_LO1)
+#else
+#define SWE_LB_LO1(BS) __launch_bounds__(BS)
+#endif
 #if SWE_WAVES_DT > 0
 #define SWE_LB_DT(BS) __launch_bounds__(BS, SWE_WAVES_DT)
 #else
@@ -100,6 +108,13 @@ __device__ __forceinline__ bool takes_steps(const InstDev& D, int flags) {
 
 __device__ __forceinline__ int64_t row_base(const Mesh& M, int i) {
     return M.sl[i >> 6].sbase + (i & 63);
+}
+
+// Column of slot p of row i: a 16-bit offset from the row (half the bytes of the 32-bit column,
+// which is read only where the offset does not fit).
+__device__ __forceinline__ int col_at(const Mesh& M, int64_t p, int i) {
+    const int d = M.dcol[p];
+    return (d != kColFar) ? i + d : M.col[p];
 }
 
 __device__ __forceinline__ void raise_flag(int* flags, int inst, int f) {
@@ -181,7 +196,9 @@ __device__ __forceinline__ double alpha_of(const Mesh& M, const Node* __restrict
         int js[kUnroll];
         double hs[kUnroll];
 #pragma unroll
-        for (int q = 0; q < kUnroll; ++q) js[q] = (q < len) ? M.col[pb + 64 * q] : i;
+        for (int q = 0; q < kUnroll; ++q) js[q] = (q < len) ? (int)M.dcol[pb + 64 * q] : 0;
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) js[q] = (js[q] != kColFar) ? i + js[q] : M.col[pb + 64 * q];
 #pragma unroll
         for (int q = 0; q < kUnroll; ++q) hs[q] = U[js[q]].h;
         double num = 0.0, denom = 0.0;
@@ -193,7 +210,7 @@ __device__ __forceinline__ double alpha_of(const Mesh& M, const Node* __restrict
             }
         }
         for (int q = kUnroll; q < len; ++q) {
-            const double hj = U[M.col[pb + 64 * (int64_t)q]].h;
+            const double hj = U[col_at(M, pb + 64 * (int64_t)q, i)].h;
             num = num + hj - hi;
             denom = denom + fabs(hj - hi);
         }
@@ -250,7 +267,7 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
             const int r = off & 63;
             const int64_t slot = S.sbase + off;
             const Node Ir = U[slice * 64 + r];
-            const Node J = U[M.col[slot]];
+            const Node J = U[col_at(M, slot, slice * 64 + r)];
             const double d = dij_of(M.n[slot], M.nrm[slot], Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
                                     sqrt(K.g * fabs(J.h)), K, err);
             lw[128 + off] = d;
@@ -364,7 +381,7 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
     const int64_t pb = row_base(M, i);
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        const int j = M.col[p];
+        const int j = col_at(M, p, i);
         if (j == i) { pd = p; dij[p] = 0.0; continue; }
         const Node J = U[j];
         const double d = dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
@@ -420,7 +437,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
     const int64_t pb = row_base(M, i);
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        const int j = M.col[p];
+        const int j = col_at(M, p, i);
         const Node J = U[j];
         const double2 c = M.c[p];
         const double ovh_j = ovh_of(J.h, K.htiny);
@@ -525,11 +542,11 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // low-order solution (smb_2_roundoff, update) and the entropy residual res_i.
 // =============================================================================================
 template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC, bool D0>
-__global__ void SWE_LB_LO(BS)
-k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
-         double2* __restrict__ ulo_q,
-         double2* __restrict__ dmu, double* __restrict__ res, const double* __restrict__ d0,
-         const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
+__device__ __forceinline__ void
+fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
+            double2* __restrict__ ulo_q,
+            double2* __restrict__ dmu, double* __restrict__ res, const double* __restrict__ d0,
+            const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
@@ -554,7 +571,7 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doub
     SWE_UNROLL(SWE_UNROLL_LO)
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        const int j = M.col[p];
+        const int j = col_at(M, p, i);
         const Node J = U[j];
         const double2 c = M.c[p];
         const double ovh_j = ovh_of(J.h, K.htiny);
@@ -675,6 +692,27 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doub
     raise_flag(flags_out, inst, err);
 }
 
+// Stages 2 and 3 (lambda inline) and stage 1 (d_ij of the dt pass; without the lambda code the
+// kernel fits SWE_WAVES_LO1 waves per SIMD).
+template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
+__global__ void SWE_LB_LO(BS)
+k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
+         double2* __restrict__ ulo_q, double2* __restrict__ dmu, double* __restrict__ res,
+         const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
+    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, false>(M, T, U, ulo_h, ulo_q, dmu, res, nullptr, alpha, roundoff,
+                                                     flags_out, remap);
+}
+
+template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
+__global__ void SWE_LB_LO1(BS)
+k_fct_lo1(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
+          double2* __restrict__ ulo_q, double2* __restrict__ dmu, double* __restrict__ res,
+          const double* __restrict__ d0, const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out,
+          int remap) {
+    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, true>(M, T, U, ulo_h, ulo_q, dmu, res, d0, alpha, roundoff, flags_out,
+                                                    remap);
+}
+
 // Antidiffusive flux f_ij of compute_fct_matrix (update.f90:732-745) for an off-diagonal entry,
 // with the high-order viscosities MIN(dL, resij), MIN(muL, resij) of update.f90:233-239.
 __device__ __forceinline__ double fct_flux(const Node& I, const Node& J, double dLp, double muLp,
@@ -715,7 +753,7 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
     SWE_UNROLL(SWE_UNROLL_RM)
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        const int j = M.col[p];
+        const int j = col_at(M, p, i);
         if (j == i) continue;    // f_ii is a zero, never < 0
         const Node J = U[j];
         double dH, muH, rij_, rji_;
@@ -754,7 +792,7 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
     SWE_UNROLL(SWE_UNROLL_AP)
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        const int j = M.col[p];
+        const int j = col_at(M, p, i);
         if (j == i) continue;    // contributes d*(0) + mu*(0)
         const Node J = U[j];
         const double2 lo = dmu[p];
@@ -816,7 +854,7 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
     const int64_t pb = row_base(M, i);
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        const int j = M.col[p];
+        const int j = col_at(M, p, i);
         const Node J = U[j];
         const double2 c = M.c[p];
         const double ovh_j = ovh_of(J.h, K.htiny);
@@ -1009,7 +1047,7 @@ __global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* _
     const double Ri = Rm[i];
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
-        lij[p] = (f[p] >= 0.0) ? Rm[M.col[p]] : Ri;
+        lij[p] = (f[p] >= 0.0) ? Rm[col_at(M, p, i)] : Ri;
     }
 }
 
@@ -1096,10 +1134,17 @@ void launch_fct_lo(Pool& P, const Node* U, const double* d0) {
     const int ro = P.mode.roundoff;
     auto go = [&](auto src, auto has_d0) {
         constexpr bool S = decltype(src)::value, D = decltype(has_d0)::value;
-        SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, S, D>),
-                                                  dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M,
-                                                  (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
-                                                  (const double*)P.alpha, ro, P.flags, L.remap))
+        if constexpr (D) {
+            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo1<BS, ALPHA, WRITE_LO, S>),
+                                                      dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream,
+                                                      M, (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
+                                                      (const double*)P.alpha, ro, P.flags, L.remap))
+        } else {
+            SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo<BS, ALPHA, WRITE_LO, S>),
+                                                      dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream,
+                                                      M, (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res,
+                                                      (const double*)P.alpha, ro, P.flags, L.remap))
+        }
     };
     using T_ = std::true_type;
     using F_ = std::false_type;
@@ -1123,7 +1168,7 @@ bool block_supported(int kid, int block) {
 }
 
 Mesh mesh_view(const Pool& P) {
-    return Mesh{P.sl, P.len, P.col, P.c, P.nvec, P.nrm, P.m, P.elist, P.dpos, P.act, P.nact, P.wmax};
+    return Mesh{P.sl, P.len, P.col, P.dcol, P.c, P.nvec, P.nrm, P.m, P.elist, P.dpos, P.act, P.nact, P.wmax};
 }
 
 void launch_compute_dt(Pool& P, const Node* U, double* d0, int all) {

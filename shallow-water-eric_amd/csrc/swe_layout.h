@@ -78,11 +78,15 @@ struct BcEntry {
     int node, inst, comp, vidx;
 };
 
+// Column offsets that do not fit 16 bits are marked with kColFar and read from the 32-bit columns.
+constexpr int kColFar = -32768;
+
 // Read-only view of the arena's mesh arrays handed to every row kernel.
 struct Mesh {
     const SliceInfo* __restrict__ sl;
     const int* __restrict__ len;        // entries of each row (0 for padding rows)
     const int* __restrict__ col;        // global column of each slot
+    const int16_t* __restrict__ dcol;   // column - row of each slot, kColFar where that does not fit
     const double2* __restrict__ c;      // (cx, cy) of each slot
     const double2* __restrict__ n;      // c_ij/|c_ij| of each slot
     const double* __restrict__ nrm;     // |c_ij| of each slot

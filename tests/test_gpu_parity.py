@@ -238,3 +238,35 @@ def test_per_stage_boundary_values_against_oracle(oracle):
             td = td + dt
         assert s.time == t == td
         assert same_bits(s.get_state(), u)
+
+
+def test_scrambled_numbering_against_oracle(oracle):
+    """A mesh numbered at random (neighbours up to the whole mesh apart, so many column offsets do
+    not fit the 16-bit form and are read from the 32-bit columns), FCT + alpha with inflow, 10 steps,
+    bit for bit against the CPU checker on the same numbering."""
+    from swe_amd import problems
+    from swe_amd.solver import Solver, assemble
+    p = problems.bumps(n_side=201, L=1000.0, seed=8)
+    perm = np.random.default_rng(1).permutation(p.np)          # new number of node k
+    jj = perm[p.mesh.jj]
+    rr = np.empty_like(p.mesh.rr)
+    rr[perm] = p.mesh.rr
+    ia, ja, diag, mass, lumped, cij = assemble(jj, rr)
+    assert np.abs(ja - np.repeat(np.arange(p.np), np.diff(ia))).max() > 32767
+    bath = np.empty_like(p.bath)
+    bath[perm] = p.bath
+    u0 = np.empty_like(p.u0)
+    u0[perm] = p.u0
+    hd, uxd, uyd = (np.sort(perm[np.asarray(x, np.int64)]) for x in p.dirichlet())
+    kind, amp, period, level = p.bc
+    orc = oracle.Oracle(ia, ja, cij, lumped, bath, gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0,
+                        viscosity_type=oracle.FCT, alpha_limit=True, type_test=4)
+    ref, rdts, rt, err = orc.run(u0, 10, bc_kind=kind, amp=amp, period=period, level=level, nodes_h=hd,
+                                 nodes_ux=uxd, nodes_uy=uyd)
+    assert err == 0
+    with Solver(ia, ja, cij, lumped, bath, h_dir=hd, ux_dir=uxd, uy_dir=uyd, **p.params) as s:
+        s.set_bc(kind, amp, period, level)
+        s.set_state(u0)
+        t, d = s.run(10, time=0.0)
+        assert same_bits(d, rdts) and t == rt
+        assert same_bits(s.get_state(), ref)
