@@ -211,8 +211,6 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
     L.col.assign((size_t)L.nslot, 0);
     L.dcol.assign((size_t)L.nslot, 0);
     L.c.assign((size_t)L.nslot, make_double2(0.0, 0.0));
-    L.nv.assign((size_t)L.nslot, make_double2(0.0, 0.0));
-    L.nr.assign((size_t)L.nslot, 0.0);
     L.csr2slot.assign((size_t)nnz, 0);
     L.m.assign((size_t)np, 0.0);
     L.z.assign((size_t)np, 0.0);
@@ -230,10 +228,6 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
             const int off = L.iperm[(size_t)j] - r;
             L.dcol[(size_t)slot] = (int16_t)((off > kColFar && off <= 32767) ? off : kColFar);
             L.c[(size_t)slot] = make_double2(d.cx[p], d.cy[p]);
-            // compute_dij's normalisation (SOURCES/update.f90:306-310), same expressions and order
-            const double norm = std::sqrt(0.0 + d.cx[p] * d.cx[p] + d.cy[p] * d.cy[p]);
-            L.nr[(size_t)slot] = norm;
-            L.nv[(size_t)slot] = make_double2(d.cx[p] / norm, d.cy[p] / norm);
             L.csr2slot[(size_t)p] = slot;
         }
         L.m[(size_t)r] = d.lumped[i];
@@ -276,15 +270,15 @@ int dalloc(T** p, size_t n) {
 // Device bytes per unit of each capacity.
 constexpr size_t kRowBytes = sizeof(int) * 3 + sizeof(double) * 2 + sizeof(Node) * 4 + sizeof(double) * 3 +
                              sizeof(double) * 4 + sizeof(double) * 3;   // len dpos perm, m z, S[3] scratch, ulo, alpha res Rm nwork, stage3
-constexpr size_t kSlotBytes = sizeof(int) + sizeof(int16_t) + sizeof(double2) * 2 + sizeof(double) * 5;  // col dcol c nvec nrm d0 dL muL ework
+constexpr size_t kSlotBytes = sizeof(int) + sizeof(int16_t) + sizeof(double2) * 2 + sizeof(double) * 2;  // col dcol c dmu d0 ework
 constexpr size_t kSliceBytes = sizeof(SliceInfo);
 constexpr size_t kInstBytes = sizeof(InstDev) + sizeof(unsigned long long) + sizeof(int);
 constexpr size_t kBcBytes = sizeof(BcEntry) + 3 * sizeof(double);   // entry, value of each stage
 
 void free_pool(Pool* P) {
     if (!P) return;
-    void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->dcol,  P->c,     P->nvec,
-                    P->nrm, P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dmu,
+    void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->dcol,  P->c,
+                    P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dmu,
                     P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
                     P->flags, P->bc,    P->bcv};
     for (void* b : bufs)
@@ -373,8 +367,6 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     ALLOC(P->col, S);
     ALLOC(P->dcol, S);
     ALLOC(P->c, S);
-    ALLOC(P->nvec, S);
-    ALLOC(P->nrm, S);
     ALLOC(P->elist, (size_t)std::max<int64_t>(caps.elist, 1));
     for (int k = 0; k < 3; ++k) ALLOC(P->S[k], R);
     ALLOC(P->scratch, R);
@@ -505,8 +497,6 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     if (e == hipSuccess) e = up(P->col + p0, colg.data(), sizeof(int) * colg.size());
     if (e == hipSuccess) e = up(P->dcol + p0, L.dcol.data(), sizeof(int16_t) * L.dcol.size());
     if (e == hipSuccess) e = up(P->c + p0, L.c.data(), sizeof(double2) * L.c.size());
-    if (e == hipSuccess) e = up(P->nvec + p0, L.nv.data(), sizeof(double2) * L.nv.size());
-    if (e == hipSuccess) e = up(P->nrm + p0, L.nr.data(), sizeof(double) * L.nr.size());
     if (e == hipSuccess) e = up(P->elist + e0, L.elist.data(), sizeof(uint16_t) * L.elist.size());
     if (e == hipSuccess) e = up(P->bc + b0, bce.data(), sizeof(BcEntry) * bce.size());
     if (e == hipSuccess) e = up(P->sl + s0, sls.data(), sizeof(SliceInfo) * sls.size());

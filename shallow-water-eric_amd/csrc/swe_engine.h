@@ -55,8 +55,8 @@ struct HostLayout {
     std::vector<int> perm, iperm, len, dpos, col;     // col: local internal numbering
     std::vector<int16_t> dcol;                        // col - row, or kColFar
     std::vector<int64_t> sptr, eoff, csr2slot;        // local slots / list offsets
-    std::vector<double2> c, nv;
-    std::vector<double> nr, m, z;
+    std::vector<double2> c;
+    std::vector<double> m, z;
     std::vector<uint16_t> elist;
     std::vector<int> lists[3];                        // Dirichlet lists, internal numbering
 };
@@ -122,8 +122,6 @@ struct Pool {
     int* col = nullptr;
     int16_t* dcol = nullptr;          // column offsets (read instead of col where they fit)
     double2* c = nullptr;
-    double2* nvec = nullptr;
-    double* nrm = nullptr;
     uint16_t* elist = nullptr;
     // the slices the step kernels cover: those of the instances that are (or may be) stepping
     int* act = nullptr;

@@ -85,8 +85,7 @@ __device__ __forceinline__ double lambda_vacc(double hl, double qxl, double qyl,
 }
 
 // First-order graph viscosity of one off-diagonal entry: |c_ij| * lambda(U_i, U_j, c_ij/|c_ij|)
-// (compute_dij, SOURCES/update.f90:305-315).  norm = SQRT(SUM(c_ij**2)) and n = c_ij/norm depend
-// only on the mesh and are evaluated once on the host with the same expressions.
+// (compute_dij, SOURCES/update.f90:305-315), given norm = SQRT(SUM(c_ij**2)) and n = c_ij/norm.
 __device__ __forceinline__ double dij_of(double2 n, double norm, const Node& I, const Node& J,
                                          double ovh_i, double ovh_j, double sql, double sqr,
                                          const Consts& K, int& err) {

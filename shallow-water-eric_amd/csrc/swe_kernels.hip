@@ -160,8 +160,8 @@ __device__ __forceinline__ void add_sources(int src, const Node& I, double mi, d
 
 // compute_dij of one off-diagonal entry from c_ij itself: the norm and the unit vector are
 // evaluated with compute_dij's expressions (SOURCES/update.f90:306-310; correctly rounded sqrt and
-// division give the bits the host computes for the stored normals), which costs a square root and
-// two divisions instead of 24 bytes of reads per entry in the passes that already load c_ij.
+// division, so the bits are the reference's), which costs a square root and two divisions per entry
+// instead of storing 24 bytes of normal and norm per entry (measured faster in every pass).
 __device__ __forceinline__ double dij_from_c(const double2 c, const Node& I, const Node& J, double ovh_i,
                                              double ovh_j, double sql, double sqr, const Consts& K, int& err) {
     const double norm = sqrt(0.0 + c.x * c.x + c.y * c.y);
@@ -268,8 +268,8 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
             const int64_t slot = S.sbase + off;
             const Node Ir = U[slice * 64 + r];
             const Node J = U[col_at(M, slot, slice * 64 + r)];
-            const double d = dij_of(M.n[slot], M.nrm[slot], Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
-                                    sqrt(K.g * fabs(J.h)), K, err);
+            const double d = dij_from_c(M.c[slot], Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
+                                        sqrt(K.g * fabs(J.h)), K, err);
             lw[128 + off] = d;
             if (d0) d0[slot] = d;
         }
@@ -384,7 +384,7 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
         const int j = col_at(M, p, i);
         if (j == i) { pd = p; dij[p] = 0.0; continue; }
         const Node J = U[j];
-        const double d = dij_of(M.n[p], M.nrm[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
+        const double d = dij_from_c(M.c[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
         dij[p] = d;
         s = s + d;
     }
@@ -1168,7 +1168,7 @@ bool block_supported(int kid, int block) {
 }
 
 Mesh mesh_view(const Pool& P) {
-    return Mesh{P.sl, P.len, P.col, P.dcol, P.c, P.nvec, P.nrm, P.m, P.elist, P.dpos, P.act, P.nact, P.wmax};
+    return Mesh{P.sl, P.len, P.col, P.dcol, P.c, P.m, P.elist, P.dpos, P.act, P.nact, P.wmax};
 }
 
 void launch_compute_dt(Pool& P, const Node* U, double* d0, int all) {

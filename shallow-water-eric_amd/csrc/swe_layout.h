@@ -88,8 +88,6 @@ struct Mesh {
     const int* __restrict__ col;        // global column of each slot
     const int16_t* __restrict__ dcol;   // column - row of each slot, kColFar where that does not fit
     const double2* __restrict__ c;      // (cx, cy) of each slot
-    const double2* __restrict__ n;      // c_ij/|c_ij| of each slot
-    const double* __restrict__ nrm;     // |c_ij| of each slot
     const double* __restrict__ m;       // lumped mass of each row
     const uint16_t* __restrict__ elist; // per slice: slot offsets (q*64 + lane) of off-diagonal entries
     const int* __restrict__ dpos;       // position q of the diagonal in each row
