@@ -103,7 +103,9 @@ int swe_assembly_destroy(swe_assembly *a);
 /* Create a solver over one mesh: copies the mesh arrays to the device.  Replaces the module
  * state built by construct_matrices (SOURCES/update.f90:18-132) and construct_mesh.
  * The environment variable SWE_SORT_WINDOW (default 64) sets the window of rows inside which
- * the internal numbering sorts rows by length (a layout choice; results do not depend on it). */
+ * the internal numbering sorts rows by length; SWE_REORDER (default 1) renumbers rows internally by
+ * reverse Cuthill-McKee when the caller's numbering scatters neighbours (mean |i-j| above
+ * 4*sqrt(np)), 0 never, 2 always.  These are layout choices; results do not depend on them. */
 int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **out);
 int swe_destroy(swe_handle *h);
 int swe_set_params(swe_handle *h, const swe_params *prm);

@@ -172,10 +172,89 @@ int validate_mesh(const swe_mesh_desc& d) {
     return SWE_OK;
 }
 
-// Internal numbering: inside consecutive windows of `sort_window` rows of the caller's
-// numbering, rows are stably sorted by length, so the 64 rows of a slice (one wave) mostly have
-// equal length.  Only the processing order changes; each row keeps its entries in the caller's
-// order, which is the order of every row sum.
+namespace {
+
+// Mean |i - j| over the off-diagonal entries of the CSR graph in the numbering `pos`
+// (pos[i] = position of row i; identity when null).
+double mean_offset(const swe_mesh_desc& d, const std::vector<int>* pos) {
+    long double sum = 0;
+    int64_t n = 0;
+    for (int i = 0; i < d.np; ++i)
+        for (int p = d.ia[i] - 1; p < d.ia[i + 1] - 1; ++p) {
+            const int j = d.ja[p] - 1;
+            if (j == i) continue;
+            const int a = pos ? (*pos)[(size_t)i] : i, b = pos ? (*pos)[(size_t)j] : j;
+            sum += (a > b) ? a - b : b - a;
+            ++n;
+        }
+    return n ? (double)(sum / n) : 0.0;
+}
+
+// Reverse Cuthill-McKee order of the CSR graph (each component from a pseudo-peripheral node, the
+// end of a breadth-first sweep from a node of least degree; neighbours by increasing degree).
+std::vector<int> rcm_order(const swe_mesh_desc& d) {
+    const int np = d.np;
+    auto deg = [&](int i) { return d.ia[i + 1] - d.ia[i]; };
+    std::vector<int> order;
+    order.reserve((size_t)np);
+    std::vector<char> seen((size_t)np, 0);
+    std::vector<int> level((size_t)np, -1), nb;
+    auto bfs = [&](int s, bool record) {
+        // breadth-first sweep from s; returns the last node reached
+        std::vector<int> q{s};
+        level[(size_t)s] = 0;
+        int last = s;
+        for (size_t h = 0; h < q.size(); ++h) {
+            const int u = q[h];
+            last = u;
+            nb.clear();
+            for (int p = d.ia[u] - 1; p < d.ia[u + 1] - 1; ++p) {
+                const int v = d.ja[p] - 1;
+                if (v != u && level[(size_t)v] < 0) {
+                    level[(size_t)v] = level[(size_t)u] + 1;
+                    nb.push_back(v);
+                }
+            }
+            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg(a) < deg(b); });
+            q.insert(q.end(), nb.begin(), nb.end());
+        }
+        if (record)
+            for (int u : q) {
+                seen[(size_t)u] = 1;
+                order.push_back(u);
+            }
+        else
+            for (int u : q) level[(size_t)u] = -1;
+        return last;
+    };
+    std::vector<int> by_deg((size_t)np);
+    for (int i = 0; i < np; ++i) by_deg[(size_t)i] = i;
+    std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg(a) < deg(b); });
+    for (int s : by_deg) {
+        if (seen[(size_t)s]) continue;
+        const int far = bfs(s, false);
+        bfs(far, true);
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+// SWE_REORDER: 0 keeps the caller's order, 2 always renumbers, otherwise (default) the rows are
+// renumbered by reverse Cuthill-McKee when the caller's numbering scatters neighbours (mean offset
+// above 4*sqrt(np)) and the new order is better.
+int reorder_mode() {
+    const char* v = std::getenv("SWE_REORDER");
+    const int m = v ? std::atoi(v) : 1;
+    return (m >= 0 && m <= 2) ? m : 1;
+}
+
+}  // namespace
+
+// Internal numbering: a base order of the rows (the caller's, or reverse Cuthill-McKee when the
+// caller's numbering scatters neighbours), then inside consecutive windows of `sort_window` rows
+// of it, rows stably sorted by length, so the 64 rows of a slice (one wave) mostly have equal
+// length.  Only the processing order and placement change; each row keeps its entries in the
+// caller's order, which is the order of every row sum.
 int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
     int rc = validate_mesh(d);
     if (rc) return rc;
@@ -187,10 +266,19 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
     auto rowlen = [&](int i) { return ia[i + 1] - ia[i]; };
     L.perm.resize((size_t)np);
     L.iperm.resize((size_t)np);
+    std::vector<int> base((size_t)np);
+    for (int i = 0; i < np; ++i) base[(size_t)i] = i;
+    const int mode = reorder_mode();
+    if (mode == 2 || (mode == 1 && mean_offset(d, nullptr) > 4.0 * std::sqrt((double)np))) {
+        std::vector<int> rcm = rcm_order(d);
+        std::vector<int> pos((size_t)np);
+        for (int r = 0; r < np; ++r) pos[(size_t)rcm[(size_t)r]] = r;
+        if (mode == 2 || mean_offset(d, &pos) < mean_offset(d, nullptr)) base.swap(rcm);
+    }
     const int W = std::max(1, sort_window);
     for (int w0 = 0; w0 < np; w0 += W) {
         const int w1 = std::min(np, w0 + W);
-        for (int i = w0; i < w1; ++i) L.perm[(size_t)i] = i;
+        for (int i = w0; i < w1; ++i) L.perm[(size_t)i] = base[(size_t)i];
         std::stable_sort(L.perm.begin() + w0, L.perm.begin() + w1,
                          [&](int a, int b) { return rowlen(a) < rowlen(b); });
     }
