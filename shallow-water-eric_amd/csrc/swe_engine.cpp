@@ -356,8 +356,8 @@ int dalloc(T** p, size_t n) {
 }
 
 // Device bytes per unit of each capacity.
-constexpr size_t kRowBytes = sizeof(int) * 3 + sizeof(double) * 2 + sizeof(Node) * 4 + sizeof(double) * 3 +
-                             sizeof(double) * 4 + sizeof(double) * 3;   // len dpos perm, m z, S[3] scratch, ulo, alpha res Rm nwork, stage3
+constexpr size_t kRowBytes = sizeof(int) + 2 * sizeof(uint8_t) + sizeof(double) * 2 + sizeof(Node) * 4 + sizeof(double) * 3 +
+                             sizeof(double) * 4 + sizeof(double) * 3;   // perm, len dpos, m z, S[3] scratch, ulo, alpha res Rm nwork, stage3
 constexpr size_t kSlotBytes = sizeof(int) + sizeof(int16_t) + sizeof(double2) * 2 + sizeof(double) * 2;  // col dcol c dmu d0 ework
 constexpr size_t kSliceBytes = sizeof(SliceInfo);
 constexpr size_t kInstBytes = sizeof(InstDev) + sizeof(unsigned long long) + sizeof(int);
@@ -504,7 +504,7 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     std::memset(t0.data(), 0, sizeof(InstDev) * t0.size());
     e = hipMemcpy(P->sl, sl0.data(), sizeof(SliceInfo) * sl0.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(P->tab, t0.data(), sizeof(InstDev) * t0.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(P->len, 0, sizeof(int) * R);
+    if (e == hipSuccess) e = hipMemset(P->len, 0, sizeof(uint8_t) * R);
     if (e == hipSuccess) e = hipMemset(P->vmax, 0, sizeof(unsigned long long) * (size_t)caps.instances);
     if (e == hipSuccess) e = hipMemset(P->flags, 0, sizeof(int) * (size_t)caps.instances);
     if (e == hipSuccess) e = hipMemset(P->bc, 0xff, sizeof(BcEntry) * (size_t)std::max<int64_t>(caps.bc, 1));
@@ -559,6 +559,7 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     for (int s = 0; s < L.nsl; ++s)
         sls[(size_t)s] = SliceInfo{p0 + L.sptr[(size_t)s], e0 + L.eoff[(size_t)s], k,
                                    (int)(L.eoff[(size_t)s + 1] - L.eoff[(size_t)s]), {0, 0}};
+    std::vector<uint8_t> len8(L.len.begin(), L.len.end()), dpos8(L.dpos.begin(), L.dpos.end());
     std::vector<int> colg((size_t)L.nslot);
     for (int64_t q = 0; q < L.nslot; ++q) colg[(size_t)q] = L.col[(size_t)q] + (int)r0;
     std::vector<BcEntry> bce((size_t)nbc);
@@ -577,8 +578,8 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, cs);
     };
     hipError_t e = hipStreamWaitEvent(cs, P->released, 0);
-    if (e == hipSuccess) e = up(P->len + r0, L.len.data(), sizeof(int) * L.len.size());
-    if (e == hipSuccess) e = up(P->dpos + r0, L.dpos.data(), sizeof(int) * L.dpos.size());
+    if (e == hipSuccess) e = up(P->len + r0, len8.data(), len8.size());
+    if (e == hipSuccess) e = up(P->dpos + r0, dpos8.data(), dpos8.size());
     if (e == hipSuccess) e = up(P->perm + r0, L.perm.data(), sizeof(int) * L.perm.size());
     if (e == hipSuccess) e = up(P->m + r0, L.m.data(), sizeof(double) * L.m.size());
     if (e == hipSuccess) e = up(P->z + r0, L.z.data(), sizeof(double) * L.z.size());

@@ -114,8 +114,8 @@ struct Pool {
     int wmax = 0;
     // device arrays: mesh
     SliceInfo* sl = nullptr;
-    int* len = nullptr;
-    int* dpos = nullptr;
+    uint8_t* len = nullptr;           // row lengths and diagonal positions (rows hold at most kMaxRow)
+    uint8_t* dpos = nullptr;
     int* perm = nullptr;              // internal row -> caller's node (local numbering)
     double* m = nullptr;
     double* z = nullptr;

@@ -84,13 +84,13 @@ constexpr int kColFar = -32768;
 // Read-only view of the arena's mesh arrays handed to every row kernel.
 struct Mesh {
     const SliceInfo* __restrict__ sl;
-    const int* __restrict__ len;        // entries of each row (0 for padding rows)
+    const uint8_t* __restrict__ len;    // entries of each row (0 for padding rows)
     const int* __restrict__ col;        // global column of each slot
     const int16_t* __restrict__ dcol;   // column - row of each slot, kColFar where that does not fit
     const double2* __restrict__ c;      // (cx, cy) of each slot
     const double* __restrict__ m;       // lumped mass of each row
     const uint16_t* __restrict__ elist; // per slice: slot offsets (q*64 + lane) of off-diagonal entries
-    const int* __restrict__ dpos;       // position q of the diagonal in each row
+    const uint8_t* __restrict__ dpos;   // position q of the diagonal in each row
     const int* __restrict__ act;        // the slices a launch covers, ascending (one wave each)
     int nact;                           // number of them
     int wmax;                           // longest row of the resident instances
