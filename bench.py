@@ -324,6 +324,7 @@ def main():
                     help="default run: leave out the reduced batch and serve measurements")
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB device copy used to calibrate the HBM byte counters")
@@ -374,6 +375,8 @@ def main():
     s = p.solver()
     s.set_state(p.u0)
     s.time = 0.0
+    if not args.no_tune:
+        s.tune(3)           # launch schedules (workgroup size, XCD order) timed on this mesh; state restored
     s.run(args.warmup)
     s.sync()
     s.enable_kernel_timing(dominant)
@@ -384,6 +387,7 @@ def main():
     nlaunch, kms = s.kernel_timing(dominant)
     s.enable_kernel_timing(None)
     hmin = float(s.get_state()[:, 0].min())
+    schedule = {k: list(v) for k, v in s.schedule.items()}
     rate = p.np * 3 * args.steps / el
     bpn = kernel_bytes_per_node(dominant, k, alpha)
     avg_s = kms / 1e3 / max(nlaunch, 1)
@@ -402,7 +406,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": p.name, "nodes": p.np, "entries": p.nnz, "viscosity": p.params["viscosity"],
-                   "alpha_limit": bool(alpha), "steps": args.steps, "hmin_end": hmin},
+                   "alpha_limit": bool(alpha), "steps": args.steps, "hmin_end": hmin, "schedule": schedule},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
                      "launch_ms": avg_s * 1e3, "bytes_per_node": bpn},
