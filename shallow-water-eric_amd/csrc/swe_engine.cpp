@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -143,6 +145,28 @@ static bool same_mode(const Mode& a, const Mode& b) {
 }
 
 // ---- host layout ----------------------------------------------------------------------------
+namespace {
+
+// f(begin, end) over [0, n) in contiguous chunks on the host's cores (serially for small n).
+template <class F>
+void parallel_for(int64_t n, F&& f, int64_t serial_below = 1 << 15) {
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (n < serial_below || nt == 1) {
+        if (n > 0) f((int64_t)0, n);
+        return;
+    }
+    const int64_t chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+        const int64_t b = (int64_t)t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([=, &f] { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
+}  // namespace
+
 int validate_mesh(const swe_mesh_desc& d) {
     if (d.np <= 0 || d.nnz <= 0 || !d.ia || !d.ja || !d.cx || !d.cy || !d.lumped || !d.bath)
         return fail(SWE_EINVAL, "mesh descriptor is incomplete");
@@ -150,17 +174,37 @@ int validate_mesh(const swe_mesh_desc& d) {
     if (d.ia[0] != 1 || d.ia[np] - 1 != nnz) return fail(SWE_EINVAL, "ia must start at 1 and end at nnz+1");
     for (int i = 0; i < np; ++i)
         if (d.ia[i + 1] < d.ia[i]) return fail(SWE_EINVAL, "ia is not monotone at row %d", i + 1);
-    for (int i = 0; i < np; ++i) {
+    // rows checked in parallel; the first bad row (in row order) is reported
+    auto row_ok = [&](int i) {
         bool has_diag = false;
         for (int p = d.ia[i] - 1; p < d.ia[i + 1] - 1; ++p) {
             const int j = d.ja[p];
-            if (j < 1 || j > np) return fail(SWE_EINVAL, "ja(%d)=%d out of range", p + 1, j);
+            if (j < 1 || j > np) return false;
             has_diag |= (j - 1 == i);
         }
-        if (!has_diag) return fail(SWE_EINVAL, "row %d has no diagonal entry", i + 1);
+        return has_diag && d.lumped[i] > 0.0;
+    };
+    std::atomic<int> bad{np};
+    parallel_for(np, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i)
+            if (!row_ok((int)i)) {
+                int cur = bad.load();
+                while (i < cur && !bad.compare_exchange_weak(cur, (int)i)) {
+                }
+                return;
+            }
+    });
+    if (bad.load() < np) {
+        const int i = bad.load();
+        for (int p = d.ia[i] - 1; p < d.ia[i + 1] - 1; ++p)
+            if (d.ja[p] < 1 || d.ja[p] > np) return fail(SWE_EINVAL, "ja(%d)=%d out of range", p + 1, d.ja[p]);
+        if (!(d.lumped[i] > 0.0)) {
+            bool has_diag = false;
+            for (int p = d.ia[i] - 1; p < d.ia[i + 1] - 1; ++p) has_diag |= (d.ja[p] - 1 == i);
+            if (has_diag) return fail(SWE_EINVAL, "lumped(%d) must be > 0", i + 1);
+        }
+        return fail(SWE_EINVAL, "row %d has no diagonal entry", i + 1);
     }
-    for (int i = 0; i < np; ++i)
-        if (!(d.lumped[i] > 0.0)) return fail(SWE_EINVAL, "lumped(%d) must be > 0", i + 1);
     const int n[3] = {d.n_h_dir, d.n_ux_dir, d.n_uy_dir};
     const int* l[3] = {d.h_dir, d.ux_dir, d.uy_dir};
     const char* names[3] = {"h_js_D", "ux_js_D", "uy_js_D"};
@@ -177,17 +221,21 @@ namespace {
 // Mean |i - j| over the off-diagonal entries of the CSR graph in the numbering `pos`
 // (pos[i] = position of row i; identity when null).
 double mean_offset(const swe_mesh_desc& d, const std::vector<int>* pos) {
-    long double sum = 0;
-    int64_t n = 0;
-    for (int i = 0; i < d.np; ++i)
-        for (int p = d.ia[i] - 1; p < d.ia[i + 1] - 1; ++p) {
-            const int j = d.ja[p] - 1;
-            if (j == i) continue;
-            const int a = pos ? (*pos)[(size_t)i] : i, b = pos ? (*pos)[(size_t)j] : j;
-            sum += (a > b) ? a - b : b - a;
-            ++n;
-        }
-    return n ? (double)(sum / n) : 0.0;
+    std::atomic<int64_t> sum{0}, n{0};
+    parallel_for(d.np, [&](int64_t b0, int64_t e0) {
+        int64_t s = 0, c = 0;
+        for (int64_t i = b0; i < e0; ++i)
+            for (int p = d.ia[i] - 1; p < d.ia[i + 1] - 1; ++p) {
+                const int j = d.ja[p] - 1;
+                if (j == (int)i) continue;
+                const int a = pos ? (*pos)[(size_t)i] : (int)i, b = pos ? (*pos)[(size_t)j] : j;
+                s += (a > b) ? a - b : b - a;
+                ++c;
+            }
+        sum += s;
+        n += c;
+    });
+    return n.load() ? (double)sum.load() / (double)n.load() : 0.0;
 }
 
 // Reverse Cuthill-McKee order of the CSR graph (each component from a pseudo-peripheral node, the
@@ -276,18 +324,25 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
         if (mode == 2 || mean_offset(d, &pos) < mean_offset(d, nullptr)) base.swap(rcm);
     }
     const int W = std::max(1, sort_window);
-    for (int w0 = 0; w0 < np; w0 += W) {
-        const int w1 = std::min(np, w0 + W);
-        for (int i = w0; i < w1; ++i) L.perm[(size_t)i] = base[(size_t)i];
-        std::stable_sort(L.perm.begin() + w0, L.perm.begin() + w1,
-                         [&](int a, int b) { return rowlen(a) < rowlen(b); });
-    }
-    for (int r = 0; r < np; ++r) L.iperm[(size_t)L.perm[(size_t)r]] = r;
+    const int nwin = (np + W - 1) / W;
+    parallel_for(nwin, [&](int64_t b, int64_t e) {
+        for (int64_t w = b; w < e; ++w) {
+            const int w0 = (int)w * W, w1 = std::min(np, w0 + W);
+            for (int i = w0; i < w1; ++i) L.perm[(size_t)i] = base[(size_t)i];
+            std::stable_sort(L.perm.begin() + w0, L.perm.begin() + w1,
+                             [&](int a, int c) { return rowlen(a) < rowlen(c); });
+        }
+    }, 512);
     L.np_pad = (np + 63) / 64 * 64;
     L.nsl = L.np_pad / 64;
     L.len.assign((size_t)L.np_pad, 0);
     L.sptr.assign((size_t)L.nsl + 1, 0);
-    for (int r = 0; r < np; ++r) L.len[(size_t)r] = rowlen(L.perm[(size_t)r]);
+    parallel_for(np, [&](int64_t b, int64_t e) {
+        for (int64_t r = b; r < e; ++r) {
+            L.iperm[(size_t)L.perm[(size_t)r]] = (int)r;
+            L.len[(size_t)r] = rowlen(L.perm[(size_t)r]);
+        }
+    });
     for (int sl = 0; sl < L.nsl; ++sl) {
         int w = 0;
         for (int l = 0; l < 64; ++l) w = std::max(w, L.len[(size_t)sl * 64 + l]);
@@ -303,37 +358,50 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
     L.m.assign((size_t)np, 0.0);
     L.z.assign((size_t)np, 0.0);
     L.dpos.assign((size_t)L.np_pad, 0);
-    for (int r = 0; r < np; ++r) {
-        const int i = L.perm[(size_t)r];
-        const int64_t base = L.sptr[(size_t)(r >> 6)] + (r & 63);
-        const int p0 = ia[i] - 1;
-        for (int q = 0; q < L.len[(size_t)r]; ++q) {
-            const int p = p0 + q;
-            const int j = d.ja[p] - 1;
-            if (j == i) L.dpos[(size_t)r] = q;
-            const int64_t slot = base + 64 * (int64_t)q;
-            L.col[(size_t)slot] = L.iperm[(size_t)j];
-            const int off = L.iperm[(size_t)j] - r;
-            L.dcol[(size_t)slot] = (int16_t)((off > kColFar && off <= 32767) ? off : kColFar);
-            L.c[(size_t)slot] = make_double2(d.cx[p], d.cy[p]);
-            L.csr2slot[(size_t)p] = slot;
+    parallel_for(np, [&](int64_t rb, int64_t re) {
+        for (int r = (int)rb; r < (int)re; ++r) {
+            const int i = L.perm[(size_t)r];
+            const int64_t base = L.sptr[(size_t)(r >> 6)] + (r & 63);
+            const int p0 = ia[i] - 1;
+            for (int q = 0; q < L.len[(size_t)r]; ++q) {
+                const int p = p0 + q;
+                const int j = d.ja[p] - 1;
+                if (j == i) L.dpos[(size_t)r] = q;
+                const int64_t slot = base + 64 * (int64_t)q;
+                L.col[(size_t)slot] = L.iperm[(size_t)j];
+                const int off = L.iperm[(size_t)j] - r;
+                L.dcol[(size_t)slot] = (int16_t)((off > kColFar && off <= 32767) ? off : kColFar);
+                L.c[(size_t)slot] = make_double2(d.cx[p], d.cy[p]);
+                L.csr2slot[(size_t)p] = slot;
+            }
+            L.m[(size_t)r] = d.lumped[i];
+            L.z[(size_t)r] = d.bath[i];
         }
-        L.m[(size_t)r] = d.lumped[i];
-        L.z[(size_t)r] = d.bath[i];
-    }
-    // compact per-slice lists of off-diagonal slots, in slot order (coalesced in the dt pass)
+    });
+    // compact per-slice lists of off-diagonal slots, in slot order (coalesced in the dt pass):
+    // counted per slice, offsets by a prefix sum, then filled per slice
     L.eoff.assign((size_t)L.nsl + 1, 0);
-    L.elist.clear();
-    L.elist.reserve((size_t)std::max(0, nnz - np));
-    for (int sl = 0; sl < L.nsl; ++sl) {
+    auto slice_entries = [&](int sl, uint16_t* out) {
         const int w = (int)((L.sptr[(size_t)sl + 1] - L.sptr[(size_t)sl]) / 64);
+        int64_t n = 0;
         for (int q = 0; q < w; ++q)
             for (int l = 0; l < 64; ++l) {
                 const int r = sl * 64 + l;
-                if (r < np && q < L.len[(size_t)r] && q != L.dpos[(size_t)r]) L.elist.push_back((uint16_t)(q * 64 + l));
+                if (r < np && q < L.len[(size_t)r] && q != L.dpos[(size_t)r]) {
+                    if (out) out[n] = (uint16_t)(q * 64 + l);
+                    ++n;
+                }
             }
-        L.eoff[(size_t)sl + 1] = (int64_t)L.elist.size();
-    }
+        return n;
+    };
+    parallel_for(L.nsl, [&](int64_t b, int64_t e) {
+        for (int64_t sl = b; sl < e; ++sl) L.eoff[(size_t)sl + 1] = slice_entries((int)sl, nullptr);
+    }, 512);
+    for (int sl = 0; sl < L.nsl; ++sl) L.eoff[(size_t)sl + 1] += L.eoff[(size_t)sl];
+    L.elist.assign((size_t)L.eoff[(size_t)L.nsl], 0);
+    parallel_for(L.nsl, [&](int64_t b, int64_t e) {
+        for (int64_t sl = b; sl < e; ++sl) slice_entries((int)sl, L.elist.data() + L.eoff[(size_t)sl]);
+    }, 512);
     const int* src[3] = {d.h_dir, d.ux_dir, d.uy_dir};
     const int ns[3] = {d.n_h_dir, d.n_ux_dir, d.n_uy_dir};
     for (int k = 0; k < 3; ++k) {

@@ -32,13 +32,7 @@ struct LaunchCfg {
     int remap = 1;      // XCD-aware workgroup order
 };
 struct Tuning {
-    LaunchCfg k[KID_COUNT];
-    // Defaults: what the tuner picks on the 16.8M-node FCT workload (the dt pass and the three FCT
-    // passes in dispatch order, the FCT passes in 128-thread workgroups).
-    Tuning() {
-        k[KID_DT].remap = 0;
-        for (int id : {KID_FCT_LO, KID_RMINUS, KID_APPLY}) k[id] = LaunchCfg{128, 0};
-    }
+    LaunchCfg k[KID_COUNT];   // defaults: 256 threads, XCD-aware order (the tuner refines per mesh)
 };
 // Workgroup sizes a kernel can be launched with.
 bool block_supported(int kid, int block);
