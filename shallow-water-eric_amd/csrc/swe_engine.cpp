@@ -909,8 +909,9 @@ int pool_snapshot(Pool* P, long long* ticket) {
 
 int pool_finished(Pool* P, long long ticket, int* ids, int cap, int* n) {
     *n = 0;
+    if (ticket < 0 || P->snaps[ticket % 4].ticket != ticket)
+        return fail(SWE_EINVAL, "snapshot %lld is not available (only the last 4 are kept)", ticket);
     Snapshot& sn = P->snaps[ticket % 4];
-    if (ticket < 0 || sn.ticket != ticket) return fail(SWE_EINVAL, "snapshot %lld is not available", ticket);
     SWE_HIPCHK(hipEventSynchronize(sn.ev));
     const int ni = std::min(P->ninst(), sn.n);
     bool unlisted = false;
