@@ -134,6 +134,13 @@ int swe_fct_positivity(swe_handle *h, const double *unext_scaled, const double *
 int swe_fct_positivity_csr(int np, int nnz, const int *ia, const int *ja, const double *unext_scaled,
                            const double *f, double htiny, double *lij);
 
+/* FCT_generic (SOURCES/SAFE/fct.f90:3-43), the two-sided limiter the reference keeps but does not
+ * call: with local bounds maxn(np), minn(np), fluxes f(nnz) on a 1-based CSR matrix and the lumped
+ * mass dg(np), unext += (SUM_j l_ij f_ij)/dg, l_ij = MIN(R+_i, R-_j) for f_ij >= 0 and
+ * MIN(R-_i, R+_j) otherwise. */
+int swe_fct_generic_csr(int np, int nnz, const int *ia, const int *ja, double *unext, const double *maxn,
+                        const double *minn, const double *f, const double *dg);
+
 /* Device-resident state. */
 int swe_set_state(swe_handle *h, const double *un);
 int swe_get_state(swe_handle *h, double *un);

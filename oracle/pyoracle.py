@@ -51,6 +51,7 @@ def lib():
         L.orc_set_params.argtypes = [P, C.POINTER(OrcParams)]
         L.orc_lambda_batch.argtypes = [C.c_int, P, C.c_double, C.c_double, P, P]
         L.orc_fct_positivity.argtypes = [C.c_int, P, P, P, P, C.c_double, P]
+        L.orc_fct_generic.argtypes = [C.c_int, P, P, P, P, P, P, P]
         L.orc_compute_dij.restype = C.c_int
         L.orc_compute_dij.argtypes = [P, P, P]
         L.orc_compute_dt.restype = C.c_double
@@ -90,6 +91,16 @@ def lambda_batch(rec8: np.ndarray, gravity: float, htiny: float):
     err = C.c_int(0)
     lib().orc_lambda_batch(rec8.shape[0], _p(rec8), gravity, htiny, _p(out), C.byref(err))
     return out, err.value
+
+
+def fct_generic(ia, ja, f, unext, maxn, minn, dg):
+    """FCT_generic (SAFE/fct.f90:3-43) on a 0-based CSR matrix; returns the limited unext."""
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    u = np.array(unext, dtype=np.float64, copy=True)
+    args = [np.ascontiguousarray(x, np.float64) for x in (f, maxn, minn, dg)]
+    lib().orc_fct_generic(ia.size - 1, _p(ia), _p(ja), _p(args[0]), _p(args[1]), _p(args[2]), _p(args[3]), _p(u))
+    return u
 
 
 def fct_positivity(ia, ja, f, unext_scaled, htiny):

@@ -7,6 +7,8 @@
  *            SOURCES/update.f90:18-132 (construct_matrices: lumped, diag, c_ij)
  *   step     SOURCES/update.f90:134-935, SOURCES/fct.f90:5-42,
  *            SOURCES/condlim.f90:21-90,425-486,879-981, SOURCES/main.f90:61-92,166-188
+ *   limiter  SOURCES/SAFE/fct.f90:3-43 (FCT_generic, the two-sided limiter the reference keeps but
+ *            does not call; pinned by its invariants, see tests/test_oracle_math.py)
  *
  * Fortran's MAX(a,b) is evaluated as (a > b ? a : b) and MIN(a,b) as (a < b ? a : b), which is
  * how flang lowers them; SUM over an array section is a sequential left-to-right loop from 0.
@@ -725,6 +727,43 @@ void orc_fct_positivity(int np, const int *ia, const int *ja, const double *f,
         for (int p = ia[i]; p < ia[i + 1]; p++)
             lij[p] = (f[p] >= 0.0) ? R[ja[p]] : R[i];
     free(R);
+}
+
+/* FCT_generic (SAFE/fct.f90:3-43): the two-sided limiter, kept but not called by the reference.
+ * 0-based CSR; unext is limited in place. */
+void orc_fct_generic(int np, const int *ia, const int *ja, const double *f, const double *maxn,
+                     const double *minn, const double *dg, double *unext)
+{
+    const double smallplus = 1.0e-15, smallminus = -1.0e-15;
+    double *Rp = (double *)xmalloc(sizeof(double) * (size_t)np);
+    double *Rm = (double *)xmalloc(sizeof(double) * (size_t)np);
+    for (int i = 0; i < np; i++) {
+        double Qp = dg[i] * (maxn[i] - unext[i]);
+        double Qm = dg[i] * (minn[i] - unext[i]);
+        double Pp = smallplus, Pm = smallminus;
+        for (int p = ia[i]; p < ia[i + 1]; p++) {
+            double fij = f[p];
+            if (fij >= 0.0) Pp = Pp + fij;
+            else Pm = Pm + fij;
+        }
+        Rp[i] = fminF(Qp / Pp, 1.0);
+        Rm[i] = fminF(Qm / Pm, 1.0);
+    }
+    /* every R is known before unext changes: the second loop reads the first loop's unext only
+       through R, and writes row i from row i's own sum */
+    for (int i = 0; i < np; i++) {
+        double x = 0.0;
+        for (int p = ia[i]; p < ia[i + 1]; p++) {
+            int j = ja[p];
+            double fij = f[p], lij;
+            if (fij >= 0.0) lij = fminF(Rp[i], Rm[j]);
+            else lij = fminF(Rm[i], Rp[j]);
+            x = x + lij * fij;
+        }
+        unext[i] = unext[i] + x / dg[i];
+    }
+    free(Rp);
+    free(Rm);
 }
 
 /* apply_viscosity (update.f90:684-718) */

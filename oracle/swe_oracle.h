@@ -50,6 +50,8 @@ double orc_lambda_vacc(const double *ul, const double *ur, const double *nij,
 void orc_lambda_batch(int n, const double *rec8, double gravity, double htiny, double *out, int *err);
 void orc_fct_positivity(int np, const int *ia, const int *ja, const double *f,
                         const double *unext_scaled, double htiny, double *lij);
+void orc_fct_generic(int np, const int *ia, const int *ja, const double *f, const double *maxn,
+                     const double *minn, const double *dg, double *unext);
 
 /* step pieces on a context */
 int orc_compute_dij(orc_ctx *c, const double *un, double *dij_out);

@@ -325,6 +325,48 @@ int swe_fct_positivity_csr(int np, int nnz, const int* ia, const int* ja, const 
     return rc;
 }
 
+int swe_fct_generic_csr(int np, int nnz, const int* ia, const int* ja, double* unext, const double* maxn,
+                        const double* minn, const double* f, const double* dg) {
+    if (np <= 0 || nnz <= 0 || !ia || !ja || !unext || !maxn || !minn || !f || !dg)
+        return fail(SWE_EINVAL, "null or empty argument");
+    if (ia[0] != 1 || ia[np] - 1 != nnz) return fail(SWE_EINVAL, "ia must start at 1 and end at nnz+1");
+    for (int i = 0; i < np; ++i)
+        if (ia[i + 1] < ia[i]) return fail(SWE_EINVAL, "ia is not monotone at row %d", i + 1);
+    for (int p = 0; p < nnz; ++p)
+        if (ja[p] < 1 || ja[p] > np) return fail(SWE_EINVAL, "ja(%d)=%d out of range", p + 1, ja[p]);
+    int *dia = nullptr, *dja = nullptr;
+    double *du = nullptr, *dmax = nullptr, *dmin = nullptr, *df = nullptr, *ddg = nullptr;
+    const size_t n = (size_t)np, z = (size_t)nnz;
+    hipError_t e = hipMalloc((void**)&dia, sizeof(int) * (n + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&dja, sizeof(int) * z);
+    if (e == hipSuccess) e = hipMalloc((void**)&du, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&dmax, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&dmin, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&ddg, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc((void**)&df, sizeof(double) * z);
+    int rc = SWE_OK;
+    if (e != hipSuccess) rc = fail(SWE_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+    if (!rc) {
+        e = hipMemcpy(dia, ia, sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dja, ja, sizeof(int) * z, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(du, unext, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dmax, maxn, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dmin, minn, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ddg, dg, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(df, f, sizeof(double) * z, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(SWE_EHIP, "swe_fct_generic_csr: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = launch_fctgen_csr(np, dia, dja, df, dmax, dmin, ddg, du);
+    if (!rc) {
+        e = hipMemcpy(unext, du, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(SWE_EHIP, "swe_fct_generic_csr: %s", hipGetErrorString(e));
+    }
+    void* bufs[] = {dia, dja, du, dmax, dmin, ddg, df};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    return rc;
+}
+
 int swe_set_time(swe_handle* h, double t) {
     if (!h) return fail(SWE_EINVAL, "null handle");
     int rc = pool_set_field(h->P, kId, offsetof(InstDev, time), &t, sizeof t);

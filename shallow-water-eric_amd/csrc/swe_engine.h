@@ -223,5 +223,7 @@ struct PokeWords {
 int launch_poke(Pool& P, void* dst, const void* src, size_t bytes);
 int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, const double* us, double htiny,
                       double* lij);
+int launch_fctgen_csr(int np, const int* ia, const int* ja, const double* f, const double* maxn, const double* minn,
+                      const double* dg, double* u);
 
 }  // namespace swe

@@ -1074,6 +1074,43 @@ __global__ void k_fctpos_csr_l(int np, const int* __restrict__ ia, const int* __
     for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) lij[p] = (f[p] >= 0.0) ? Rm[ja[p] - 1] : Rm[i];
 }
 
+// FCT_generic (SOURCES/SAFE/fct.f90:3-43), the two-sided limiter on a plain 1-based CSR matrix:
+// Q+- = dg*(max/min - u), P+- = +-1e-15 + sums of the positive / negative fluxes in entry order,
+// R+- = MIN(Q/P, 1); then u_i += (SUM_p l_ij f_ij)/dg_i with l_ij = MIN(R+_i, R-_j) for f_ij >= 0,
+// MIN(R-_i, R+_j) otherwise.
+__global__ void k_fctgen_csr_r(int np, const int* __restrict__ ia, const double* __restrict__ f,
+                               const double* __restrict__ u, const double* __restrict__ maxn,
+                               const double* __restrict__ minn, const double* __restrict__ dg,
+                               double2* __restrict__ R) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    const double Qp = dg[i] * (maxn[i] - u[i]);
+    const double Qm = dg[i] * (minn[i] - u[i]);
+    double Pp = 1.0e-15, Pm = -1.0e-15;
+    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) {
+        const double fij = f[p];
+        if (fij >= 0.0) Pp = Pp + fij;
+        else Pm = Pm + fij;
+    }
+    R[i] = make_double2(fminF(Qp / Pp, 1.0), fminF(Qm / Pm, 1.0));
+}
+
+__global__ void k_fctgen_csr_apply(int np, const int* __restrict__ ia, const int* __restrict__ ja,
+                                   const double* __restrict__ f, const double2* __restrict__ R,
+                                   const double* __restrict__ dg, double* __restrict__ u) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    const double2 Ri = R[i];
+    double x = 0.0;
+    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) {
+        const double fij = f[p];
+        const double2 Rj = R[ja[p] - 1];
+        const double lij = (fij >= 0.0) ? fminF(Ri.x, Rj.y) : fminF(Ri.y, Rj.x);
+        x = x + lij * fij;
+    }
+    u[i] = u[i] + x / dg[i];
+}
+
 inline unsigned blocks_of(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Dispatch a kernel template over the supported workgroup sizes.
@@ -1351,6 +1388,22 @@ int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, con
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(dR);
     if (e != hipSuccess) return fail(SWE_EHIP, "FCT_positivity kernels: %s", hipGetErrorString(e));
+    return SWE_OK;
+}
+
+int launch_fctgen_csr(int np, const int* ia, const int* ja, const double* f, const double* maxn, const double* minn,
+                      const double* dg, double* u) {
+    double2* dR = nullptr;
+    hipError_t e = hipMalloc((void**)&dR, sizeof(double2) * (size_t)np);
+    if (e != hipSuccess) return fail(SWE_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+    const unsigned g = blocks_of(np, 256);
+    // every R+- is complete before any row applies its fluxes: two launches on one stream
+    hipLaunchKernelGGL(k_fctgen_csr_r, dim3(g), dim3(256), 0, 0, np, ia, f, (const double*)u, maxn, minn, dg, dR);
+    hipLaunchKernelGGL(k_fctgen_csr_apply, dim3(g), dim3(256), 0, 0, np, ia, ja, f, (const double2*)dR, dg, u);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(dR);
+    if (e != hipSuccess) return fail(SWE_EHIP, "FCT_generic kernels: %s", hipGetErrorString(e));
     return SWE_OK;
 }
 

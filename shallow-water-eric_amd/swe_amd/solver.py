@@ -85,6 +85,7 @@ SIGNATURES = {
     "swe_get_schedule": (_I, [_P, _P, _P]),
     "swe_set_schedule": (_I, [_P, _P, _P]),
     "swe_use_graph": (_I, [_P, _I]),
+    "swe_fct_generic_csr": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "swe_step_begin": (_I, [_P, C.POINTER(_D)]),
     "swe_step_finish": (_I, [_P, _P]),
     "swe_batch_bytes": (C.c_ulonglong, [C.POINTER(BatchCaps)]),
@@ -192,6 +193,18 @@ def fct_positivity_csr(ia, ja, unext_scaled, f, htiny):
     _check(lib.swe_fct_positivity_csr(ia1.size - 1, ja1.size, _ptr(ia1), _ptr(ja1), _ptr(us), _ptr(f),
                                       float(htiny), _ptr(lij)))
     return lij
+
+
+def fct_generic_csr(ia, ja, unext, maxn, minn, f, dg):
+    """FCT_generic (SOURCES/SAFE/fct.f90:3-43) on the GPU; 0-based CSR (ia, ja); returns the limited
+    unext (a new array)."""
+    lib = load_library()
+    ia1 = np.ascontiguousarray(ia, np.int32) + 1
+    ja1 = np.ascontiguousarray(ja, np.int32) + 1
+    u = np.array(unext, dtype=np.float64, copy=True)
+    args = [np.ascontiguousarray(x, np.float64) for x in (maxn, minn, f, dg)]
+    _check(lib.swe_fct_generic_csr(ia1.size - 1, ja1.size, _ptr(ia1), _ptr(ja1), _ptr(u), *[_ptr(a) for a in args]))
+    return u
 
 
 def make_params(*, gravity, htiny, cfl, max_water_h, viscosity="viscous", viscous_type=2, alpha_limit=False,

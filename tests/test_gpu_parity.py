@@ -270,3 +270,20 @@ def test_scrambled_numbering_against_oracle(oracle):
         t, d = s.run(10, time=0.0)
         assert same_bits(d, rdts) and t == rt
         assert same_bits(s.get_state(), ref)
+
+
+def test_fct_generic_against_oracle(oracle):
+    """FCT_generic (SAFE/fct.f90:3-43) on the GPU, bit for bit against the CPU checker."""
+    from swe_amd.solver import fct_generic_csr
+    rng = np.random.default_rng(11)
+    for n, k in ((50, 4), (5000, 7), (60000, 9)):
+        rows = [np.unique(np.concatenate([[i], rng.integers(0, n, k)])) for i in range(n)]
+        ia = np.concatenate([[0], np.cumsum([r.size for r in rows])]).astype(np.int32)
+        ja = np.concatenate(rows).astype(np.int32)
+        f = rng.normal(size=ja.size)
+        u = rng.uniform(0, 1, n)
+        maxn = u + rng.uniform(0, 0.05, n)
+        minn = u - rng.uniform(0, 0.05, n)
+        dg = rng.uniform(0.5, 2.0, n)
+        print("fct_generic", n, flush=True)
+        assert same_bits(fct_generic_csr(ia, ja, u, maxn, minn, f, dg), oracle.fct_generic(ia, ja, f, u, maxn, minn, dg))
