@@ -303,8 +303,38 @@ def bench_setup(n_side=4097, reps=2):
         del host, dev
         print("# setup: host %.2f s, device %.2f s, identical %s" % (t_host[-1], t_dev[-1], same), file=sys.stderr,
               flush=True)
+    # the whole set-up of a solver: host assembly + host layout (swe_create) against device assembly +
+    # layout built on the GPU (swe_create_from_assembly); then two steps of each must agree bitwise
+    from swe_amd import problems as PR
+    from swe_amd.solver import BC_INFLOW, Solver
+    bath = PR.bump_field(m.rr, 1000.0, 64, 20261202)
+    prm = dict(gravity=9.81, htiny=1.0e-4, cfl=0.25, max_water_h=1.0, viscosity="fct", viscous_type=2,
+               alpha_limit=True, type_test=4, mannings=0.0)
+    lists = dict(h_dir=M.dirichlet_nodes(m, (4,)), ux_dir=M.dirichlet_nodes(m, (2,)), uy_dir=M.dirichlet_nodes(m, (1, 3)))
+    u0 = np.zeros((m.np, 3))
+    u0[:, 0] = np.maximum(1.0 - bath, 0.0)
+    res = []
+    for dev_path in (False, True):
+        t0 = time.perf_counter()
+        if dev_path:
+            s = Solver.from_elements(m.jj, m.rr, bath, **lists, **prm)
+        else:
+            ia, ja, _diag, _mass, lumped, cij = assemble(m.jj, m.rr)
+            s = Solver(ia, ja, cij, lumped, bath, **lists, **prm)
+            del ia, ja, _diag, _mass, lumped, cij
+        dt_create = time.perf_counter() - t0
+        s.set_bc(BC_INFLOW, 0.05, 20.0, 1.0)
+        s.set_state(u0)
+        _t, dts = s.run(2, time=0.0)
+        res.append((dt_create, s.get_state(), dts))
+        s.close()
+        del s
+        print("# setup: solver created (%s) in %.2f s" % ("device" if dev_path else "host", dt_create),
+              file=sys.stderr, flush=True)
+    same_run = bool(np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2]))
     return {"nodes": int(m.np), "elements": int(m.jj.shape[0]), "host_s": min(t_host), "device_s": min(t_dev),
-            "host_threads": os.cpu_count(), "identical": bool(same)}
+            "host_threads": os.cpu_count(), "identical": bool(same),
+            "solver_host_s": res[0][0], "solver_device_s": res[1][0], "solver_runs_identical": same_run}
 
 
 def main():

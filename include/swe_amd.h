@@ -107,6 +107,16 @@ int swe_assembly_destroy(swe_assembly *a);
  * reverse Cuthill-McKee when the caller's numbering scatters neighbours (mean |i-j| above
  * 4*sqrt(np)), 0 never, 2 always.  These are layout choices; results do not depend on them. */
 int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **out);
+/* swe_create over a mesh assembled by swe_assemble_device, without taking the matrices to the
+ * host: the solver's layout is built on the GPU from the assembly (rows sorted by length inside
+ * 64-row windows of the caller's numbering, no reverse Cuthill-McKee), bath and the Dirichlet
+ * lists (1-based, as in swe_mesh_desc) are copied from the host.  The solver does not keep the
+ * assembly; results are bitwise those of swe_create on the same arrays.  This replaces, together
+ * with swe_assemble_device, construct_matrices (SOURCES/update.f90:18-132, called at
+ * SOURCES/main.f90:21) for runs that keep their matrices on the device. */
+int swe_create_from_assembly(const swe_assembly *a, const double *bath, int n_h_dir, const int *h_dir,
+                             int n_ux_dir, const int *ux_dir, int n_uy_dir, const int *uy_dir,
+                             const swe_params *prm, swe_handle **out);
 int swe_destroy(swe_handle *h);
 int swe_set_params(swe_handle *h, const swe_params *prm);
 

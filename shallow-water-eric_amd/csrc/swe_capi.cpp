@@ -192,6 +192,56 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
     return SWE_OK;
 }
 
+int swe_create_from_assembly(const swe_assembly* a, const double* bath, int n_h_dir, const int* h_dir,
+                             int n_ux_dir, const int* ux_dir, int n_uy_dir, const int* uy_dir, const swe_params* prm,
+                             swe_handle** out) {
+    clear_error();
+    if (!a || !bath || !prm || !out) return fail(SWE_EINVAL, "null argument");
+    *out = nullptr;
+    int rc;
+    Mode md;
+    if ((rc = mode_of(*prm, md))) return rc;
+    DeviceLayout DL;
+    if ((rc = build_layout_device(a, DL))) return rc;
+    if (DL.nnz > INT_MAX) {
+        free_device_layout(DL);
+        return fail(SWE_EINVAL, "%lld entries do not fit the solver's 32-bit CSR", (long long)DL.nnz);
+    }
+    const int* lists[3] = {h_dir, ux_dir, uy_dir};
+    const int nlist[3] = {n_h_dir, n_ux_dir, n_uy_dir};
+    PoolCaps caps;
+    caps.rows = (int64_t)DL.nsl * 64;
+    caps.slots = DL.nslot;
+    caps.elist = DL.ne;
+    caps.bc = (int64_t)std::max(0, n_h_dir) + std::max(0, n_ux_dir) + std::max(0, n_uy_dir);
+    caps.instances = 1;
+    swe_handle* h = new swe_handle();
+    h->np = DL.np;
+    h->nnz = (int)DL.nnz;
+    if ((rc = pool_create(caps, &h->P))) {
+        delete h;
+        free_device_layout(DL);
+        return rc;
+    }
+    h->P->use_graph = true;
+    hipError_t e = hipMalloc((void**)&h->csr2slot, sizeof(int64_t) * (size_t)std::max<int64_t>(1, DL.nnz));
+    if (e != hipSuccess) {
+        free_handle(h);
+        free_device_layout(DL);
+        return fail(SWE_EHIP, "swe_create_from_assembly: %s", hipGetErrorString(e));
+    }
+    int id = -1;
+    rc = pool_add_assembled(h->P, a, DL, bath, lists, nlist, *prm, 0.0, std::numeric_limits<double>::infinity(),
+                            LLONG_MAX, &id, h->csr2slot);
+    free_device_layout(DL);
+    if (rc) {
+        free_handle(h);
+        return rc;
+    }
+    *out = h;
+    return SWE_OK;
+}
+
 int swe_destroy(swe_handle* h) {
     if (!h) return SWE_OK;
     free_handle(h);

@@ -29,7 +29,6 @@ namespace swe {
 
 namespace {
 thread_local std::string g_err;
-constexpr int kMaxRow = 30;   // longest row: the dt pass keeps (2 + w)*64 doubles per wave in LDS
 }  // namespace
 
 int fail(int code, const char* fmt, ...) {
@@ -139,7 +138,7 @@ int mode_of(const swe_params& p, Mode& m) {
     return SWE_OK;
 }
 
-static bool same_mode(const Mode& a, const Mode& b) {
+bool same_mode(const Mode& a, const Mode& b) {
     return a.viscosity_type == b.viscosity_type && a.viscous_type == b.viscous_type && a.alpha == b.alpha &&
            a.roundoff == b.roundoff;
 }
@@ -666,10 +665,7 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
         P->a_bc.release(b0, nbc);
         return fail(SWE_EHIP, "upload of an admitted mesh: %s", hipGetErrorString(e));
     }
-    InstHost& I = P->inst[(size_t)k];
-    I = InstHost{};
-    I.used = true;
-    I.since = P->next_ticket;
+    InstHost I;
     I.np = d.np;
     I.nnz = d.nnz;
     I.np_pad = L.np_pad;
@@ -685,14 +681,32 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     I.nh = (int)L.lists[0].size();
     I.nux = (int)L.lists[1].size();
     I.nuy = (int)L.lists[2].size();
+    if ((rc = pool_activate(P, k, I, prm, un != nullptr, t0, tfinal, max_steps))) return rc;
+    *id = k;
+    return SWE_OK;
+}
+
+int pool_activate(Pool* P, int k, const InstHost& layout, const swe_params& prm, bool packed_state, double t0,
+                  double tfinal, long long max_steps) {
+    Mode md;
+    int rc = mode_of(prm, md);
+    if (rc) return rc;
+    Consts K;
+    if ((rc = make_consts(prm, K))) return rc;
+    InstHost& I = P->inst[(size_t)k];
+    I = layout;
+    I.used = true;
+    I.final_known = false;
+    I.since = P->next_ticket;
     I.prm = prm;
-    P->wmax = std::max(P->wmax, L.wmax);
+    const int64_t r0 = I.slice0 * 64;
+    P->wmax = std::max(P->wmax, I.wmax);
     if (!P->mode_set) {
         P->mode = md;
         P->mode_set = true;
     }
     // the state, then the table entry: from the next queued step on the instance is live
-    if (un) launch_pack(*P, I, P->stage3 + 3 * r0, P->S[0]);
+    if (packed_state) launch_pack(*P, I, P->stage3 + 3 * r0, P->S[0]);
     InstDev D;
     std::memset(&D, 0, sizeof D);
     D.K = K;
@@ -709,9 +723,7 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     if ((rc = launch_poke(*P, P->vmax + k, &z64, sizeof z64))) return rc;
     if ((rc = launch_poke(*P, P->flags + k, &z32, sizeof z32))) return rc;
     P->listed[(size_t)k] = 1;
-    if ((rc = pool_update_active(P))) return rc;
-    *id = k;
-    return SWE_OK;
+    return pool_update_active(P);
 }
 
 int pool_update_active(Pool* P) {

@@ -23,6 +23,9 @@ void clear_error();
         if (e_ != hipSuccess) return ::swe::fail(SWE_EHIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
+// Longest row: the dt pass keeps (2 + w)*64 doubles per wave in LDS.
+constexpr int kMaxRow = 30;
+
 // ---- launch schedules (chosen per kernel by the autotuner) -----------------------------------
 enum KernelId : int { KID_DT = 0, KID_ALPHA, KID_STAGE_LO, KID_FCT_LO, KID_RMINUS, KID_APPLY, KID_EV, KID_COUNT };
 extern const char* const kKernelNames[KID_COUNT];
@@ -83,6 +86,7 @@ struct Mode {
     int viscosity_type = SWE_VISCOUS, viscous_type = 2, alpha = 0, roundoff = 1;
 };
 int mode_of(const swe_params& p, Mode& m);
+bool same_mode(const Mode& a, const Mode& b);
 
 // Table snapshots taken after queued steps (pinned copies, completion event, ticket).
 struct Snapshot {
@@ -177,6 +181,10 @@ void pool_destroy(Pool* P);
 // when the caller needs its maps (csr2slot, lists).
 int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const double* un, double t0,
              double tfinal, long long max_steps, int* id, HostLayout* keep);
+// Make instance k live once its mesh arrays are in place (layout: its ranges and sizes); with
+// packed_state, its state is in stage3 in the caller's numbering.  The instance joins the next step.
+int pool_activate(Pool* P, int k, const InstHost& layout, const swe_params& prm, bool packed_state, double t0,
+                  double tfinal, long long max_steps);
 int pool_remove(Pool* P, int id);
 int pool_set_params(Pool* P, int id, const swe_params& prm);
 int pool_set_bc(Pool* P, int id, int kind, double amp, double period, double level);
@@ -199,6 +207,26 @@ int pool_finished(Pool* P, long long ticket, int* ids, int cap, int* n);
 // Final state of an instance reported by pool_finished, read on the copy stream while steps run.
 int pool_result(Pool* P, int id, double* un);
 int pool_tune(Pool* P, int steps, int trials);
+
+// ---- layout of a mesh assembled on the device (swe_setup_device.hip) --------------------------
+// The same internal numbering and sliced layout build_layout makes on the host (rows sorted by
+// length inside 64-row windows of the caller's numbering), computed on the GPU from an assembly.
+struct DeviceLayout {
+    int np = 0, nsl = 0, wmax = 0;
+    int64_t nnz = 0, nslot = 0, ne = 0;
+    int* perm = nullptr;       // internal row -> caller's node
+    int* iperm = nullptr;      // caller's node -> internal row
+    int* len = nullptr;        // entries of each internal row (np_pad)
+    int64_t* sptr = nullptr;   // slot offset of each slice (nsl + 1)
+    int64_t* eoff = nullptr;   // compact-list offset of each slice (nsl + 1)
+};
+int build_layout_device(const swe_assembly* a, DeviceLayout& DL);
+void free_device_layout(DeviceLayout& DL);
+// Admit an assembled mesh with its device layout; bath(np) and the 1-based Dirichlet lists are host
+// arrays; csr2slot (may be null) receives the slot of each CSR entry, relative to the instance.
+int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, const double* bath,
+                       const int* const lists[3], const int nlist[3], const swe_params& prm, double t0,
+                       double tfinal, long long max_steps, int* id, int64_t* csr2slot);
 
 // ---- launch wrappers (swe_kernels.hip) --------------------------------------------------------
 Mesh mesh_view(const Pool& P);

@@ -25,7 +25,7 @@
 namespace {
 
 constexpr int kMaxElems = 32;
-constexpr int kMaxRow = 3 * kMaxElems;
+constexpr int kMaxPattern = 3 * kMaxElems;
 
 using swe_p1::ElemGeom;
 using swe_p1::P1Rule;
@@ -98,7 +98,7 @@ __global__ void k_row_count(int np, const int64_t* __restrict__ start, const int
                             const int* __restrict__ jj, int64_t* __restrict__ cnt) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= np) return;
-    int row[kMaxRow];
+    int row[kMaxPattern];
     cnt[i] = row_pattern(i, start, elem, jj, row);
 }
 
@@ -118,7 +118,7 @@ __global__ void k_row_fill(int np, const int64_t* __restrict__ start, const int*
                            double* __restrict__ lumped, double* __restrict__ cx, double* __restrict__ cy) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= np) return;
-    int row[kMaxRow];
+    int row[kMaxPattern];
     const int n = row_pattern(i, start, elem, jj, row);
     const int64_t p0 = ia0[i];
     for (int q = 0; q < n; ++q) {
@@ -316,3 +316,314 @@ extern "C" int swe_assembly_destroy(swe_assembly* a) {
     release(a);
     return SWE_OK;
 }
+
+// =============================================================================================
+// The sliced layout of an assembled mesh, built on the GPU (the same internal numbering and slots
+// as build_layout on the host: rows stably sorted by length inside each 64-row window of the
+// caller's numbering, one 64-row slice per window).
+// =============================================================================================
+namespace {
+
+__global__ void k_lay_len(int np, const int64_t* __restrict__ ia0, int* __restrict__ lenc, int* __restrict__ bad) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    const int64_t l = ia0[i + 1] - ia0[i];
+    lenc[i] = (int)l;
+    if (l > swe::kMaxRow || l < 1) atomicOr(bad, 1);
+}
+
+// One wave per window: each lane's rank among the window's valid rows by (length, position).
+__global__ void k_lay_sort(int np, int nwin, const int* __restrict__ lenc, int* __restrict__ perm) {
+    const int w = (int)(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const int i = w * 64 + lane;
+    const bool valid = w < nwin && i < np;
+    const int li = valid ? lenc[i] : 0x7fffffff;
+    int rank = 0;
+    for (int m = 0; m < 64; ++m) {
+        const int lm = __shfl(li, m, 64);
+        const bool vm = w < nwin && (w * 64 + m) < np;
+        rank += (vm && (lm < li || (lm == li && m < lane))) ? 1 : 0;
+    }
+    if (valid) perm[w * 64 + rank] = i;
+}
+
+__global__ void k_lay_iperm(int np, int np_pad, const int* __restrict__ perm, const int* __restrict__ lenc,
+                            int* __restrict__ iperm, int* __restrict__ lenr) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np_pad) return;
+    if (r < np) {
+        const int i = perm[r];
+        iperm[i] = r;
+        lenr[r] = lenc[i];
+    } else {
+        lenr[r] = 0;
+    }
+}
+
+// Per slice: 64 x its longest row (slots) and its off-diagonal entries (compact-list length).
+__global__ void k_lay_slices(int nsl, int np, const int* __restrict__ lenr, int64_t* __restrict__ slots,
+                             int64_t* __restrict__ ecnt, int* __restrict__ wmax) {
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= nsl) return;
+    int w = 0;
+    int64_t e = 0;
+    for (int l = 0; l < 64; ++l) {
+        const int r = s * 64 + l;
+        const int L = lenr[r];
+        w = (L > w) ? L : w;
+        if (r < np) e += L - 1;
+    }
+    slots[s] = 64 * (int64_t)w;
+    ecnt[s] = e;
+    atomicMax(wmax, w);
+}
+
+struct LayOut {
+    uint8_t* len8;
+    uint8_t* dpos8;
+    int* perm;
+    double* m;
+    double* z;
+    int* col;
+    int16_t* dcol;
+    double2* c;
+};
+
+// Per internal row: its slots (columns, offsets, c_ij) and node data, as build_layout fills them.
+__global__ void k_lay_fill(int np, int np_pad, int64_t r0, int64_t p0, const int* __restrict__ perm,
+                           const int* __restrict__ iperm, const int* __restrict__ lenr,
+                           const int64_t* __restrict__ sptr, const int64_t* __restrict__ ia0,
+                           const int* __restrict__ ja, const double* __restrict__ cx, const double* __restrict__ cy,
+                           const double* __restrict__ lumped, const double* __restrict__ bath, LayOut o,
+                           int64_t* __restrict__ csr2slot) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np_pad) return;
+    if (r >= np) {
+        o.len8[r0 + r] = 0;
+        o.dpos8[r0 + r] = 0;
+        return;
+    }
+    const int i = perm[r];
+    const int L = lenr[r];
+    const int64_t base = sptr[r >> 6] + (r & 63);
+    const int64_t pi = ia0[i];
+    int dp = 0;
+    for (int q = 0; q < L; ++q) {
+        const int64_t p = pi + q;
+        const int j = ja[p] - 1;
+        const int jr = iperm[j];
+        const int64_t slot = base + 64 * (int64_t)q;
+        o.col[p0 + slot] = (int)(r0 + jr);
+        const int off = jr - r;
+        o.dcol[p0 + slot] = (int16_t)((off > swe::kColFar && off <= 32767) ? off : swe::kColFar);
+        o.c[p0 + slot] = make_double2(cx[p], cy[p]);
+        if (csr2slot) csr2slot[p] = slot;
+        if (j == i) dp = q;
+    }
+    o.len8[r0 + r] = (uint8_t)L;
+    o.dpos8[r0 + r] = (uint8_t)dp;
+    o.perm[r0 + r] = i;
+    o.m[r0 + r] = lumped[i];
+    o.z[r0 + r] = bath[i];
+}
+
+// One wave per slice: its compact list of off-diagonal slots, entry by entry, lanes in order.
+__global__ void k_lay_elist(int nsl, int np, int64_t r0, int64_t e0, const int* __restrict__ lenr,
+                            const uint8_t* __restrict__ dpos8, const int64_t* __restrict__ sptr,
+                            const int64_t* __restrict__ eoff, uint16_t* __restrict__ elist) {
+    const int s = (int)(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
+    if (s >= nsl) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int r = s * 64 + lane;
+    const int L = (r < np) ? lenr[r] : 0;
+    const int dp = (r < np) ? (int)dpos8[r0 + r] : 0;
+    const int w = (int)((sptr[s + 1] - sptr[s]) / 64);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t off = eoff[s];
+    for (int q = 0; q < w; ++q) {
+        const bool active = q < L && q != dp;
+        const unsigned long long mask = __ballot(active);
+        if (active) elist[e0 + off + __popcll(mask & below)] = (uint16_t)(q * 64 + lane);
+        off += __popcll(mask);
+    }
+}
+
+__global__ void k_lay_slice_info(int nsl, int64_t s0, int64_t p0, int64_t e0, int inst,
+                                 const int64_t* __restrict__ sptr, const int64_t* __restrict__ eoff,
+                                 swe::SliceInfo* __restrict__ sl) {
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= nsl) return;
+    swe::SliceInfo v;
+    v.sbase = p0 + sptr[s];
+    v.ebase = e0 + eoff[s];
+    v.inst = inst;
+    v.ecount = (int)(eoff[s + 1] - eoff[s]);
+    v.pad_[0] = 0;
+    v.pad_[1] = 0;
+    sl[s0 + s] = v;
+}
+
+__global__ void k_lay_bc(int nbc, int nh, int nux, int64_t r0, int64_t b0, int inst, const int* __restrict__ nodes1,
+                         const int* __restrict__ iperm, swe::BcEntry* __restrict__ bc) {
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= nbc) return;
+    const int comp = (q < nh) ? 0 : (q < nh + nux) ? 1 : 2;
+    bc[b0 + q] = swe::BcEntry{(int)(r0 + iperm[nodes1[q] - 1]), inst, comp, (int)(b0 + q)};
+}
+
+}  // namespace
+
+namespace swe {
+
+void free_device_layout(DeviceLayout& DL) {
+    void* bufs[] = {DL.perm, DL.iperm, DL.len, DL.sptr, DL.eoff};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    DL = DeviceLayout{};
+}
+
+int build_layout_device(const swe_assembly* a, DeviceLayout& DL) {
+    if (!a) return fail(SWE_EINVAL, "null assembly");
+    DL = DeviceLayout{};
+    const int np = a->np;
+    DL.np = np;
+    DL.nnz = a->nnz;
+    const int nsl = (np + 63) / 64, np_pad = nsl * 64;
+    DL.nsl = nsl;
+    hipStream_t s = a->stream;
+    int* lenc = nullptr;
+    int* flag = nullptr;
+    int64_t *slots = nullptr, *ecnt = nullptr;
+    hipError_t e = dalloc(&DL.perm, (size_t)np);
+    if (e == hipSuccess) e = dalloc(&DL.iperm, (size_t)np);
+    if (e == hipSuccess) e = dalloc(&DL.len, (size_t)np_pad);
+    if (e == hipSuccess) e = dalloc(&DL.sptr, (size_t)nsl + 1);
+    if (e == hipSuccess) e = dalloc(&DL.eoff, (size_t)nsl + 1);
+    if (e == hipSuccess) e = dalloc(&lenc, (size_t)np);
+    if (e == hipSuccess) e = dalloc(&flag, 2);
+    if (e == hipSuccess) e = dalloc(&slots, (size_t)nsl + 1);
+    if (e == hipSuccess) e = dalloc(&ecnt, (size_t)nsl + 1);
+    if (e == hipSuccess) e = hipMemsetAsync(flag, 0, 2 * sizeof(int), s);
+    if (e == hipSuccess) e = hipMemsetAsync(slots + nsl, 0, sizeof(int64_t), s);
+    if (e == hipSuccess) e = hipMemsetAsync(ecnt + nsl, 0, sizeof(int64_t), s);
+    int hflag[2] = {0, 0};
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_lay_len, dim3(grid_of(np, 256)), dim3(256), 0, s, np, (const int64_t*)a->ia0, lenc, flag);
+        hipLaunchKernelGGL(k_lay_sort, dim3(grid_of(nsl, 4)), dim3(256), 0, s, np, nsl, (const int*)lenc, DL.perm);
+        hipLaunchKernelGGL(k_lay_iperm, dim3(grid_of(np_pad, 256)), dim3(256), 0, s, np, np_pad, (const int*)DL.perm,
+                           (const int*)lenc, DL.iperm, DL.len);
+        hipLaunchKernelGGL(k_lay_slices, dim3(grid_of(nsl, 256)), dim3(256), 0, s, nsl, np, (const int*)DL.len, slots,
+                           ecnt, flag + 1);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = exclusive_scan(slots, DL.sptr, nsl + 1, s);
+    if (e == hipSuccess) e = exclusive_scan(ecnt, DL.eoff, nsl + 1, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hflag, flag, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&DL.nslot, DL.sptr + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(&DL.ne, DL.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    void* tmp[] = {lenc, flag, slots, ecnt};
+    for (void* b : tmp)
+        if (b) (void)hipFree(b);
+    if (e != hipSuccess) {
+        free_device_layout(DL);
+        return fail(SWE_EHIP, "device layout: %s", hipGetErrorString(e));
+    }
+    if (hflag[0]) {
+        free_device_layout(DL);
+        return fail(SWE_EINVAL, "a row has more than %d entries", kMaxRow);
+    }
+    DL.wmax = hflag[1];
+    return SWE_OK;
+}
+
+int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, const double* bath,
+                       const int* const lists[3], const int nlist[3], const swe_params& prm, double t0,
+                       double tfinal, long long max_steps, int* id, int64_t* csr2slot) {
+    if (!a || !bath || !id) return fail(SWE_EINVAL, "null argument");
+    Mode md;
+    int rc = mode_of(prm, md);
+    if (rc) return rc;
+    if (P->mode_set && !same_mode(md, P->mode)) return fail(SWE_EBADOPT, "this instance's scheme differs from the pool's");
+    const int np = DL.np;
+    for (int c = 0; c < 3; ++c) {
+        if (nlist[c] < 0 || (nlist[c] > 0 && !lists[c])) return fail(SWE_EINVAL, "Dirichlet list %d is malformed", c);
+        for (int q = 0; q < nlist[c]; ++q)
+            if (lists[c][q] < 1 || lists[c][q] > np) return fail(SWE_EINVAL, "Dirichlet node %d out of range", lists[c][q]);
+    }
+    int k = -1;
+    for (int q = 0; q < P->inst_cap; ++q)
+        if (!P->inst[(size_t)q].used) { k = q; break; }
+    const int64_t nbc = (int64_t)nlist[0] + nlist[1] + nlist[2];
+    const int64_t s0 = (k < 0) ? -1 : P->a_slices.alloc(DL.nsl);
+    const int64_t p0 = (s0 < 0) ? -1 : P->a_slots.alloc(DL.nslot);
+    const int64_t e0 = (p0 < 0) ? -1 : P->a_elist.alloc(DL.ne);
+    const int64_t b0 = (e0 < 0) ? -1 : P->a_bc.alloc(nbc);
+    if (k < 0 || s0 < 0 || p0 < 0 || e0 < 0 || b0 < 0) {
+        if (s0 >= 0) P->a_slices.release(s0, DL.nsl);
+        if (p0 >= 0) P->a_slots.release(p0, DL.nslot);
+        if (e0 >= 0) P->a_elist.release(e0, DL.ne);
+        return fail(SWE_EFULL, "the pool has no room for a mesh of %d nodes now", np);
+    }
+    const int64_t r0 = s0 * 64;
+    const int np_pad = DL.nsl * 64;
+    hipStream_t s = P->stream;
+    double* dbath = nullptr;
+    int* dnodes = nullptr;
+    hipError_t e = dalloc(&dbath, (size_t)np);
+    if (e == hipSuccess) e = dalloc(&dnodes, (size_t)nbc);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbath, bath, sizeof(double) * (size_t)np, hipMemcpyHostToDevice, s);
+    for (int c = 0, q = 0; c < 3 && e == hipSuccess; q += nlist[c], ++c)
+        if (nlist[c] > 0)
+            e = hipMemcpyAsync(dnodes + q, lists[c], sizeof(int) * (size_t)nlist[c], hipMemcpyHostToDevice, s);
+    // ranges freed by a removal are rewritten only after the removal's kernels
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, P->released, 0);
+    if (e == hipSuccess) {
+        LayOut o{P->len, P->dpos, P->perm, P->m, P->z, P->col, P->dcol, P->c};
+        hipLaunchKernelGGL(k_lay_fill, dim3(grid_of(np_pad, 256)), dim3(256), 0, s, np, np_pad, r0, p0,
+                           (const int*)DL.perm, (const int*)DL.iperm, (const int*)DL.len, (const int64_t*)DL.sptr,
+                           (const int64_t*)a->ia0, (const int*)a->ja, (const double*)a->cx, (const double*)a->cy,
+                           (const double*)a->lumped, (const double*)dbath, o, csr2slot);
+        hipLaunchKernelGGL(k_lay_elist, dim3(grid_of(DL.nsl, 4)), dim3(256), 0, s, DL.nsl, np, r0, e0,
+                           (const int*)DL.len, (const uint8_t*)P->dpos, (const int64_t*)DL.sptr,
+                           (const int64_t*)DL.eoff, P->elist);
+        hipLaunchKernelGGL(k_lay_slice_info, dim3(grid_of(DL.nsl, 256)), dim3(256), 0, s, DL.nsl, s0, p0, e0, k,
+                           (const int64_t*)DL.sptr, (const int64_t*)DL.eoff, P->sl);
+        if (nbc > 0)
+            hipLaunchKernelGGL(k_lay_bc, dim3(grid_of(nbc, 256)), dim3(256), 0, s, (int)nbc, nlist[0], nlist[1], r0, b0,
+                               k, (const int*)dnodes, (const int*)DL.iperm, P->bc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (dbath) (void)hipFree(dbath);
+    if (dnodes) (void)hipFree(dnodes);
+    if (e != hipSuccess) {
+        P->a_slices.release(s0, DL.nsl);
+        P->a_slots.release(p0, DL.nslot);
+        P->a_elist.release(e0, DL.ne);
+        P->a_bc.release(b0, nbc);
+        return fail(SWE_EHIP, "admission of an assembled mesh: %s", hipGetErrorString(e));
+    }
+    InstHost I;
+    I.np = np;
+    I.nnz = (int)DL.nnz;
+    I.np_pad = np_pad;
+    I.wmax = DL.wmax;
+    I.slice0 = s0;
+    I.nslices = DL.nsl;
+    I.slot0 = p0;
+    I.nslots = DL.nslot;
+    I.e0 = e0;
+    I.ne = DL.ne;
+    I.bc0 = b0;
+    I.nbc = nbc;
+    I.nh = nlist[0];
+    I.nux = nlist[1];
+    I.nuy = nlist[2];
+    if ((rc = pool_activate(P, k, I, prm, false, t0, tfinal, max_steps))) return rc;
+    *id = k;
+    return SWE_OK;
+}
+
+}  // namespace swe

@@ -57,6 +57,34 @@ MODULE swe_amd_c
        INTEGER(c_int) :: rc
      END FUNCTION swe_create
 
+     FUNCTION swe_assemble_device(np, me, jj, rr, assembly, nnz) BIND(C, NAME='swe_assemble_device') RESULT(rc)
+       IMPORT :: c_int, c_double, c_ptr, c_long_long
+       INTEGER(c_int), VALUE :: np, me
+       INTEGER(c_int), INTENT(IN) :: jj(*)
+       REAL(c_double), INTENT(IN) :: rr(*)
+       TYPE(c_ptr) :: assembly
+       INTEGER(c_long_long) :: nnz
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_assemble_device
+
+     FUNCTION swe_assembly_destroy(assembly) BIND(C, NAME='swe_assembly_destroy') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: assembly
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_assembly_destroy
+
+     FUNCTION swe_create_from_assembly(assembly, bath, n_h_dir, h_dir, n_ux_dir, ux_dir, n_uy_dir, uy_dir, &
+          prm, handle) BIND(C, NAME='swe_create_from_assembly') RESULT(rc)
+       IMPORT :: c_int, c_double, c_ptr, swe_params
+       TYPE(c_ptr), VALUE :: assembly
+       REAL(c_double), INTENT(IN) :: bath(*)
+       INTEGER(c_int), VALUE :: n_h_dir, n_ux_dir, n_uy_dir
+       INTEGER(c_int), INTENT(IN) :: h_dir(*), ux_dir(*), uy_dir(*)
+       TYPE(swe_params), INTENT(IN) :: prm
+       TYPE(c_ptr) :: handle
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_create_from_assembly
+
      FUNCTION swe_destroy(handle) BIND(C, NAME='swe_destroy') RESULT(rc)
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: handle

@@ -54,11 +54,16 @@ class Problem:
         return (M.dirichlet_nodes(self.mesh, self.h_sides), M.dirichlet_nodes(self.mesh, self.ux_sides),
                 M.dirichlet_nodes(self.mesh, self.uy_sides))
 
-    def solver(self, **overrides) -> Solver:
+    def solver(self, device_setup=False, **overrides) -> Solver:
+        """A solver for this case; device_setup=True assembles and lays the mesh out on the GPU
+        (Solver.from_elements) instead of taking the host assembly."""
         prm = dict(self.params)
         prm.update(overrides)
         hd, uxd, uyd = self.dirichlet()
-        s = Solver(self.ia, self.ja, self.cij, self.lumped, self.bath, h_dir=hd, ux_dir=uxd, uy_dir=uyd, **prm)
+        if device_setup:
+            s = Solver.from_elements(self.mesh.jj, self.mesh.rr, self.bath, h_dir=hd, ux_dir=uxd, uy_dir=uyd, **prm)
+        else:
+            s = Solver(self.ia, self.ja, self.cij, self.lumped, self.bath, h_dir=hd, ux_dir=uxd, uy_dir=uyd, **prm)
         s.set_bc(*self.bc)
         return s
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "swe_assembly_get": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "swe_assembly_destroy": (_I, [_P]),
     "swe_create": (_I, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(_P)]),
+    "swe_create_from_assembly": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, C.POINTER(Params), C.POINTER(_P)]),
     "swe_destroy": (_I, [_P]),
     "swe_set_params": (_I, [_P, C.POINTER(Params)]),
     "swe_set_bc": (_I, [_P, _I, _D, _D, _D]),
@@ -256,6 +257,37 @@ class Solver:
         h = C.c_void_p()
         _check(lib.swe_create(C.byref(self.mesh.desc), C.byref(self.params), C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_elements(cls, jj, rr, bath, *, h_dir=(), ux_dir=(), uy_dir=(), **params):
+        """Set-up and solver on the GPU: construct_matrices by swe_assemble_device, then the solver
+        built from the device assembly (swe_create_from_assembly); the matrices stay on the device.
+        jj (me, 3) and the Dirichlet lists are 0-based; `params` are those of the constructor."""
+        lib = load_library()
+        jj1 = np.ascontiguousarray(jj, dtype=np.int32) + 1
+        rr = np.ascontiguousarray(rr, dtype=np.float64)
+        bath = np.ascontiguousarray(bath, dtype=np.float64)
+        if bath.shape != (rr.shape[0],):
+            raise SweError(SWE_EINVAL, "bath must have shape (%d,)" % rr.shape[0])
+        lists = [np.ascontiguousarray(x, np.int32) + 1 for x in (h_dir, ux_dir, uy_dir)]
+        self = cls.__new__(cls)
+        self.h = None
+        self.mesh = None
+        self.params = make_params(**params)
+        a = C.c_void_p()
+        n = C.c_longlong()
+        _check(lib.swe_assemble_device(rr.shape[0], jj1.shape[0], _ptr(jj1), _ptr(rr), C.byref(a), C.byref(n)))
+        try:
+            h = C.c_void_p()
+            _check(lib.swe_create_from_assembly(a, _ptr(bath), lists[0].size, _ptr(lists[0]), lists[1].size,
+                                                _ptr(lists[1]), lists[2].size, _ptr(lists[2]),
+                                                C.byref(self.params), C.byref(h)))
+        finally:
+            lib.swe_assembly_destroy(a)
+        self.h = h
+        self.np = rr.shape[0]
+        self.nnz = n.value
+        return self
 
     # -- lifetime -------------------------------------------------------------------------------
     def close(self):

@@ -46,3 +46,61 @@ def test_device_assembly_rejects_large_fans_and_bad_input():
         assemble(jj, rr, device=True)
     with pytest.raises(SweError):
         assemble(np.array([[0, 1, 7]]), np.zeros((3, 2)), device=True)
+
+
+
+def _scrambled(p, seed):
+    """The case `p` with its nodes renumbered at random (neighbours far apart in the numbering)."""
+    q = np.random.default_rng(seed).permutation(p.np)      # new number of each node
+    inv = np.argsort(q)
+    lists = [q[np.asarray(x, np.int64)] for x in p.dirichlet()]
+    return q[p.mesh.jj], p.mesh.rr[inv], p.bath[inv], p.u0[inv], lists
+
+
+@pytest.mark.parametrize("case", ["paraboloid", "dam_break", "bumps", "bumps_scrambled"])
+def test_solver_from_device_assembly_steps_like_host_solver(case):
+    """swe_create_from_assembly: the layout built on the GPU from the device assembly gives every
+    bit of the states and time steps of swe_create on the host assembly (inflow, walls, friction,
+    FCT with the alpha limiter, first-order viscosity; a scrambled numbering with far columns)."""
+    from swe_amd import problems as PR
+    from swe_amd.solver import Solver, assemble
+    n = 20
+    if case == "paraboloid":
+        p = PR.paraboloid(n=41)
+    elif case == "dam_break":
+        p = PR.dam_break(nx=201, ny=41)
+    elif case == "bumps":
+        p = PR.bumps(n_side=129)
+    else:
+        p = PR.bumps(n_side=257)        # 66,049 nodes: scrambled columns beyond 16-bit offsets
+    print(case, p.np, flush=True)
+    if case == "bumps_scrambled":
+        jj, rr, bath, u0, lists = _scrambled(p, 5)
+        ia, ja, _diag, _mass, lumped, cij = assemble(jj, rr)
+        kw = dict(h_dir=lists[0], ux_dir=lists[1], uy_dir=lists[2], **p.params)
+        host = Solver(ia, ja, cij, lumped, bath, **kw)
+        dev = Solver.from_elements(jj, rr, bath, **kw)
+        for s in (host, dev):
+            s.set_bc(*p.bc)
+    else:
+        u0 = p.u0
+        host = p.solver()
+        dev = p.solver(device_setup=True)
+    out = []
+    for s in (host, dev):
+        s.set_state(u0)
+        t, dts = s.run(n, time=0.0)
+        out.append((s.get_state(), t, dts))
+        s.close()
+    (a, ta, da), (b, tb, db) = out
+    assert same_bits(a, b) and same_bits(da, db) and ta == tb
+
+
+def test_solver_from_device_assembly_rejects_bad_input():
+    from swe_amd import problems as PR
+    from swe_amd.solver import Solver, SweError
+    p = PR.paraboloid(n=11)
+    with pytest.raises(SweError):
+        Solver.from_elements(p.mesh.jj, p.mesh.rr, p.bath, h_dir=[p.np], **p.params)    # node out of range
+    with pytest.raises(SweError):
+        Solver.from_elements(p.mesh.jj, p.mesh.rr, p.bath[:-1], **p.params)             # bath too short
