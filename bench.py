@@ -337,13 +337,65 @@ def bench_setup(n_side=4097, reps=2):
             "solver_host_s": res[0][0], "solver_device_s": res[1][0], "solver_runs_identical": same_run}
 
 
+def bench_errors(n=4097, reps=5):
+    """compute_errors (the analytic-error check of the paraboloid, test 5) on an n x n grid of
+    [0,4]^2 after 2 steps: the device evaluator (kernel plus the 96-byte read-back) against the
+    oracle's sequential sums on one host core; the two agree within the sequential sums' own
+    rounding bound (the device sums are the exactly rounded ones)."""
+    from swe_amd import problems as PR
+    from swe_amd import mesh as M
+    from swe_amd.solver import Solver
+    t0 = time.perf_counter()
+    m = M.structured_mesh(n, n, 0.0, 4.0, 0.0, 4.0)
+    g, h0, a, hL = 9.81, 0.1, 1.0, 4.0
+    htiny = 1.0e-4 * hL
+    bath = -h0 * (1.0 - ((m.rr[:, 0] - hL / 2.0) ** 2 + (m.rr[:, 1] - hL / 2.0) ** 2) / a ** 2)
+    u0 = PR.paraboloid_state(m.rr, 0.0, htiny)
+    print("# errors: grid of %d nodes, %d elements in %.1f s" % (m.np, m.jj.shape[0], time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    s = Solver.from_elements(m.jj, m.rr, bath, gravity=g, htiny=htiny, cfl=0.25, max_water_h=hL,
+                             viscosity="viscous", type_test=5)
+    s.set_state(u0)
+    s.run(2, time=0.0)
+    ev = s.error_evaluator(m.jj, m.rr)
+    ev.sums()
+    ts = []
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        dev = ev.sums()
+        ts.append(time.perf_counter() - t1)
+    u, t = s.get_state(), s.time
+    rel = ev.relative()
+    ev.close()
+    s.close()
+    res = {"nodes": int(m.np), "elements": int(m.jj.shape[0]), "device_s": min(ts), "l1_h": rel["l1_h"],
+           "l2_h": rel["l2_h"]}
+    try:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import pyoracle
+        t1 = time.perf_counter()
+        ref = pyoracle.compute_errors(m.jj, m.rr, u, 5, t, g, htiny)
+        res["oracle_s"] = time.perf_counter() - t1
+        res["max_rel_diff"] = float(np.max(np.abs(dev - ref) / np.maximum(np.abs(ref), 1e-300)))
+        # the summands are >= 0, so the sequential sums are within (n-1)u/(1-(n-1)u) of the exact
+        # ones, relative (half that after the square root of the L2 sums)
+        nq = 3 * m.jj.shape[0]
+        res["sequential_rounding_bound"] = (nq - 1) * 2.0 ** -53 / (1 - (nq - 1) * 2.0 ** -53)
+        res["within_bound"] = bool(res["max_rel_diff"] <= res["sequential_rounding_bound"])
+    except (ImportError, OSError) as e:
+        res["oracle"] = "not available: %s" % e
+    print("# errors: device %.4f s; %s" % (res["device_s"], {k: v for k, v in res.items() if k != "device_s"}),
+          file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="bumps",
-                    choices=["bumps", "dam_break", "paraboloid", "batch", "serve", "setup"])
+                    choices=["bumps", "dam_break", "paraboloid", "batch", "serve", "setup", "errors"])
     ap.add_argument("--instances", type=int, default=256, help="batch: meshes in the batch")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
     ap.add_argument("--rates", default="20", help="serve: arrival rates (per s), comma separated")
@@ -378,6 +430,12 @@ def main():
                     "ms_per_step": r["ms_per_step"], "roofline": r.pop("roofline"),
                     "config": {"workload": "batch", **r}})
         print(json.dumps(res))
+        return
+    if args.workload == "errors":
+        r = bench_errors()
+        print(json.dumps({"metric": "compute_errors time (analytic L1/L2 check), device", "value": r["device_s"],
+                          "unit": "s", "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": "errors", **r}}), flush=True)
         return
     if args.workload == "setup":
         r = bench_setup()

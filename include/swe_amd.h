@@ -117,6 +117,20 @@ int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **ou
 int swe_create_from_assembly(const swe_assembly *a, const double *bath, int n_h_dir, const int *h_dir,
                              int n_ux_dir, const int *ux_dir, int n_uy_dir, const int *uy_dir,
                              const swe_params *prm, swe_handle **out);
+/* compute_errors (SOURCES/main.f90:198-263) on the device state of a solver: sol_anal
+ * (SOURCES/condlim.f90:311-831) at the Gauss points of r_gauss (main.f90:303-322) and the sums of
+ * ns_anal_l1 / ns_anal_0 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1679-1716, 1513-1553) for the analytic
+ * tests 4, 5, 11 and 12 (the solver's type_test; others give SWE_EBADOPT).  swe_errors_create takes
+ * the element list jj(3,me) (1-based, the solver's node numbering) and rr(2,np); the handle must
+ * outlive the evaluator.  swe_compute_errors writes out[12]: for k = h, qx, qy in turn the L1 error
+ * of u(k,:) - exact, its L1 norm (0 - exact), the L2 error and the L2 norm.  Each sum is accumulated
+ * in double-double over a fixed partition (the exactly rounded sum, the same on every run), not
+ * sequentially as the reference does: the two differ by at most the sequential sum's rounding
+ * error.  The reference prints err/norm of h (main.f90:214,221). */
+typedef struct swe_errors swe_errors;
+int swe_errors_create(swe_handle *h, int me, const int *jj, const double *rr, swe_errors **out);
+int swe_compute_errors(swe_errors *e, double time, double *out);
+int swe_errors_destroy(swe_errors *e);
 int swe_destroy(swe_handle *h);
 int swe_set_params(swe_handle *h, const swe_params *prm);
 

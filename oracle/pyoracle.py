@@ -58,6 +58,12 @@ def lib():
         L.orc_compute_dt.argtypes = [P, P, P]
         L.orc_euler.restype = C.c_int
         L.orc_euler.argtypes = [P, P, P, C.c_double, P]
+        L.orc_compute_errors.restype = C.c_int
+        L.orc_compute_errors.argtypes = [C.c_int, C.c_int, P, P, P, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         P, P]
+        L.orc_sol_anal.restype = C.c_double
+        L.orc_sol_anal.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(C.c_int)]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [P, P, C.c_int, P, P, C.c_int, C.c_double, C.c_double, C.c_double,
                               C.c_int, P, C.c_int, P, C.c_int, P]
@@ -101,6 +107,32 @@ def fct_generic(ia, ja, f, unext, maxn, minn, dg):
     args = [np.ascontiguousarray(x, np.float64) for x in (f, maxn, minn, dg)]
     lib().orc_fct_generic(ia.size - 1, _p(ia), _p(ja), _p(args[0]), _p(args[1]), _p(args[2]), _p(args[3]), _p(u))
     return u
+
+
+def compute_errors(jj0, rr, u, type_test, t, gravity, htiny, terms=False):
+    """compute_errors' quadrature norms (main.f90:198-263): array (3, 4) of [L1 error, L1 norm,
+    L2 error, L2 norm] per component (h, qx, qy); with terms=True also every summand, (3, 4, 3*me)."""
+    jj0 = np.ascontiguousarray(jj0, dtype=np.int32)
+    rr = np.ascontiguousarray(rr, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    me = jj0.shape[0]
+    out = np.zeros(12)
+    tq = np.zeros(12 * 3 * me) if terms else None
+    rc = lib().orc_compute_errors(rr.shape[0], me, _p(jj0), _p(rr), _p(u), int(type_test), float(t), float(gravity),
+                                  float(htiny), _p(out), _p(tq) if terms else None)
+    if rc != 0:
+        raise ValueError("compute_errors: no analytic solution for test %d" % type_test)
+    out = out.reshape(3, 4)
+    return (out, tq.reshape(3, 4, 3 * me)) if terms else out
+
+
+def sol_anal(type_test, k, x, y, t, gravity, htiny):
+    ok = C.c_int(0)
+    v = lib().orc_sol_anal(int(type_test), int(k), float(x), float(y), float(t), float(gravity), float(htiny),
+                           C.byref(ok))
+    if not ok.value:
+        raise ValueError("sol_anal: test %d not restated" % type_test)
+    return v
 
 
 def fct_positivity(ia, ja, f, unext_scaled, htiny):

@@ -21,6 +21,12 @@
 !       nsteps SSP-RK3 steps as in the time loop of SOURCES/main.f90:61-92 (fixed step count
 !       instead of the Tfinal loop); writes dt per step, U after each stage of step 1, the final
 !       U and optional snapshots.
+!   ref_driver errors <data> <init|none> <state> <time> <out>
+!       the quadrature norms of compute_errors (SOURCES/main.f90:198-263) at a given state and
+!       time: for each component k = 1..3 of sol_anal (SOURCES/condlim.f90:311-831) at the
+!       Gauss points, ns_anal_l1 and ns_anal_0 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1513-1553,
+!       1679-1716) of U(k,:) - exact and of 0 - exact; writes 12 reals (k-major: L1 error, L1 norm,
+!       L2 error, L2 norm).
 !
 ! <init> is an optional custom initial-data file for problems that have no init case in the
 ! reference (see custom_init below); 'none' calls the reference init (SOURCES/condlim.f90:92).
@@ -33,6 +39,7 @@ PROGRAM ref_driver
   USE boundary_conditions
   USE fct
   USE matrix_type
+  USE fem_tn
   USE iso_c_binding, ONLY : c_double
   IMPLICIT NONE
   INTERFACE
@@ -62,6 +69,8 @@ PROGRAM ref_driver
      CALL do_euler
   CASE ('run')
      CALL do_run
+  CASE ('errors')
+     CALL do_errors
   CASE DEFAULT
      WRITE(*,*) 'ref_driver: unknown command ', TRIM(cmd)
      STOP 2
@@ -316,6 +325,47 @@ CONTAINS
     CALL write_state(TRIM(od)//'/unext.f8', unext)
     CALL write_real(TRIM(od)//'/dij.f8', dij%aa)
   END SUBROUTINE do_euler
+
+  !---------------------------------------------------------------------------------------------
+  ! The sums of compute_errors (SOURCES/main.f90:198-263).  The Gauss point coordinates are
+  ! r_gauss's (SOURCES/main.f90:303-322, an internal procedure of the main program): the same
+  ! element / Gauss point / coordinate loops and the same sum, written out here.
+  SUBROUTINE do_errors
+    CHARACTER(LEN=512) :: s
+    REAL(KIND=8) :: t, r
+    REAL(KIND=8), DIMENSION(:,:), ALLOCATABLE :: rg
+    REAL(KIND=8), DIMENSION(:), ALLOCATABLE :: uex, zero
+    REAL(KIND=8), DIMENSION(12) :: out
+    INTEGER :: idx, k, l, m, n
+    CALL prepare(arg(2), arg(3))
+    CALL read_state(arg(4), un)
+    s = arg(5)
+    READ(s, *) t
+    inputs%time = t
+    ALLOCATE(rg(k_dim, mesh%gauss%l_G*mesh%me), uex(mesh%gauss%l_G*mesh%me), zero(mesh%np))
+    idx = 0
+    DO m = 1, mesh%me
+       DO l = 1, mesh%gauss%l_G
+          idx = idx + 1
+          DO k = 1, mesh%gauss%k_d
+             r = 0
+             DO n = 1, mesh%gauss%n_w
+                r = r + mesh%gauss%ww(n,l)*mesh%rr(k,mesh%jj(n,m))
+             END DO
+             rg(k,idx) = r
+          END DO
+       END DO
+    END DO
+    zero = 0.d0
+    DO k = 1, 3
+       uex = sol_anal(k, rg, inputs%time)
+       CALL ns_anal_l1(mesh, un(k,:), uex, out(4*k-3))
+       CALL ns_anal_l1(mesh, zero, uex, out(4*k-2))
+       CALL ns_anal_0(mesh, un(k,:), uex, out(4*k-1))
+       CALL ns_anal_0(mesh, zero, uex, out(4*k))
+    END DO
+    CALL write_real(arg(6), out)
+  END SUBROUTINE do_errors
 
   !---------------------------------------------------------------------------------------------
   ! Fixed-step form of the SSP-RK3 loop of SOURCES/main.f90:61-92.

@@ -9,6 +9,10 @@
  *            SOURCES/condlim.f90:21-90,425-486,879-981, SOURCES/main.f90:61-92,166-188
  *   limiter  SOURCES/SAFE/fct.f90:3-43 (FCT_generic, the two-sided limiter the reference keeps but
  *            does not call; pinned by its invariants, see tests/test_oracle_math.py)
+ *   errors   SOURCES/main.f90:198-263,303-322 (compute_errors, r_gauss), SOURCES/condlim.f90:385-456,
+ *            649-750 (sol_anal of tests 4, 5, 11, 12), SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1513-1553,
+ *            1679-1716 (ns_anal_0, ns_anal_l1); pinned by tests/golden/errors.npz (the reference's
+ *            own routines run by oracle/ref_driver.f90)
  *
  * Fortran's MAX(a,b) is evaluated as (a > b ? a : b) and MIN(a,b) as (a < b ? a : b), which is
  * how flang lowers them; SUM over an array section is a sequential left-to-right loop from 0.
@@ -1049,4 +1053,150 @@ int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
     free(uo);
     free(ui);
     return err;
+}
+
+/* ============================================================================================ */
+/* compute_errors                                                                                 */
+/* ============================================================================================ */
+
+/* sol_anal (condlim.f90:311-831) at one point, in the reference's evaluation order: Fortran's
+ * unary minus applies to the whole product that follows it, X**2 is X*X, and the integer powers of
+ * the test constants are exact. */
+double orc_sol_anal(int type_test, int k, double x, double y, double t, double gravity, double htiny, int *ok)
+{
+    *ok = 1;
+    if (type_test == 4) {            /* condlim.f90:385-428 */
+        const double x0 = 5.0, hL = 0.005;
+        const double speed = sqrt(gravity * hL);
+        if (k == 1) {
+            if (t <= 1.0e-14) return (x <= x0) ? hL : 0.0;
+            const double xx = x - x0;
+            if (xx <= -(t * speed)) return hL;
+            if (xx <= 2.0 * t * speed) {
+                const double w = speed - xx / (2.0 * t);
+                return (4.0 / (9.0 * gravity)) * (w * w);
+            }
+            return 0.0 + htiny;
+        }
+        if (k == 2) {
+            if (t <= 1.0e-14) return 0.0;
+            const double xx = x - x0;
+            if (xx <= -(t * speed)) return 0.0;
+            if (xx <= 2.0 * t * speed) {
+                const double w = speed - xx / (2.0 * t);
+                return (8.0 / (27.0 * gravity)) * (speed + xx / t) * (w * w);
+            }
+            return 0.0;
+        }
+        return 0.0;
+    }
+    if (type_test == 5) {            /* condlim.f90:429-456 */
+        const double a = 1.0, h0 = 0.1, hL = 4.0, eta = 0.5;
+        const double omega = sqrt(2.0 * gravity * h0) / a;
+        const double dx = x - hL / 2.0, dy = y - hL / 2.0;
+        const double bathi = -(h0 * (1.0 - (dx * dx + dy * dy) / (a * a)));
+        double vv = (eta * h0 / (a * a)) * (2.0 * dx * cos(omega * t) + 2.0 * dy * sin(omega * t)) - bathi;
+        vv = fmaxF(vv, 0.0);
+        if (k == 1) return vv + htiny;
+        if (k == 2) return -(eta * omega * sin(omega * t) * vv);
+        return eta * omega * cos(omega * t) * vv;
+    }
+    if (type_test == 11) {           /* condlim.f90:649-700; the case sets gravity to 9.81 */
+        const double g = 9.81, a = 1.0, h0 = 0.1, hL = 4.0, kappa = 0.2, gamma = 0.5;
+        const double p = sqrt((8.0 * g * h0) / (a * a));
+        const double s = 1.0 / 2.0 * sqrt(p * p - kappa * kappa);
+        const double dx = x - hL / 2.0, dy = y - hL / 2.0;
+        const double bathi = -(h0 * (1.0 - (dx * dx + dy * dy) / (a * a)));
+        const double htilde = -bathi - (gamma * gamma) / (2.0 * g) * exp(-(kappa * t))
+                              - gamma / (2.0 * g) * exp(-(kappa * t / 2.0)) * dx * (2.0 * s * cos(s * t) + kappa * sin(s * t))
+                              - gamma / (2.0 * g) * exp(-(kappa * t / 2.0)) * dy * (kappa * cos(s * t) - 2.0 * s * sin(s * t));
+        const double vv = fmaxF(htilde, 0.0);
+        if (k == 1) return vv;
+        if (k == 2) return vv * gamma * exp(-(kappa * t / 2.0)) * sin(s * t);
+        return vv * gamma * exp(-(kappa * t / 2.0)) * cos(s * t);
+    }
+    if (type_test == 12) {           /* condlim.f90:701-750; the case sets gravity to 9.81 */
+        const double g = 9.81, a = 3.0, h0 = 0.5, hL = 0.0, f = 2.0, eta = 0.5;
+        const double a2 = a * a, a4 = a2 * a2, a6 = a4 * a2, a8 = a4 * a4, f2 = f * f, f4 = f2 * f2;
+        const double den = f2 * a4 + sqrt(8.0 * f2 * g * h0 * a6 + f4 * a8);
+        const double delta = 2.0 * f * a4 / den;
+        const double w = -(1.0 / (sqrt(2.0) * a2) * sqrt(4.0 * g * h0 * a2 + den));
+        const double dx = x - hL / 2.0, dy = y - hL / 2.0;
+        const double bathi = -(h0 * (1.0 - (dx * dx + dy * dy) / a2));
+        const double c = cos(w * t), sn = sin(w * t);
+        const double htilde = -bathi - eta / g * c * (1.0 - delta * f) * dy
+                              + eta / g * sn * (f / w - delta * w) * dx
+                              + eta * eta / (2.0 * g) * (delta * delta * c * c + sn * sn / (w * w));
+        const double vv = fmaxF(htilde, 0.0);
+        if (k == 1) return vv;
+        if (k == 2) return -(vv * delta * eta * c);
+        return vv * eta * sn / w;
+    }
+    *ok = 0;
+    return 0.0;
+}
+
+int orc_compute_errors(int np, int me, const int *jj, const double *rr, const double *u, int type_test,
+                       double t, double gravity, double htiny, double *out, double *terms)
+{
+    (void)np;
+    int ok;
+    orc_sol_anal(type_test, 1, 0.0, 0.0, t, gravity, htiny, &ok);
+    if (!ok) return ORC_EBADOPT;
+    /* the 3-point rule of element_2d (gauss_points_2d_p1.f90:347-393) and its geometry (106-132) */
+    const double one = 1, two = 2, three = 3, six = 6;
+    const double xx[3] = {one / six, two / three, one / six};
+    const double yy[3] = {one / six, one / six, two / three};
+    double ww[3][3];
+    const double dd[2][3] = {{-one, one, 0.0}, {-one, 0.0, one}};
+    const double pp = one / six;
+    for (int l = 0; l < 3; l++) {
+        ww[0][l] = one - xx[l] - yy[l];
+        ww[1][l] = xx[l];
+        ww[2][l] = yy[l];
+    }
+    const long nq = 3L * me;
+    double sum[12];
+    for (int q = 0; q < 12; q++) sum[q] = 0.0;
+    for (int m = 0; m < me; m++) {
+        const int *nd = jj + 3 * m;
+        for (int l = 0; l < 3; l++) {
+            double dr[2][2];
+            for (int k = 0; k < 2; k++)
+                for (int k1 = 0; k1 < 2; k1++) {
+                    double s = 0.0;
+                    for (int n = 0; n < 3; n++) s = s + rr[2 * nd[n] + k] * dd[k1][n];
+                    dr[k][k1] = s;
+                }
+            const double rj = (dr[0][0] * dr[1][1] - dr[0][1] * dr[1][0]) * pp;
+            /* r_gauss: rr_gauss(k,index) = SUM over n of ww(n,l)*rr(k,jj(n,m)), from 0 */
+            double rg[2];
+            for (int k = 0; k < 2; k++) {
+                double s = 0.0;
+                for (int n = 0; n < 3; n++) s = s + ww[n][l] * rr[2 * nd[n] + k];
+                rg[k] = s;
+            }
+            for (int k = 0; k < 3; k++) {
+                const double fa = orc_sol_anal(type_test, k + 1, rg[0], rg[1], t, gravity, htiny, &ok);
+                double fl = 0.0, f0 = 0.0;
+                for (int n = 0; n < 3; n++) {
+                    fl = fl + u[3 * nd[n] + k] * ww[n][l];
+                    f0 = f0 + 0.0 * ww[n][l];
+                }
+                const double d = fl - fa, d0 = f0 - fa;
+                const double tq[4] = {fabs(d) * rj, fabs(d0) * rj, d * d * rj, d0 * d0 * rj};
+                for (int q = 0; q < 4; q++) {
+                    sum[4 * k + q] = sum[4 * k + q] + tq[q];
+                    if (terms) terms[(4 * k + q) * nq + 3L * m + l] = tq[q];
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        out[4 * k + 0] = sum[4 * k + 0];
+        out[4 * k + 1] = sum[4 * k + 1];
+        out[4 * k + 2] = sqrt(sum[4 * k + 2]);
+        out[4 * k + 3] = sqrt(sum[4 * k + 3]);
+    }
+    return ORC_OK;
 }

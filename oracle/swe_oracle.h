@@ -68,6 +68,19 @@ int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
 /* sin(x) rounded to nearest (double-double evaluation, one final rounding), |x| < 2^29. */
 double orc_sin_rn(double x);
 
+/* sol_anal(k, (x, y), t) (SOURCES/condlim.f90:311-831) for the analytic tests 4, 5, 11 and 12,
+ * components k = 1 (h), 2 (qx), 3 (qy); *ok = 0 for other tests. */
+double orc_sol_anal(int type_test, int k, double x, double y, double t, double gravity, double htiny, int *ok);
+
+/* The quadrature norms of compute_errors (SOURCES/main.f90:198-263): r_gauss (303-322),
+ * sol_anal at the Gauss points, ns_anal_l1 and ns_anal_0 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:
+ * 1679-1716, 1513-1553) of u(k,:) - exact and of 0 - exact, sequential sums in element and Gauss
+ * point order.  jj is 0-based jj[3*m+n]; out[4*(k-1) + {0,1,2,3}] = L1 error, L1 norm, L2 error,
+ * L2 norm.  terms (may be NULL) receives every summand: terms[(4*(k-1)+q)*3*me + 3*m + l]
+ * (before the square root of the L2 sums).  Returns ORC_EBADOPT for tests without sol_anal here. */
+int orc_compute_errors(int np, int me, const int *jj, const double *rr, const double *u, int type_test,
+                       double t, double gravity, double htiny, double *out, double *terms);
+
 #ifdef __cplusplus
 }
 #endif

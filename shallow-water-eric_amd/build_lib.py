@@ -17,7 +17,7 @@ REPO = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libswe_amd.so")
 CSRC = os.path.join(PKG, "csrc")
 SOURCES = [os.path.join(CSRC, f) for f in ("swe_kernels.hip", "swe_engine.cpp", "swe_capi.cpp", "swe_setup.cpp",
-                                                              "swe_setup_device.hip")]
+                                                              "swe_setup_device.hip", "swe_errors.hip")]
 HEADERS = [os.path.join(REPO, "include", "swe_amd.h")] + [os.path.join(CSRC, f) for f in
                                                           ("swe_kernels.h", "swe_layout.h", "swe_engine.h", "swe_p1.h")]
 

@@ -34,6 +34,11 @@ struct swe_batch {
     Pool* P = nullptr;
 };
 
+struct swe_errors {
+    swe_handle* h = nullptr;
+    swe::ErrEval* E = nullptr;
+};
+
 namespace {
 
 constexpr int kId = 0;   // the handle's instance
@@ -239,6 +244,33 @@ int swe_create_from_assembly(const swe_assembly* a, const double* bath, int n_h_
         return rc;
     }
     *out = h;
+    return SWE_OK;
+}
+
+int swe_errors_create(swe_handle* h, int me, const int* jj, const double* rr, swe_errors** out) {
+    clear_error();
+    if (!h || !jj || !rr || !out) return fail(SWE_EINVAL, "null argument");
+    *out = nullptr;
+    swe::ErrEval* E = nullptr;
+    const int rc = errors_create(h->P, kId, me, jj, rr, &E);
+    if (rc) return rc;
+    *out = new swe_errors{h, E};
+    return SWE_OK;
+}
+
+int swe_compute_errors(swe_errors* e, double time, double* out) {
+    if (!e || !out) return fail(SWE_EINVAL, "null argument");
+    swe_handle* h = e->h;
+    if (!h->state_set) return fail(SWE_EINVAL, "no state has been set");
+    if (h->step_open) return fail(SWE_EINVAL, "a step is open (swe_step_begin without swe_step_finish)");
+    const swe_params& prm = h->P->inst[(size_t)kId].prm;
+    return errors_eval(h->P, e->E, h->P->S[0], prm.type_test, time, prm.gravity, prm.htiny, out);
+}
+
+int swe_errors_destroy(swe_errors* e) {
+    if (!e) return SWE_OK;
+    errors_destroy(e->E);
+    delete e;
     return SWE_OK;
 }
 

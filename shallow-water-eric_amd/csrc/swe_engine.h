@@ -224,6 +224,13 @@ int build_layout_device(const swe_assembly* a, DeviceLayout& DL);
 void free_device_layout(DeviceLayout& DL);
 // Admit an assembled mesh with its device layout; bath(np) and the 1-based Dirichlet lists are host
 // arrays; csr2slot (may be null) receives the slot of each CSR entry, relative to the instance.
+// compute_errors on the device (swe_errors.hip): element list and coordinates of one instance
+struct ErrEval;
+int errors_create(Pool* P, int k, int me, const int* jj1, const double* rr, ErrEval** out);
+int errors_eval(Pool* P, const ErrEval* E, const Node* U, int type_test, double t, double g, double htiny,
+                double out[12]);
+void errors_destroy(ErrEval* E);
+
 int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, const double* bath,
                        const int* const lists[3], const int nlist[3], const swe_params& prm, double t0,
                        double tfinal, long long max_steps, int* id, int64_t* csr2slot);

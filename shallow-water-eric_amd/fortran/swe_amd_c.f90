@@ -85,6 +85,30 @@ MODULE swe_amd_c
        INTEGER(c_int) :: rc
      END FUNCTION swe_create_from_assembly
 
+     FUNCTION swe_errors_create(handle, me, jj, rr, errors) BIND(C, NAME='swe_errors_create') RESULT(rc)
+       IMPORT :: c_int, c_double, c_ptr
+       TYPE(c_ptr), VALUE :: handle
+       INTEGER(c_int), VALUE :: me
+       INTEGER(c_int), INTENT(IN) :: jj(*)
+       REAL(c_double), INTENT(IN) :: rr(*)
+       TYPE(c_ptr) :: errors
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_errors_create
+
+     FUNCTION swe_compute_errors(errors, time, out) BIND(C, NAME='swe_compute_errors') RESULT(rc)
+       IMPORT :: c_int, c_double, c_ptr
+       TYPE(c_ptr), VALUE :: errors
+       REAL(c_double), VALUE :: time
+       REAL(c_double), INTENT(OUT) :: out(12)
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_compute_errors
+
+     FUNCTION swe_errors_destroy(errors) BIND(C, NAME='swe_errors_destroy') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: errors
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_errors_destroy
+
      FUNCTION swe_destroy(handle) BIND(C, NAME='swe_destroy') RESULT(rc)
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: handle

@@ -64,6 +64,9 @@ SIGNATURES = {
     "swe_create": (_I, [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(_P)]),
     "swe_create_from_assembly": (_I, [_P, _P, _I, _P, _I, _P, _I, _P, C.POINTER(Params), C.POINTER(_P)]),
     "swe_destroy": (_I, [_P]),
+    "swe_errors_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
+    "swe_compute_errors": (_I, [_P, C.c_double, _P]),
+    "swe_errors_destroy": (_I, [_P]),
     "swe_set_params": (_I, [_P, C.POINTER(Params)]),
     "swe_set_bc": (_I, [_P, _I, _D, _D, _D]),
     "swe_set_bc_values": (_I, [_P, _P, _P, _P]),
@@ -433,3 +436,47 @@ class Solver:
 
     def use_graph(self, on=True):
         _check(load_library().swe_use_graph(self.h, int(bool(on))))
+
+    def error_evaluator(self, jj, rr):
+        """compute_errors (main.f90:198-263) on the device state; jj (me, 3) 0-based, rr (np, 2)."""
+        return ErrorEvaluator(self, jj, rr)
+
+
+class ErrorEvaluator:
+    """The quadrature norms of compute_errors for one solver (swe_errors_create)."""
+
+    def __init__(self, solver, jj, rr):
+        jj1 = np.ascontiguousarray(jj, np.int32) + 1
+        rr = np.ascontiguousarray(rr, np.float64)
+        if rr.shape != (solver.np, 2) or jj1.ndim != 2 or jj1.shape[1] != 3:
+            raise SweError(SWE_EINVAL, "jj must be (me, 3) and rr (%d, 2)" % solver.np)
+        self.solver = solver                    # the handle must outlive the evaluator
+        e = C.c_void_p()
+        _check(load_library().swe_errors_create(solver.h, jj1.shape[0], _ptr(jj1), _ptr(rr), C.byref(e)))
+        self.e = e
+
+    def sums(self, time=None):
+        """(3, 4): per component (h, qx, qy) the L1 error, L1 norm, L2 error, L2 norm at `time`
+        (default: the solver's time)."""
+        out = np.zeros(12)
+        t = self.solver.time if time is None else float(time)
+        _check(load_library().swe_compute_errors(self.e, t, _ptr(out)))
+        return out.reshape(3, 4)
+
+    def relative(self, time=None):
+        """What compute_errors prints: the relative L1 and L2 errors on h (main.f90:214,221), and
+        the relative errors on q of its commented-out lines ((err_x+err_y)/(norm_x+norm_y))."""
+        s = self.sums(time)
+        return {"l1_h": s[0, 0] / s[0, 1], "l2_h": s[0, 2] / s[0, 3],
+                "l1_q": (s[1, 0] + s[2, 0]) / (s[1, 1] + s[2, 1]), "l2_q": (s[1, 2] + s[2, 2]) / (s[1, 3] + s[2, 3])}
+
+    def close(self):
+        if getattr(self, "e", None):
+            load_library().swe_errors_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -16,6 +16,8 @@ links the reference modules) and the outputs it wrote.  Cases:
   cav025_run.npz      the reference's own mesh MESHES/DAM_BREAK/cav_0.25.FEM, test 4 init, FCT + alpha
   cdisk_coriolis.npz  the reference's own mesh MESHES/PARABOLOID/cdisk.1_mesh.FEM, test 12 (Coriolis)
   par_friction.npz    test 11 (linear friction) on a 41x41 grid of the paraboloid, viscous + alpha
+  errors.npz          the quadrature norms of compute_errors (SOURCES/main.f90:198-263) on states of the
+                      fixtures above (tests 4, 5, 11, 12), by the reference's sol_anal, ns_anal_l1, ns_anal_0
 """
 from __future__ import annotations
 
@@ -305,20 +307,57 @@ def gen_par_friction():
     case.close()
 
 
+def gen_errors():
+    """compute_errors' sums, (3, 4) per case: [L1 error, L1 norm, L2 error, L2 norm] of h, qx, qy.
+    The states and meshes are those of the fixtures named in each key."""
+    out = {}
+
+    def errs(case, u, t):
+        np.ascontiguousarray(u, dtype="<f8").tofile(os.path.join(case.d, "state.f8"))
+        O.run_ref(["errors", "data", case.init, "state.f8", repr(float(t)), "errors.f8"], cwd=case.d)
+        return np.fromfile(os.path.join(case.d, "errors.f8")).reshape(3, 4)
+
+    z = np.load(os.path.join(HERE, "par41_modes.npz"))
+    case = RefCase(M.structured_mesh(41, 41, 0.0, 4.0, 0.0, 4.0), test=5, viscosity="viscous", cfl=0.25)
+    S = case.setup()
+    assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
+    out["par41_modes|states0|0.0"] = errs(case, z["states"][0], 0.0)
+    out["par41_modes|states1|0.75"] = errs(case, z["states"][1], 0.75)
+    case.close()
+    for name, fem, test in (("cav025_run", os.path.join(REF, "MESHES", "DAM_BREAK", "cav_0.25.FEM"), 4),
+                            ("cdisk_coriolis", os.path.join(REF, "MESHES", "PARABOLOID", "cdisk.1_mesh.FEM"), 12),
+                            ("par_friction", None, 11)):
+        z = np.load(os.path.join(HERE, name + ".npz"))
+        if fem:
+            case = RefCase(fem_path=fem, test=test, viscosity="fct", alpha=True)
+        else:
+            case = RefCase(M.structured_mesh(41, 41, 0.0, 4.0, 0.0, 4.0), test=test, viscosity="viscous", alpha=True)
+        S = case.setup()
+        assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
+        out["%s|final|time" % name] = errs(case, z["final"], float(z["time"]))
+        case.close()
+    save("errors.npz", **out)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built: run `make -C oracle ref` first")
     only = set(sys.argv[1:])
     gens = [("lambda", gen_lambda), ("fct", gen_fct), ("par41", gen_par41_modes), ("par81", gen_par81_run),
             ("dambreak", gen_dambreak), ("bumps", gen_bumps), ("cav025", gen_cav025),
-            ("cdisk", gen_cdisk_coriolis), ("friction", gen_par_friction)]
+            ("cdisk", gen_cdisk_coriolis), ("friction", gen_par_friction), ("errors", gen_errors)]
     for name, fn in gens:
         if not only or name in only:
             fn()
     manifest = {f: os.path.getsize(os.path.join(HERE, f)) for f in sorted(os.listdir(HERE)) if f.endswith(".npz")}
-    with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
-        json.dump({"generator": "tests/golden/make_golden.py", "reference": "t-bltg/shallow-water-eric built with "
-                   "amdflang -O2 -ffp-contract=off by oracle/Makefile", "seed": SEED, "files": manifest}, fh, indent=1)
+    path = os.path.join(HERE, "MANIFEST.json")
+    old = json.load(open(path)) if os.path.exists(path) else {}
+    doc = {"generator": "tests/golden/make_golden.py", "reference": "t-bltg/shallow-water-eric built with "
+           "amdflang -O2 -ffp-contract=off by oracle/Makefile", "seed": SEED, "files": manifest}
+    if "copied" in old:
+        doc["copied"] = old["copied"]
+    with open(path, "w") as fh:
+        json.dump(doc, fh, indent=1)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,71 @@
+"""compute_errors (SOURCES/main.f90:198-263) in the oracle: pinned bitwise against the reference's own
+sol_anal, ns_anal_l1 and ns_anal_0 (tests/golden/errors.npz, written by the reference build), and its
+sequential sums against the exact sums of the same summands (math.fsum) within the rigorous bound of
+recursive summation, |S_seq - S| <= (n-1)u/(1-(n-1)u) * sum|t_i|."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+
+def _case(key):
+    name, st, t = key.split("|")
+    z = golden(name + ".npz")
+    u = z["states"][0] if st == "states0" else z["states"][1] if st == "states1" else z["final"]
+    t = float(z["time"]) if t == "time" else float(t)
+    return z, u, t
+
+
+def _keys():
+    return sorted(golden("errors.npz"))
+
+
+@pytest.mark.parametrize("key", _keys())
+def test_oracle_errors_are_the_references(oracle, key):
+    z, u, t = _case(key)
+    g, htiny = z["scalars"][0], z["scalars"][1]
+    out = oracle.compute_errors(z["jj"], z["rr"], u, int(z["type_test"]), t, g, htiny)
+    assert np.array_equal(out, golden("errors.npz")[key])
+
+
+@pytest.mark.parametrize("key", _keys())
+def test_oracle_error_sums_within_the_summation_bound(oracle, key):
+    z, u, t = _case(key)
+    g, htiny = z["scalars"][0], z["scalars"][1]
+    out, terms = oracle.compute_errors(z["jj"], z["rr"], u, int(z["type_test"]), t, g, htiny, terms=True)
+    n = terms.shape[-1]
+    gam = (n - 1) * 2.0 ** -53 / (1 - (n - 1) * 2.0 ** -53)
+    for k in range(3):
+        for q in range(4):
+            exact = math.fsum(terms[k, q])
+            seq = out[k, q] ** 2 if q >= 2 else out[k, q]
+            # the L2 entries are square roots: compare the sums (sqrt is correctly rounded, so
+            # squaring it back is within 2 ulp of the sequential sum)
+            slack = 4 * 2.0 ** -53 * abs(seq) if q >= 2 else 0.0
+            assert abs(seq - exact) <= gam * np.abs(terms[k, q]).sum() + slack
+
+
+def test_sol_anal_of_test_5_is_the_initial_state(oracle):
+    """sol_anal(k, rr, 0) of test 5 at the mesh nodes is the reference's initial state (init, the
+    run fixture's u0), evaluated there by the reference itself."""
+    z = golden("par41_modes.npz")
+    g, htiny = z["scalars"][0], z["scalars"][1]
+    rr = z["rr"]
+    u = np.array([[oracle.sol_anal(5, k, x, y, 0.0, g, htiny) for k in (1, 2, 3)] for x, y in rr])
+    assert np.array_equal(u, z["u0"])
+
+
+def test_error_of_the_exact_field_vanishes(oracle):
+    """A plain invariant: with u set to the exact solution where it is constant (test 4 for
+    x < x0 - t*speed: h = hL, q = 0 on all of [0,4]^2) the errors are the rounding of the P1
+    interpolation (a few ulp of the norm), the norms are hL * area in L1 and hL * sqrt(area) in L2."""
+    from swe_amd import mesh as M
+    m = M.structured_mesh(9, 9, 0.0, 4.0, 0.0, 4.0)
+    u = np.zeros((m.np, 3))
+    u[:, 0] = 0.005
+    out = oracle.compute_errors(m.jj, m.rr, u, 4, 1.0, 9.81, 1e-7)
+    assert out[0, 0] <= 8 * 2.0 ** -53 * out[0, 1] and out[0, 2] <= 8 * 2.0 ** -53 * out[0, 3]
+    assert abs(out[0, 1] - 0.005 * 16.0) <= 1e-15 and abs(out[0, 3] - 0.005 * 4.0) <= 1e-15
+    assert np.all(out[1:, :] == 0.0)

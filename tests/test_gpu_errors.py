@@ -1,0 +1,98 @@
+"""compute_errors on the device (swe_compute_errors) against the oracle, which is bitwise the
+reference's own routines on these states (test_errors_oracle.py).  The device sums are double-double
+over a fixed partition, i.e. the exactly rounded sums: they must equal math.fsum of the oracle's
+summands to 2 ulp, and the reference's sequential sums within that sum's rounding bound (far below
+1e-11 relative on these meshes)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"par41_modes": ("viscous", False, 5), "cav025_run": ("fct", True, 4),
+         "cdisk_coriolis": ("viscous", True, 12), "par_friction": ("viscous", True, 11)}
+
+
+def _solver(z, viscosity, alpha, test):
+    from swe_amd.solver import Solver
+    g, htiny, hmax, cfl = z["scalars"]
+    return Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+                  max_water_h=hmax, viscosity=viscosity, alpha_limit=alpha, type_test=test,
+                  h_dir=z["h_dir"], ux_dir=z["ux_dir"], uy_dir=z["uy_dir"])
+
+
+def _check(dev, oracle_out, terms, ref=None):
+    n = terms.shape[-1]
+    gam = (n - 1) * 2.0 ** -53 / (1 - (n - 1) * 2.0 ** -53)
+    for k in range(3):
+        for q in range(4):
+            exact = math.fsum(terms[k, q])
+            exact = math.sqrt(exact) if q >= 2 else exact
+            assert abs(dev[k, q] - exact) <= 4 * 2.0 ** -53 * abs(exact), (k, q, dev[k, q], exact)
+            bound = gam * np.abs(terms[k, q]).sum()
+            if q >= 2:
+                bound = bound / (2 * max(exact, 1e-300)) + 4 * 2.0 ** -53 * exact
+            assert abs(dev[k, q] - oracle_out[k, q]) <= bound + 4 * 2.0 ** -53 * abs(exact)
+            if ref is not None:
+                assert abs(dev[k, q] - ref[k, q]) <= 1e-11 * max(abs(ref[k, q]), 1e-300)
+
+
+@pytest.mark.parametrize("key", sorted(golden("errors.npz")))
+def test_device_errors_match_the_reference(oracle, key):
+    name, st, t = key.split("|")
+    z = golden(name + ".npz")
+    u = z["states"][0] if st == "states0" else z["states"][1] if st == "states1" else z["final"]
+    t = float(z["time"]) if t == "time" else float(t)
+    print(key, flush=True)
+    with _solver(z, *CASES[name]) as s:
+        s.set_state(u)
+        ev = s.error_evaluator(z["jj"], z["rr"])
+        dev = ev.sums(t)
+        again = ev.sums(t)
+        ev.close()
+    g, htiny = z["scalars"][0], z["scalars"][1]
+    out, terms = oracle.compute_errors(z["jj"], z["rr"], u, int(z["type_test"]), t, g, htiny, terms=True)
+    assert np.array_equal(dev, again)
+    _check(dev, out, terms, golden("errors.npz")[key])
+
+
+def test_device_errors_after_a_run_at_scale(oracle):
+    """The paraboloid (test 5) on a 301x301 grid after 50 steps, at the solver's own time; the same
+    bits from a solver laid out on the device (another internal numbering)."""
+    from swe_amd import problems as PR
+    p = PR.paraboloid(n=301)
+    print("paraboloid", p.np, flush=True)
+    res = []
+    for dev_setup in (False, True):
+        with p.solver(device_setup=dev_setup) as s:
+            s.set_state(p.u0)
+            s.run(50, time=0.0)
+            u = s.get_state()
+            t = s.time
+            ev = s.error_evaluator(p.mesh.jj, p.mesh.rr)
+            res.append((ev.sums(), ev.relative(), u, t))
+            ev.close()
+    assert np.array_equal(res[0][0], res[1][0])
+    dev, rel, u, t = res[0]
+    prm = p.params
+    out, terms = oracle.compute_errors(p.mesh.jj, p.mesh.rr, u, 5, t, prm["gravity"], prm["htiny"], terms=True)
+    _check(dev, out, terms)
+    assert abs(rel["l1_h"] - out[0, 0] / out[0, 1]) <= 1e-11 * rel["l1_h"]
+
+
+def test_device_errors_reject_tests_without_sol_anal_and_bad_lists():
+    from swe_amd.solver import SweError
+    z = golden("dambreak_run.npz")            # test 10: no analytic solution
+    with _solver(z, "fct", True, 10) as s:
+        s.set_state(z["u0"])
+        ev = s.error_evaluator(z["jj"], z["rr"])
+        with pytest.raises(SweError):
+            ev.sums(1.0)
+        ev.close()
+        bad = np.array(z["jj"], copy=True)
+        bad[0, 0] = z["rr"].shape[0]          # node out of range
+        with pytest.raises(SweError):
+            s.error_evaluator(bad, z["rr"])
