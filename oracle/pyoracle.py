@@ -256,9 +256,17 @@ def write_custom_init(path, bath, u, *, gravity, max_water_h, bc_kind=0, amp=0.0
         f.write(np.ascontiguousarray(u, dtype="<f8").tobytes())
 
 
+def _big_stack():
+    # the reference keeps (3, np) work arrays as automatic arrays on the stack: lift the soft stack
+    # limit to the hard one in the child so that meshes beyond a few thousand nodes fit
+    import resource
+    _, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
 def run_ref(args, cwd, timeout=3600):
     r = subprocess.run([REF_DRIVER] + [str(a) for a in args], cwd=cwd, capture_output=True,
-                       text=True, timeout=timeout)
+                       text=True, timeout=timeout, preexec_fn=_big_stack)
     if r.returncode != 0 or "BUG" in r.stdout or "STOP" in r.stdout:
         raise RuntimeError("ref_driver %s failed (%d):\n%s\n%s" % (args[0], r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     return r.stdout

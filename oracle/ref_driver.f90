@@ -20,7 +20,7 @@
 !   ref_driver run <data> <init|none> <nsteps> <outdir> [dump_every]
 !       nsteps SSP-RK3 steps as in the time loop of SOURCES/main.f90:61-92 (fixed step count
 !       instead of the Tfinal loop); writes dt per step, U after each stage of step 1, the final
-!       U and optional snapshots.
+!       U, optional snapshots and the wall time of the loop.
 !   ref_driver errors <data> <init|none> <state> <time> <out>
 !       the quadrature norms of compute_errors (SOURCES/main.f90:198-263) at a given state and
 !       time: for each component k = 1..3 of sol_anal (SOURCES/condlim.f90:311-831) at the
@@ -374,6 +374,7 @@ CONTAINS
     REAL(KIND=8), DIMENSION(:), ALLOCATABLE :: dts
     CHARACTER(LEN=512) :: od, s, fname
     INTEGER :: nsteps, every, it
+    INTEGER(KIND=8) :: c0, c1, crate
     REAL(KIND=8) :: to
     CALL prepare(arg(2), arg(3))
     s = arg(4)
@@ -386,6 +387,7 @@ CONTAINS
     END IF
     ALLOCATE(ui(inputs%syst_size, mesh%np), uo(inputs%syst_size, mesh%np), dts(MAX(nsteps,1)))
     CALL write_state(TRIM(od)//'/u0.f8', un)
+    CALL SYSTEM_CLOCK(c0, crate)
     DO it = 1, nsteps
        IF (inputs%type_test==9) THEN
           inputs%htiny = 1.d-3
@@ -417,6 +419,10 @@ CONTAINS
           END IF
        END IF
     END DO
+    CALL SYSTEM_CLOCK(c1)
+    ! wall time of the time loop (the reference's own timer, user_time in SOURCES/main.f90:190-196,
+    ! reads the same clock)
+    CALL write_real(TRIM(od)//'/loop_seconds.f8', [REAL(c1 - c0, 8) / REAL(crate, 8)])
     CALL write_real(TRIM(od)//'/dt.f8', dts(1:nsteps))
     CALL write_state(TRIM(od)//'/final.f8', un)
     CALL write_real(TRIM(od)//'/time.f8', [inputs%time])
