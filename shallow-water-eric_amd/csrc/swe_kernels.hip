@@ -567,6 +567,7 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, fix = 0.0;   // smb_2_roundoff (or smb_2)
     double e1 = 0.0, e2 = 0.0, e3 = 0.0;              // smb_2 of entropy_residual
     double emax = 0.0, emin = 0.0;
+    bool split = false;   // the entropy residual's momentum sums have left the low-order ones
     const int64_t pb = row_base(M, i);
     SWE_UNROLL(SWE_UNROLL_LO)
     for (int q = 0; q < len; ++q) {
@@ -608,6 +609,15 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
         xx3 = xx3 - c.x * (v1j * J.qy);
         xx3 = xx3 - c.y * (v2j * J.qy);
         const double topo = K.g * I.h * (J.h + J.z - I.h - I.z);
+        // The momentum sums of the low-order update (thresholds htiny) and of the entropy residual
+        // (thresholds 0) take the same terms in the same order while every entry's ratios agree, so
+        // they are the same numbers: e2, e3 are copied from r2, r3 at the first entry whose ratios
+        // differ (or at the end) instead of being summed alongside.
+        if (WRITE_LO && !split && !(ratjir == ratji0 && ratijr == ratij0)) {
+            split = true;
+            e2 = r2;
+            e3 = r3;
+        }
         if (WRITE_LO) {
             if (roundoff) {
                 if (!off) fix = fix + xm;
@@ -629,15 +639,17 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
             r3 = r3 - topo * c.y;
         }
         e1 = e1 + xe;
-        e2 = e2 + xx2;
-        e3 = e3 + xx3;
-        if (off) {
-            e1 = e1 + d * (J.h * ratji0 - I.h * ratij0) + mu * (J.h * (1.0 - ratji0) - I.h * (1.0 - ratij0));
-            e2 = e2 + d * (J.qx * ratji0 - I.qx * ratij0) + mu * (J.qx * (1.0 - ratji0) - I.qx * (1.0 - ratij0));
-            e3 = e3 + d * (J.qy * ratji0 - I.qy * ratij0) + mu * (J.qy * (1.0 - ratji0) - I.qy * (1.0 - ratij0));
+        if (off) e1 = e1 + d * (J.h * ratji0 - I.h * ratij0) + mu * (J.h * (1.0 - ratji0) - I.h * (1.0 - ratij0));
+        if (!WRITE_LO || split) {
+            e2 = e2 + xx2;
+            e3 = e3 + xx3;
+            if (off) {
+                e2 = e2 + d * (J.qx * ratji0 - I.qx * ratij0) + mu * (J.qx * (1.0 - ratji0) - I.qx * (1.0 - ratij0));
+                e3 = e3 + d * (J.qy * ratji0 - I.qy * ratij0) + mu * (J.qy * (1.0 - ratji0) - I.qy * (1.0 - ratij0));
+            }
+            e2 = e2 - topo * c.x;
+            e3 = e3 - topo * c.y;
         }
-        e2 = e2 - topo * c.x;
-        e3 = e3 - topo * c.y;
         // entropy of the neighbour for maxmin (update.f90:924-935)
         double Ej = K.g * (J.h * J.h);
         Ej = Ej + v1j * J.qx;
@@ -648,6 +660,10 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
             emax = (Ej > emax) ? Ej : emax;
             emin = (Ej < emin) ? Ej : emin;
         }
+    }
+    if (WRITE_LO && !split) {
+        e2 = r2;
+        e3 = r3;
     }
     add_sources<HAS_SRC>(K.src, I, mi, v1i, v2i, dt, K, e2, e3);
     if (WRITE_LO) {
