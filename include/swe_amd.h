@@ -266,6 +266,10 @@ int swe_batch_status(swe_batch *b, int id, int *state, double *time, long long *
 /* Instances that are done or failed, up to cap of them (waits for queued work). */
 int swe_batch_poll(swe_batch *b, int *ids, int cap, int *n);
 int swe_batch_get_state(swe_batch *b, int id, double *un);
+/* swe_errors_create for instance `id` of the batch (compute_errors on its state, with its parameters
+ * and the time passed to swe_compute_errors); the evaluator reports SWE_EINVAL once the instance has
+ * been removed. */
+int swe_batch_errors_create(swe_batch *b, int id, int me, const int *jj, const double *rr, swe_errors **out);
 
 /* Pipelined serving: the host never waits for the step it just queued.
  * swe_batch_step_async queues nsteps steps, then a copy of the instance table, and returns that

@@ -35,7 +35,10 @@ struct swe_batch {
 };
 
 struct swe_errors {
-    swe_handle* h = nullptr;
+    swe_handle* h = nullptr;     // a solver's evaluator, or
+    swe::Pool* P = nullptr;      // a batch instance's: the pool, the id and its admission's ticket
+    int k = 0;
+    long long since = 0;
     swe::ErrEval* E = nullptr;
 };
 
@@ -254,17 +257,20 @@ int swe_errors_create(swe_handle* h, int me, const int* jj, const double* rr, sw
     swe::ErrEval* E = nullptr;
     const int rc = errors_create(h->P, kId, me, jj, rr, &E);
     if (rc) return rc;
-    *out = new swe_errors{h, E};
+    *out = new swe_errors{h, h->P, kId, h->P->inst[(size_t)kId].since, E};
     return SWE_OK;
 }
 
 int swe_compute_errors(swe_errors* e, double time, double* out) {
     if (!e || !out) return fail(SWE_EINVAL, "null argument");
-    swe_handle* h = e->h;
-    if (!h->state_set) return fail(SWE_EINVAL, "no state has been set");
-    if (h->step_open) return fail(SWE_EINVAL, "a step is open (swe_step_begin without swe_step_finish)");
-    const swe_params& prm = h->P->inst[(size_t)kId].prm;
-    return errors_eval(h->P, e->E, h->P->S[0], prm.type_test, time, prm.gravity, prm.htiny, out);
+    if (swe_handle* h = e->h) {
+        if (!h->state_set) return fail(SWE_EINVAL, "no state has been set");
+        if (h->step_open) return fail(SWE_EINVAL, "a step is open (swe_step_begin without swe_step_finish)");
+    }
+    Pool* P = e->P;
+    const InstHost& I = P->inst[(size_t)e->k];
+    if (!I.used || I.since != e->since) return fail(SWE_EINVAL, "instance %d is no longer the evaluator's mesh", e->k);
+    return errors_eval(P, e->E, P->S[0], I.prm.type_test, time, I.prm.gravity, I.prm.htiny, out);
 }
 
 int swe_errors_destroy(swe_errors* e) {
@@ -719,6 +725,18 @@ int swe_batch_poll(swe_batch* b, int* ids, int cap, int* n) {
         if (*n < cap) ids[(*n)++] = k;
     }
     return unlisted ? pool_update_active(P) : SWE_OK;
+}
+
+int swe_batch_errors_create(swe_batch* b, int id, int me, const int* jj, const double* rr, swe_errors** out) {
+    clear_error();
+    if (!b || !jj || !rr || !out) return fail(SWE_EINVAL, "null argument");
+    *out = nullptr;
+    if (!live(b->P, id)) return fail(SWE_EINVAL, "no instance %d", id);
+    swe::ErrEval* E = nullptr;
+    const int rc = errors_create(b->P, id, me, jj, rr, &E);
+    if (rc) return rc;
+    *out = new swe_errors{nullptr, b->P, id, b->P->inst[(size_t)id].since, E};
+    return SWE_OK;
 }
 
 int swe_batch_get_state(swe_batch* b, int id, double* un) {

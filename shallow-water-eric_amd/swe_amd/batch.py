@@ -121,6 +121,11 @@ class Batch:
         _check(load_library().swe_batch_poll(self.h, _ptr(ids), cap, C.byref(n)))
         return [int(x) for x in ids[:n.value]]
 
+    def error_evaluator(self, k, jj, rr):
+        """compute_errors (main.f90:198-263) on instance k's device state; jj (me, 3) 0-based, rr (np, 2)."""
+        from .solver import ErrorEvaluator
+        return ErrorEvaluator(self, jj, rr, k=k)
+
     def get_state(self, k):
         out = np.zeros((self._np[k], 3))
         _check(load_library().swe_batch_get_state(self.h, int(k), _ptr(out)))

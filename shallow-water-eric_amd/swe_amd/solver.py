@@ -67,6 +67,7 @@ SIGNATURES = {
     "swe_errors_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
     "swe_compute_errors": (_I, [_P, C.c_double, _P]),
     "swe_errors_destroy": (_I, [_P]),
+    "swe_batch_errors_create": (_I, [_P, _I, _I, _P, _P, C.POINTER(_P)]),
     "swe_set_params": (_I, [_P, C.POINTER(Params)]),
     "swe_set_bc": (_I, [_P, _I, _D, _D, _D]),
     "swe_set_bc_values": (_I, [_P, _P, _P, _P]),
@@ -443,23 +444,35 @@ class Solver:
 
 
 class ErrorEvaluator:
-    """The quadrature norms of compute_errors for one solver (swe_errors_create)."""
+    """The quadrature norms of compute_errors for one solver (swe_errors_create) or for instance `k`
+    of a batch (swe_batch_errors_create)."""
 
-    def __init__(self, solver, jj, rr):
+    def __init__(self, owner, jj, rr, k=None):
         jj1 = np.ascontiguousarray(jj, np.int32) + 1
         rr = np.ascontiguousarray(rr, np.float64)
-        if rr.shape != (solver.np, 2) or jj1.ndim != 2 or jj1.shape[1] != 3:
-            raise SweError(SWE_EINVAL, "jj must be (me, 3) and rr (%d, 2)" % solver.np)
-        self.solver = solver                    # the handle must outlive the evaluator
+        npn = owner.np if k is None else owner._np[k]
+        if rr.shape != (npn, 2) or jj1.ndim != 2 or jj1.shape[1] != 3:
+            raise SweError(SWE_EINVAL, "jj must be (me, 3) and rr (%d, 2)" % npn)
+        self.owner = owner                      # the handle must outlive the evaluator
+        self.k = k
         e = C.c_void_p()
-        _check(load_library().swe_errors_create(solver.h, jj1.shape[0], _ptr(jj1), _ptr(rr), C.byref(e)))
+        if k is None:
+            _check(load_library().swe_errors_create(owner.h, jj1.shape[0], _ptr(jj1), _ptr(rr), C.byref(e)))
+        else:
+            _check(load_library().swe_batch_errors_create(owner.h, int(k), jj1.shape[0], _ptr(jj1), _ptr(rr),
+                                                          C.byref(e)))
         self.e = e
 
     def sums(self, time=None):
         """(3, 4): per component (h, qx, qy) the L1 error, L1 norm, L2 error, L2 norm at `time`
-        (default: the solver's time)."""
+        (default: the solver's or the instance's time)."""
         out = np.zeros(12)
-        t = self.solver.time if time is None else float(time)
+        if time is not None:
+            t = float(time)
+        elif self.k is None:
+            t = self.owner.time
+        else:
+            t = self.owner.status(self.k)[1]
         _check(load_library().swe_compute_errors(self.e, t, _ptr(out)))
         return out.reshape(3, 4)
 
