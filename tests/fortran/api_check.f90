@@ -1,0 +1,77 @@
+!==============================================================================================
+! api_check.f90 -- the device set-up and compute_errors entry points called from Fortran through
+! the interface blocks of shallow-water-eric_amd/fortran/swe_amd_c.f90, the way a reference driver
+! calls them: construct_matrices on the device (swe_assemble_device + swe_create_from_assembly),
+! the time loop (swe_run), then compute_errors (swe_errors_create + swe_compute_errors).
+!
+! Reads in.bin and writes out.bin in the current directory (raw little-endian streams):
+!   in:  int32 np, me, nh, nux, nuy, nsteps, bc_kind; real8 bc_amp, bc_period, bc_level;
+!        real8 gravity, htiny, cfl, max_water_h, mannings; int32 type_test, viscosity_type,
+!        viscous_type, alpha_limit, roundoff_fix; int32 jj(3,me) (1-based); real8 rr(2,np),
+!        bath(np), un(3,np); int32 h_dir(nh), ux_dir(nux), uy_dir(nuy) (1-based)
+!   out: real8 un(3,np), time, dts(nsteps), errors(12)
+!==============================================================================================
+PROGRAM api_check
+  USE iso_c_binding
+  USE swe_amd_c
+  IMPLICIT NONE
+  INTEGER(c_int) :: np, me, nh, nux, nuy, nsteps, bc_kind, rc
+  INTEGER(c_int) :: ints(5)
+  REAL(c_double) :: bcr(3), reals(5), t, e12(12)
+  INTEGER(c_int), ALLOCATABLE :: jj(:,:), hd(:), uxd(:), uyd(:)
+  REAL(c_double), ALLOCATABLE :: rr(:,:), bath(:), un(:,:), dts(:)
+  TYPE(swe_params) :: prm
+  TYPE(c_ptr) :: asm, h, errs
+  INTEGER(c_long_long) :: nnz
+
+  OPEN(10, FILE='in.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
+  READ(10) np, me, nh, nux, nuy, nsteps, bc_kind
+  READ(10) bcr, reals, ints
+  ALLOCATE(jj(3,me), rr(2,np), bath(np), un(3,np), hd(MAX(nh,1)), uxd(MAX(nux,1)), uyd(MAX(nuy,1)), &
+       dts(MAX(nsteps,1)))
+  READ(10) jj, rr, bath, un
+  IF (nh > 0) READ(10) hd(1:nh)
+  IF (nux > 0) READ(10) uxd(1:nux)
+  IF (nuy > 0) READ(10) uyd(1:nuy)
+  CLOSE(10)
+  prm%gravity = reals(1)
+  prm%htiny = reals(2)
+  prm%cfl = reals(3)
+  prm%max_water_h = reals(4)
+  prm%mannings = reals(5)
+  prm%type_test = ints(1)
+  prm%viscosity_type = ints(2)
+  prm%viscous_type = ints(3)
+  prm%alpha_limit = ints(4)
+  prm%roundoff_fix = ints(5)
+
+  CALL check(swe_assemble_device(np, me, jj, rr, asm, nnz), 'swe_assemble_device')
+  CALL check(swe_create_from_assembly(asm, bath, nh, hd, nux, uxd, nuy, uyd, prm, h), 'swe_create_from_assembly')
+  CALL check(swe_assembly_destroy(asm), 'swe_assembly_destroy')
+  CALL check(swe_set_bc(h, bc_kind, bcr(1), bcr(2), bcr(3)), 'swe_set_bc')
+  CALL check(swe_set_state(h, un), 'swe_set_state')
+  t = 0.d0
+  CALL check(swe_run(h, nsteps, t, dts), 'swe_run')
+  CALL check(swe_get_state(h, un), 'swe_get_state')
+  CALL check(swe_errors_create(h, me, jj, rr, errs), 'swe_errors_create')
+  CALL check(swe_compute_errors(errs, t, e12), 'swe_compute_errors')
+  CALL check(swe_errors_destroy(errs), 'swe_errors_destroy')
+  CALL check(swe_destroy(h), 'swe_destroy')
+  WRITE(*,*) ' Relative L1 error on h (Gaussian)', e12(1)/e12(2)
+  WRITE(*,*) ' Relative L2 error on h (Gaussian)', e12(3)/e12(4)
+
+  OPEN(11, FILE='out.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE')
+  WRITE(11) un, t, dts(1:nsteps), e12
+  CLOSE(11)
+
+CONTAINS
+
+  SUBROUTINE check(code, what)
+    INTEGER(c_int), INTENT(IN) :: code
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    IF (code /= SWE_OK) THEN
+       WRITE(*,*) what, ' failed: ', code, ' ', TRIM(swe_error_message())
+       STOP 1
+    END IF
+  END SUBROUTINE check
+END PROGRAM api_check
