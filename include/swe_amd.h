@@ -250,6 +250,14 @@ int swe_batch_destroy(swe_batch *b);
  * room now.  The instance joins the steps queued after this call. */
 int swe_batch_add(swe_batch *b, const swe_mesh_desc *mesh, const swe_params *prm, const double *un,
                   double t0, double tfinal, long long max_steps, int *id);
+/* swe_batch_add for a mesh assembled by swe_assemble_device: the layout is built on the GPU from the
+ * assembly (as swe_create_from_assembly does) and filled on the batch's copy stream while queued steps
+ * run; bath, the Dirichlet lists (1-based) and un(3,np) come from the host.  The assembly may be
+ * destroyed when the call returns.  Results are bitwise those of swe_batch_add on the same arrays. */
+int swe_batch_add_assembled(swe_batch *b, const swe_assembly *a, const double *bath, int n_h_dir, const int *h_dir,
+                            int n_ux_dir, const int *ux_dir, int n_uy_dir, const int *uy_dir,
+                            const swe_params *prm, const double *un, double t0, double tfinal,
+                            long long max_steps, int *id);
 int swe_batch_set_bc(swe_batch *b, int id, int kind, double amp, double period, double level);
 int swe_batch_set_bc_values(swe_batch *b, int id, const double *h_vals, const double *ux_vals,
                             const double *uy_vals);

@@ -239,8 +239,8 @@ int swe_create_from_assembly(const swe_assembly* a, const double* bath, int n_h_
         return fail(SWE_EHIP, "swe_create_from_assembly: %s", hipGetErrorString(e));
     }
     int id = -1;
-    rc = pool_add_assembled(h->P, a, DL, bath, lists, nlist, *prm, 0.0, std::numeric_limits<double>::infinity(),
-                            LLONG_MAX, &id, h->csr2slot);
+    rc = pool_add_assembled(h->P, a, DL, bath, lists, nlist, *prm, nullptr, 0.0,
+                            std::numeric_limits<double>::infinity(), LLONG_MAX, &id, h->csr2slot);
     free_device_layout(DL);
     if (rc) {
         free_handle(h);
@@ -625,6 +625,26 @@ int swe_batch_add(swe_batch* b, const swe_mesh_desc* d, const swe_params* prm, c
     if (!b || !d || !prm || !un || !id) return fail(SWE_EINVAL, "null argument");
     if (max_steps < 0) return fail(SWE_EINVAL, "max_steps < 0");
     return pool_add(b->P, *d, *prm, un, t0, tfinal, max_steps, id, nullptr);
+}
+
+int swe_batch_add_assembled(swe_batch* b, const swe_assembly* a, const double* bath, int n_h_dir, const int* h_dir,
+                            int n_ux_dir, const int* ux_dir, int n_uy_dir, const int* uy_dir, const swe_params* prm,
+                            const double* un, double t0, double tfinal, long long max_steps, int* id) {
+    clear_error();
+    if (!b || !a || !bath || !prm || !un || !id) return fail(SWE_EINVAL, "null argument");
+    if (max_steps < 0) return fail(SWE_EINVAL, "max_steps < 0");
+    DeviceLayout DL;
+    int rc = build_layout_device(a, DL);
+    if (rc) return rc;
+    if (DL.nnz > INT_MAX) {
+        free_device_layout(DL);
+        return fail(SWE_EINVAL, "%lld entries do not fit the solver's 32-bit CSR", (long long)DL.nnz);
+    }
+    const int* lists[3] = {h_dir, ux_dir, uy_dir};
+    const int nlist[3] = {n_h_dir, n_ux_dir, n_uy_dir};
+    rc = pool_add_assembled(b->P, a, DL, bath, lists, nlist, *prm, un, t0, tfinal, max_steps, id, nullptr);
+    free_device_layout(DL);
+    return rc;
 }
 
 int swe_batch_set_bc(swe_batch* b, int id, int kind, double amp, double period, double level) {

@@ -219,6 +219,7 @@ struct DeviceLayout {
     int* len = nullptr;        // entries of each internal row (np_pad)
     int64_t* sptr = nullptr;   // slot offset of each slice (nsl + 1)
     int64_t* eoff = nullptr;   // compact-list offset of each slice (nsl + 1)
+    hipStream_t stream = nullptr;   // the assembly's stream: the buffers are allocated and freed on it
 };
 int build_layout_device(const swe_assembly* a, DeviceLayout& DL);
 void free_device_layout(DeviceLayout& DL);
@@ -232,8 +233,8 @@ int errors_eval(Pool* P, const ErrEval* E, const Node* U, int type_test, double 
 void errors_destroy(ErrEval* E);
 
 int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, const double* bath,
-                       const int* const lists[3], const int nlist[3], const swe_params& prm, double t0,
-                       double tfinal, long long max_steps, int* id, int64_t* csr2slot);
+                       const int* const lists[3], const int nlist[3], const swe_params& prm, const double* un,
+                       double t0, double tfinal, long long max_steps, int* id, int64_t* csr2slot);
 
 // ---- launch wrappers (swe_kernels.hip) --------------------------------------------------------
 Mesh mesh_view(const Pool& P);

@@ -177,30 +177,39 @@ struct swe_assembly {
 
 namespace {
 
+// Device memory of the set-up is stream-ordered (hipMallocAsync / hipFreeAsync on the set-up's own
+// stream): unlike hipFree, releasing it does not wait for the work queued on other streams, so a
+// batch can admit meshes while its steps run.
+template <class T>
+hipError_t dalloc(T** p, size_t n, hipStream_t s) {
+    return hipMallocAsync((void**)p, sizeof(T) * (n ? n : 1), s);
+}
+
+void dfree(void* p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
 void release(swe_assembly* a) {
     if (!a) return;
     void* bufs[] = {a->ia0, a->ja, a->diag, a->mass, a->lumped, a->cx, a->cy};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
-    if (a->stream) (void)hipStreamDestroy(a->stream);
+    for (void* b : bufs) dfree(b, a->stream);
+    if (a->stream) {
+        (void)hipStreamSynchronize(a->stream);
+        (void)hipStreamDestroy(a->stream);
+    }
     delete a;
 }
 
-template <class T>
-hipError_t dalloc(T** p, size_t n) {
-    return hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
-}
-
+// Exclusive prefix sum on stream s (ordered with the stream's other work; nothing waits here).
 hipError_t exclusive_scan(const int64_t* in, int64_t* out, int n, hipStream_t s) {
     size_t bytes = 0;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s);
     if (e != hipSuccess) return e;
     void* tmp = nullptr;
-    if ((e = hipMalloc(&tmp, bytes ? bytes : 1)) != hipSuccess) return e;
+    if ((e = hipMallocAsync(&tmp, bytes ? bytes : 1, s)) != hipSuccess) return e;
     e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, s);
-    const hipError_t e2 = hipStreamSynchronize(s);
-    (void)hipFree(tmp);
-    return e != hipSuccess ? e : e2;
+    dfree(tmp, s);
+    return e;
 }
 
 }  // namespace
@@ -222,14 +231,14 @@ extern "C" int swe_assemble_device(int np, int me, const int* jj, const double* 
     int too_many = 0;
     hipError_t e = hipStreamCreate(&a->stream);
     hipStream_t s = a->stream;
-    if (e == hipSuccess) e = dalloc(&djj, (size_t)n3);
-    if (e == hipSuccess) e = dalloc(&drr, 2 * (size_t)np);
-    if (e == hipSuccess) e = dalloc(&elem, (size_t)n3);
-    if (e == hipSuccess) e = dalloc(&fill, (size_t)np);
-    if (e == hipSuccess) e = dalloc(&flag, 1);
-    if (e == hipSuccess) e = dalloc(&cnt, (size_t)np + 1);
-    if (e == hipSuccess) e = dalloc(&start, (size_t)np + 1);
-    if (e == hipSuccess) e = dalloc(&a->ia0, (size_t)np + 1);
+    if (e == hipSuccess) e = dalloc(&djj, (size_t)n3, s);
+    if (e == hipSuccess) e = dalloc(&drr, 2 * (size_t)np, s);
+    if (e == hipSuccess) e = dalloc(&elem, (size_t)n3, s);
+    if (e == hipSuccess) e = dalloc(&fill, (size_t)np, s);
+    if (e == hipSuccess) e = dalloc(&flag, 1, s);
+    if (e == hipSuccess) e = dalloc(&cnt, (size_t)np + 1, s);
+    if (e == hipSuccess) e = dalloc(&start, (size_t)np + 1, s);
+    if (e == hipSuccess) e = dalloc(&a->ia0, (size_t)np + 1, s);
     if (e == hipSuccess) e = hipMemcpyAsync(djj, jj, sizeof(int) * (size_t)n3, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(drr, rr, sizeof(double) * 2 * (size_t)np, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemsetAsync(cnt, 0, sizeof(int64_t) * ((size_t)np + 1), s);
@@ -259,12 +268,12 @@ extern "C" int swe_assemble_device(int np, int me, const int* jj, const double* 
             rc = swe::fail(SWE_EINVAL, "nnz = %lld does not fit the 32-bit row pointers", (long long)total);
         } else if (e == hipSuccess) {
             a->nnz = total;
-            e = dalloc(&a->ja, (size_t)total);
-            if (e == hipSuccess) e = dalloc(&a->diag, (size_t)np);
-            if (e == hipSuccess) e = dalloc(&a->mass, (size_t)total);
-            if (e == hipSuccess) e = dalloc(&a->lumped, (size_t)np);
-            if (e == hipSuccess) e = dalloc(&a->cx, (size_t)total);
-            if (e == hipSuccess) e = dalloc(&a->cy, (size_t)total);
+            e = dalloc(&a->ja, (size_t)total, s);
+            if (e == hipSuccess) e = dalloc(&a->diag, (size_t)np, s);
+            if (e == hipSuccess) e = dalloc(&a->mass, (size_t)total, s);
+            if (e == hipSuccess) e = dalloc(&a->lumped, (size_t)np, s);
+            if (e == hipSuccess) e = dalloc(&a->cx, (size_t)total, s);
+            if (e == hipSuccess) e = dalloc(&a->cy, (size_t)total, s);
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(k_row_fill, dim3(grid_of(np, 128)), dim3(128), 0, s, np, start, elem, djj, drr,
                                    a->ia0, a->ja, a->diag, a->mass, a->lumped, a->cx, a->cy);
@@ -274,8 +283,7 @@ extern "C" int swe_assemble_device(int np, int me, const int* jj, const double* 
         }
     }
     void* tmp[] = {djj, drr, elem, fill, flag, cnt, start};
-    for (void* b : tmp)
-        if (b) (void)hipFree(b);
+    for (void* b : tmp) dfree(b, s);
     if (e != hipSuccess) rc = swe::fail(SWE_EHIP, "device assembly: %s", hipGetErrorString(e));
     if (rc != SWE_OK) {
         release(a);
@@ -478,8 +486,7 @@ namespace swe {
 
 void free_device_layout(DeviceLayout& DL) {
     void* bufs[] = {DL.perm, DL.iperm, DL.len, DL.sptr, DL.eoff};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
+    for (void* b : bufs) dfree(b, DL.stream);
     DL = DeviceLayout{};
 }
 
@@ -492,18 +499,19 @@ int build_layout_device(const swe_assembly* a, DeviceLayout& DL) {
     const int nsl = (np + 63) / 64, np_pad = nsl * 64;
     DL.nsl = nsl;
     hipStream_t s = a->stream;
+    DL.stream = s;
     int* lenc = nullptr;
     int* flag = nullptr;
     int64_t *slots = nullptr, *ecnt = nullptr;
-    hipError_t e = dalloc(&DL.perm, (size_t)np);
-    if (e == hipSuccess) e = dalloc(&DL.iperm, (size_t)np);
-    if (e == hipSuccess) e = dalloc(&DL.len, (size_t)np_pad);
-    if (e == hipSuccess) e = dalloc(&DL.sptr, (size_t)nsl + 1);
-    if (e == hipSuccess) e = dalloc(&DL.eoff, (size_t)nsl + 1);
-    if (e == hipSuccess) e = dalloc(&lenc, (size_t)np);
-    if (e == hipSuccess) e = dalloc(&flag, 2);
-    if (e == hipSuccess) e = dalloc(&slots, (size_t)nsl + 1);
-    if (e == hipSuccess) e = dalloc(&ecnt, (size_t)nsl + 1);
+    hipError_t e = dalloc(&DL.perm, (size_t)np, s);
+    if (e == hipSuccess) e = dalloc(&DL.iperm, (size_t)np, s);
+    if (e == hipSuccess) e = dalloc(&DL.len, (size_t)np_pad, s);
+    if (e == hipSuccess) e = dalloc(&DL.sptr, (size_t)nsl + 1, s);
+    if (e == hipSuccess) e = dalloc(&DL.eoff, (size_t)nsl + 1, s);
+    if (e == hipSuccess) e = dalloc(&lenc, (size_t)np, s);
+    if (e == hipSuccess) e = dalloc(&flag, 2, s);
+    if (e == hipSuccess) e = dalloc(&slots, (size_t)nsl + 1, s);
+    if (e == hipSuccess) e = dalloc(&ecnt, (size_t)nsl + 1, s);
     if (e == hipSuccess) e = hipMemsetAsync(flag, 0, 2 * sizeof(int), s);
     if (e == hipSuccess) e = hipMemsetAsync(slots + nsl, 0, sizeof(int64_t), s);
     if (e == hipSuccess) e = hipMemsetAsync(ecnt + nsl, 0, sizeof(int64_t), s);
@@ -524,8 +532,7 @@ int build_layout_device(const swe_assembly* a, DeviceLayout& DL) {
     if (e == hipSuccess) e = hipMemcpyAsync(&DL.ne, DL.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     void* tmp[] = {lenc, flag, slots, ecnt};
-    for (void* b : tmp)
-        if (b) (void)hipFree(b);
+    for (void* b : tmp) dfree(b, s);
     if (e != hipSuccess) {
         free_device_layout(DL);
         return fail(SWE_EHIP, "device layout: %s", hipGetErrorString(e));
@@ -539,8 +546,8 @@ int build_layout_device(const swe_assembly* a, DeviceLayout& DL) {
 }
 
 int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, const double* bath,
-                       const int* const lists[3], const int nlist[3], const swe_params& prm, double t0,
-                       double tfinal, long long max_steps, int* id, int64_t* csr2slot) {
+                       const int* const lists[3], const int nlist[3], const swe_params& prm, const double* un,
+                       double t0, double tfinal, long long max_steps, int* id, int64_t* csr2slot) {
     if (!a || !bath || !id) return fail(SWE_EINVAL, "null argument");
     Mode md;
     int rc = mode_of(prm, md);
@@ -568,17 +575,21 @@ int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, c
     }
     const int64_t r0 = s0 * 64;
     const int np_pad = DL.nsl * 64;
-    hipStream_t s = P->stream;
+    // the admission runs on the pool's copy stream, as an upload does: the steps queued on the compute
+    // stream go on meanwhile (the new rows are not in their launch lists yet)
+    hipStream_t s = P->copy;
     double* dbath = nullptr;
     int* dnodes = nullptr;
-    hipError_t e = dalloc(&dbath, (size_t)np);
-    if (e == hipSuccess) e = dalloc(&dnodes, (size_t)nbc);
+    hipError_t e = dalloc(&dbath, (size_t)np, s);
+    if (e == hipSuccess) e = dalloc(&dnodes, (size_t)nbc, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dbath, bath, sizeof(double) * (size_t)np, hipMemcpyHostToDevice, s);
     for (int c = 0, q = 0; c < 3 && e == hipSuccess; q += nlist[c], ++c)
         if (nlist[c] > 0)
             e = hipMemcpyAsync(dnodes + q, lists[c], sizeof(int) * (size_t)nlist[c], hipMemcpyHostToDevice, s);
     // ranges freed by a removal are rewritten only after the removal's kernels
     if (e == hipSuccess) e = hipStreamWaitEvent(s, P->released, 0);
+    if (e == hipSuccess && un)
+        e = hipMemcpyAsync(P->stage3 + 3 * r0, un, sizeof(double) * 3 * (size_t)np, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
         LayOut o{P->len, P->dpos, P->perm, P->m, P->z, P->col, P->dcol, P->c};
         hipLaunchKernelGGL(k_lay_fill, dim3(grid_of(np_pad, 256)), dim3(256), 0, s, np, np_pad, r0, p0,
@@ -595,9 +606,9 @@ int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, c
                                k, (const int*)dnodes, (const int*)DL.iperm, P->bc);
         e = hipGetLastError();
     }
+    dfree(dbath, s);
+    dfree(dnodes, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (dbath) (void)hipFree(dbath);
-    if (dnodes) (void)hipFree(dnodes);
     if (e != hipSuccess) {
         P->a_slices.release(s0, DL.nsl);
         P->a_slots.release(p0, DL.nslot);
@@ -621,7 +632,7 @@ int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, c
     I.nh = nlist[0];
     I.nux = nlist[1];
     I.nuy = nlist[2];
-    if ((rc = pool_activate(P, k, I, prm, false, t0, tfinal, max_steps))) return rc;
+    if ((rc = pool_activate(P, k, I, prm, un != nullptr, t0, tfinal, max_steps))) return rc;
     *id = k;
     return SWE_OK;
 }

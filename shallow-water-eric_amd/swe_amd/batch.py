@@ -83,13 +83,49 @@ class Batch:
         self._np[k.value] = mesh.np
         return k.value
 
-    def add_problem(self, problem, *, tfinal=math.inf, max_steps=None, t0=0.0):
-        """Admit a :class:`swe_amd.problems.Problem` with its boundary program."""
+    def add_elements(self, jj, rr, bath, params, u0, *, h_dir=(), ux_dir=(), uy_dir=(), t0=0.0, tfinal=math.inf,
+                     max_steps=2 ** 62, bc=(BC_NONE, 0.0, 1.0, 0.0)):
+        """Admit a mesh given by its element list (jj (me, 3) and the Dirichlet lists 0-based, rr (np, 2)):
+        construct_matrices and the layout run on the GPU (swe_assemble_device, swe_batch_add_assembled),
+        while queued steps go on.  Returns its id, or None when the batch has no room now."""
+        lib = load_library()
+        jj1 = np.ascontiguousarray(jj, np.int32) + 1
+        rr = np.ascontiguousarray(rr, np.float64)
+        npn = rr.shape[0]
+        bath = np.ascontiguousarray(bath, np.float64)
+        u0 = np.ascontiguousarray(u0, np.float64)
+        if bath.shape != (npn,) or u0.shape != (npn, 3):
+            raise SweError(5, "bath must have shape (%d,) and the state (%d, 3)" % (npn, npn))
+        lists = [np.ascontiguousarray(x, np.int32) + 1 for x in (h_dir, ux_dir, uy_dir)]
+        a = C.c_void_p()
+        n = C.c_longlong()
+        _check(lib.swe_assemble_device(npn, jj1.shape[0], _ptr(jj1), _ptr(rr), C.byref(a), C.byref(n)))
+        k = C.c_int(-1)
+        try:
+            rc = lib.swe_batch_add_assembled(self.h, a, _ptr(bath), lists[0].size, _ptr(lists[0]), lists[1].size,
+                                             _ptr(lists[1]), lists[2].size, _ptr(lists[2]), C.byref(params), _ptr(u0),
+                                             float(t0), float(tfinal), int(max_steps), C.byref(k))
+        finally:
+            lib.swe_assembly_destroy(a)
+        if rc == SWE_EFULL:
+            return None
+        _check(rc)
+        if bc[0] != BC_NONE:
+            _check(lib.swe_batch_set_bc(self.h, k.value, int(bc[0]), float(bc[1]), float(bc[2]), float(bc[3])))
+        self._np[k.value] = npn
+        return k.value
+
+    def add_problem(self, problem, *, tfinal=math.inf, max_steps=None, t0=0.0, device_setup=False):
+        """Admit a :class:`swe_amd.problems.Problem` with its boundary program; device_setup=True
+        assembles and lays the mesh out on the GPU from its element list (add_elements)."""
         hd, uxd, uyd = problem.dirichlet()
-        mesh = MeshArrays(problem.ia, problem.ja, problem.cij, problem.lumped, problem.bath, hd, uxd, uyd)
         prm = make_params(**problem.params)
-        return self.add(mesh, prm, problem.u0, t0=t0, tfinal=tfinal,
-                        max_steps=problem.nsteps if max_steps is None else max_steps, bc=problem.bc)
+        steps = problem.nsteps if max_steps is None else max_steps
+        if device_setup:
+            return self.add_elements(problem.mesh.jj, problem.mesh.rr, problem.bath, prm, problem.u0, h_dir=hd,
+                                     ux_dir=uxd, uy_dir=uyd, t0=t0, tfinal=tfinal, max_steps=steps, bc=problem.bc)
+        mesh = MeshArrays(problem.ia, problem.ja, problem.cij, problem.lumped, problem.bath, hd, uxd, uyd)
+        return self.add(mesh, prm, problem.u0, t0=t0, tfinal=tfinal, max_steps=steps, bc=problem.bc)
 
     def remove(self, k):
         _check(load_library().swe_batch_remove(self.h, int(k)))

@@ -53,11 +53,13 @@ class Request:
 class BatchEngine:
     """The serving loop's view of a :class:`swe_amd.batch.Batch`."""
 
-    def __init__(self, batch):
+    def __init__(self, batch, device_setup=False):
         self.batch = batch
+        self.device_setup = device_setup     # admit from the element list: set-up and layout on the GPU
 
     def add(self, req: Request):
-        return self.batch.add_problem(req.problem, tfinal=req.tfinal, max_steps=MAX_STEPS)
+        return self.batch.add_problem(req.problem, tfinal=req.tfinal, max_steps=MAX_STEPS,
+                                      device_setup=self.device_setup)
 
     def step(self, n):
         return self.batch.step_async(n)

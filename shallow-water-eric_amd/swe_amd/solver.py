@@ -97,6 +97,8 @@ SIGNATURES = {
     "swe_batch_create": (_I, [C.POINTER(BatchCaps), C.POINTER(_P)]),
     "swe_batch_destroy": (_I, [_P]),
     "swe_batch_add": (_I, [_P, C.POINTER(MeshDesc), C.POINTER(Params), _P, _D, _D, C.c_longlong, C.POINTER(_I)]),
+    "swe_batch_add_assembled": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, _P, C.POINTER(Params), _P, _D, _D,
+                                     C.c_longlong, C.POINTER(_I)]),
     "swe_batch_set_bc": (_I, [_P, _I, _I, _D, _D, _D]),
     "swe_batch_set_bc_values": (_I, [_P, _I, _P, _P, _P]),
     "swe_batch_remove": (_I, [_P, _I]),

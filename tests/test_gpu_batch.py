@@ -95,6 +95,24 @@ def test_admission_mid_run_and_removal():
             assert b.status(k)[2] == n
 
 
+def test_admission_from_elements_while_steps_run():
+    """swe_batch_add_assembled: meshes admitted from their element lists (construct_matrices and the
+    layout on the GPU, filled on the copy stream) while the steps queued for the others run; every
+    instance ends with the bits of its run alone."""
+    ps = fct_problems()
+    n = 30
+    alone = {i: run_alone(p, n) for i, p in enumerate(ps)}
+    with new_batch(ps) as b:
+        ks = {0: b.add_problem(ps[0], max_steps=n), 1: b.add_problem(ps[1], max_steps=n, device_setup=True)}
+        b.step_async(10)                               # queued; the host does not wait for it
+        ks[2] = b.add_problem(ps[2], max_steps=n, device_setup=True)
+        ks[3] = b.add_problem(ps[3], max_steps=n, device_setup=True)
+        b.step(n + 3)
+        for i, k in ks.items():
+            assert same_bits(b.get_state(k), alone[i][0]), i
+            assert b.status(k)[2] == n
+
+
 def test_final_time_stops_like_the_reference_loop():
     """DO WHILE (time < Tfinal) (SOURCES/main.f90:61): the instance takes steps while its time is
     below tfinal, so its last step ends at or beyond tfinal."""
