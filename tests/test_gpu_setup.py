@@ -104,3 +104,25 @@ def test_solver_from_device_assembly_rejects_bad_input():
         Solver.from_elements(p.mesh.jj, p.mesh.rr, p.bath, h_dir=[p.np], **p.params)    # node out of range
     with pytest.raises(SweError):
         Solver.from_elements(p.mesh.jj, p.mesh.rr, p.bath[:-1], **p.params)             # bath too short
+
+
+@pytest.mark.parametrize("n", [2, 3, 9])
+def test_solver_from_device_assembly_on_tiny_meshes(n):
+    """Meshes of 4, 9 and 81 nodes (one partial 64-row slice, rows of 3 to 7 entries): the device
+    layout and the host layout give the same bits over 10 first-order paraboloid steps."""
+    from swe_amd import mesh as M
+    from swe_amd import problems as PR
+    from swe_amd.solver import Solver, assemble
+    m = M.renumber_first_touch(M.structured_mesh(n, n, 0.0, 4.0, 0.0, 4.0))
+    htiny = 4.0e-4
+    bath = -0.1 * (1.0 - ((m.rr[:, 0] - 2.0) ** 2 + (m.rr[:, 1] - 2.0) ** 2))
+    u0 = PR.paraboloid_state(m.rr, 0.0, htiny)
+    kw = dict(gravity=9.81, htiny=htiny, cfl=0.25, max_water_h=4.0, viscosity="viscous", type_test=5)
+    ia, ja, _diag, _mass, lumped, cij = assemble(m.jj, m.rr)
+    out = []
+    for s in (Solver(ia, ja, cij, lumped, bath, **kw), Solver.from_elements(m.jj, m.rr, bath, **kw)):
+        s.set_state(u0)
+        t, dts = s.run(10, time=0.0)
+        out.append((s.get_state(), t, dts))
+        s.close()
+    assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][2], out[1][2]) and out[0][1] == out[1][1]
