@@ -18,6 +18,11 @@ MODULE swe_amd_c
      INTEGER(c_int)    :: type_test, viscosity_type, viscous_type, alpha_limit, roundoff_fix
   END TYPE swe_params
 
+  TYPE, BIND(C) :: swe_batch_caps
+     INTEGER(c_long_long) :: max_nodes, max_entries, max_boundary
+     INTEGER(c_int)       :: max_instances
+  END TYPE swe_batch_caps
+
   TYPE, BIND(C) :: swe_mesh_desc
      INTEGER(c_int)    :: np, nnz
      TYPE(c_ptr)       :: ia, ja, cx, cy, lumped, bath
@@ -108,6 +113,103 @@ MODULE swe_amd_c
        TYPE(c_ptr), VALUE :: errors
        INTEGER(c_int) :: rc
      END FUNCTION swe_errors_destroy
+
+     ! ---- batches: many meshes advanced together (include/swe_amd.h, swe_batch_*) ----
+     FUNCTION swe_batch_create(caps, batch) BIND(C, NAME='swe_batch_create') RESULT(rc)
+       IMPORT :: c_int, c_ptr, swe_batch_caps
+       TYPE(swe_batch_caps), INTENT(IN) :: caps
+       TYPE(c_ptr) :: batch
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_create
+
+     FUNCTION swe_batch_destroy(batch) BIND(C, NAME='swe_batch_destroy') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_destroy
+
+     FUNCTION swe_batch_add(batch, mesh, prm, un, t0, tfinal, max_steps, id) BIND(C, NAME='swe_batch_add') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double, c_long_long, swe_mesh_desc, swe_params
+       TYPE(c_ptr), VALUE :: batch
+       TYPE(swe_mesh_desc), INTENT(IN) :: mesh
+       TYPE(swe_params), INTENT(IN) :: prm
+       REAL(c_double), INTENT(IN) :: un(*)
+       REAL(c_double), VALUE :: t0, tfinal
+       INTEGER(c_long_long), VALUE :: max_steps
+       INTEGER(c_int) :: id
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_add
+
+     FUNCTION swe_batch_add_assembled(batch, assembly, bath, n_h_dir, h_dir, n_ux_dir, ux_dir, n_uy_dir, uy_dir, &
+          prm, un, t0, tfinal, max_steps, id) BIND(C, NAME='swe_batch_add_assembled') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double, c_long_long, swe_params
+       TYPE(c_ptr), VALUE :: batch, assembly
+       REAL(c_double), INTENT(IN) :: bath(*), un(*)
+       INTEGER(c_int), VALUE :: n_h_dir, n_ux_dir, n_uy_dir
+       INTEGER(c_int), INTENT(IN) :: h_dir(*), ux_dir(*), uy_dir(*)
+       TYPE(swe_params), INTENT(IN) :: prm
+       REAL(c_double), VALUE :: t0, tfinal
+       INTEGER(c_long_long), VALUE :: max_steps
+       INTEGER(c_int) :: id
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_add_assembled
+
+     FUNCTION swe_batch_set_bc(batch, id, kind, amp, period, level) BIND(C, NAME='swe_batch_set_bc') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int), VALUE :: id, kind
+       REAL(c_double), VALUE :: amp, period, level
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_set_bc
+
+     FUNCTION swe_batch_remove(batch, id) BIND(C, NAME='swe_batch_remove') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int), VALUE :: id
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_remove
+
+     FUNCTION swe_batch_step(batch, nsteps) BIND(C, NAME='swe_batch_step') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int), VALUE :: nsteps
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_step
+
+     FUNCTION swe_batch_sync(batch) BIND(C, NAME='swe_batch_sync') RESULT(rc)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_sync
+
+     FUNCTION swe_batch_status(batch, id, state, time, steps, error) BIND(C, NAME='swe_batch_status') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double, c_long_long
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int), VALUE :: id
+       INTEGER(c_int) :: state, error
+       REAL(c_double) :: time
+       INTEGER(c_long_long) :: steps
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_status
+
+     FUNCTION swe_batch_get_state(batch, id, un) BIND(C, NAME='swe_batch_get_state') RESULT(rc)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int), VALUE :: id
+       REAL(c_double), INTENT(OUT) :: un(*)
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_get_state
+
+     FUNCTION swe_batch_errors_create(batch, id, me, jj, rr, errors) BIND(C, NAME='swe_batch_errors_create') &
+          RESULT(rc)
+       IMPORT :: c_int, c_double, c_ptr
+       TYPE(c_ptr), VALUE :: batch
+       INTEGER(c_int), VALUE :: id, me
+       INTEGER(c_int), INTENT(IN) :: jj(*)
+       REAL(c_double), INTENT(IN) :: rr(*)
+       TYPE(c_ptr) :: errors
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_batch_errors_create
 
      FUNCTION swe_destroy(handle) BIND(C, NAME='swe_destroy') RESULT(rc)
        IMPORT :: c_int, c_ptr

@@ -2,14 +2,16 @@
 ! api_check.f90 -- the device set-up and compute_errors entry points called from Fortran through
 ! the interface blocks of shallow-water-eric_amd/fortran/swe_amd_c.f90, the way a reference driver
 ! calls them: construct_matrices on the device (swe_assemble_device + swe_create_from_assembly),
-! the time loop (swe_run), then compute_errors (swe_errors_create + swe_compute_errors).
+! the time loop (swe_run), then compute_errors (swe_errors_create + swe_compute_errors); then the same
+! mesh admitted twice into a batch (swe_batch_add_assembled), stepped, read back, and its errors.
 !
 ! Reads in.bin and writes out.bin in the current directory (raw little-endian streams):
 !   in:  int32 np, me, nh, nux, nuy, nsteps, bc_kind; real8 bc_amp, bc_period, bc_level;
 !        real8 gravity, htiny, cfl, max_water_h, mannings; int32 type_test, viscosity_type,
 !        viscous_type, alpha_limit, roundoff_fix; int32 jj(3,me) (1-based); real8 rr(2,np),
 !        bath(np), un(3,np); int32 h_dir(nh), ux_dir(nux), uy_dir(nuy) (1-based)
-!   out: real8 un(3,np), time, dts(nsteps), errors(12)
+!   out: real8 un(3,np), time, dts(nsteps), errors(12), then for the two batch instances
+!        un(3,np) each, time each, the steps each (as real8) and the first instance's errors(12)
 !==============================================================================================
 PROGRAM api_check
   USE iso_c_binding
@@ -21,8 +23,12 @@ PROGRAM api_check
   INTEGER(c_int), ALLOCATABLE :: jj(:,:), hd(:), uxd(:), uyd(:)
   REAL(c_double), ALLOCATABLE :: rr(:,:), bath(:), un(:,:), dts(:)
   TYPE(swe_params) :: prm
-  TYPE(c_ptr) :: asm, h, errs
-  INTEGER(c_long_long) :: nnz
+  TYPE(c_ptr) :: asm, h, errs, b
+  INTEGER(c_long_long) :: nnz, nst(2)
+  TYPE(swe_batch_caps) :: caps
+  INTEGER(c_int) :: ids(2), st, er, q
+  REAL(c_double) :: tb(2), e12b(12)
+  REAL(c_double), ALLOCATABLE :: u0(:,:), ub(:,:,:)
 
   OPEN(10, FILE='in.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
   READ(10) np, me, nh, nux, nuy, nsteps, bc_kind
@@ -30,6 +36,8 @@ PROGRAM api_check
   ALLOCATE(jj(3,me), rr(2,np), bath(np), un(3,np), hd(MAX(nh,1)), uxd(MAX(nux,1)), uyd(MAX(nuy,1)), &
        dts(MAX(nsteps,1)))
   READ(10) jj, rr, bath, un
+  ALLOCATE(u0(3,np), ub(3,np,2))
+  u0 = un
   IF (nh > 0) READ(10) hd(1:nh)
   IF (nux > 0) READ(10) uxd(1:nux)
   IF (nuy > 0) READ(10) uyd(1:nuy)
@@ -60,8 +68,33 @@ PROGRAM api_check
   WRITE(*,*) ' Relative L1 error on h (Gaussian)', e12(1)/e12(2)
   WRITE(*,*) ' Relative L2 error on h (Gaussian)', e12(3)/e12(4)
 
+  ! the same mesh twice in one batch, each instance stopping after nsteps steps
+  caps%max_nodes = 2 * (np + 64)
+  caps%max_entries = 4 * nnz + 8192
+  caps%max_boundary = 2 * (nh + nux + nuy) + 64
+  caps%max_instances = 2
+  CALL check(swe_batch_create(caps, b), 'swe_batch_create')
+  CALL check(swe_assemble_device(np, me, jj, rr, asm, nnz), 'swe_assemble_device')
+  DO q = 1, 2
+     CALL check(swe_batch_add_assembled(b, asm, bath, nh, hd, nux, uxd, nuy, uyd, prm, u0, 0.d0, HUGE(1.d0), &
+          INT(nsteps, c_long_long), ids(q)), 'swe_batch_add_assembled')
+     CALL check(swe_batch_set_bc(b, ids(q), bc_kind, bcr(1), bcr(2), bcr(3)), 'swe_batch_set_bc')
+  END DO
+  CALL check(swe_assembly_destroy(asm), 'swe_assembly_destroy')
+  CALL check(swe_batch_step(b, nsteps + 2), 'swe_batch_step')
+  CALL check(swe_batch_sync(b), 'swe_batch_sync')
+  DO q = 1, 2
+     CALL check(swe_batch_get_state(b, ids(q), ub(:,:,q)), 'swe_batch_get_state')
+     CALL check(swe_batch_status(b, ids(q), st, tb(q), nst(q), er), 'swe_batch_status')
+  END DO
+  CALL check(swe_batch_errors_create(b, ids(1), me, jj, rr, errs), 'swe_batch_errors_create')
+  CALL check(swe_compute_errors(errs, tb(1), e12b), 'swe_compute_errors')
+  CALL check(swe_errors_destroy(errs), 'swe_errors_destroy')
+  CALL check(swe_batch_destroy(b), 'swe_batch_destroy')
+
   OPEN(11, FILE='out.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE')
   WRITE(11) un, t, dts(1:nsteps), e12
+  WRITE(11) ub, tb, REAL(nst, c_double), e12b
   CLOSE(11)
 
 CONTAINS

@@ -56,7 +56,16 @@ def test_fortran_calls_equal_python_calls(tmp_path, case):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     out = np.fromfile(str(tmp_path / "out.bin"), "<f8")
     npn = p.np
-    fu, ft, fd, fe = out[:3 * npn].reshape(npn, 3), out[3 * npn], out[3 * npn + 1:3 * npn + 1 + n], out[-12:]
+    m = 3 * npn + 1 + n + 12
+    fu, ft, fd, fe = out[:3 * npn].reshape(npn, 3), out[3 * npn], out[3 * npn + 1:3 * npn + 1 + n], out[m - 12:m]
     assert np.array_equal(fu, u) and ft == t and np.array_equal(fd, dts)
     assert np.array_equal(fe, errs)
     assert "Relative L1 error on h" in r.stdout
+    # the batch part: two instances of the same mesh, each the solver's bits
+    rest = out[m:]
+    ub = rest[:6 * npn].reshape(2, npn, 3)
+    tb, nst, eb = rest[6 * npn:6 * npn + 2], rest[6 * npn + 2:6 * npn + 4], rest[6 * npn + 4:]
+    assert eb.size == 12
+    assert np.array_equal(ub[0], u) and np.array_equal(ub[1], u)
+    assert np.all(tb == t) and np.all(nst == n)
+    assert np.array_equal(eb, errs)
