@@ -9,8 +9,8 @@
  *            SOURCES/condlim.f90:21-90,425-486,879-981, SOURCES/main.f90:61-92,166-188
  *   limiter  SOURCES/SAFE/fct.f90:3-43 (FCT_generic, the two-sided limiter the reference keeps but
  *            does not call; pinned by its invariants, see tests/test_oracle_math.py)
- *   errors   SOURCES/main.f90:198-263,303-322 (compute_errors, r_gauss), SOURCES/condlim.f90:385-456,
- *            649-750 (sol_anal of tests 4, 5, 11, 12), SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1513-1553,
+ *   errors   SOURCES/main.f90:198-263,303-322 (compute_errors, r_gauss), SOURCES/condlim.f90:346-605,
+ *            649-750 (sol_anal of tests 3, 4, 5, 6, 7, 11, 12), SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1513-1553,
  *            1679-1716 (ns_anal_0, ns_anal_l1); pinned by tests/golden/errors.npz (the reference's
  *            own routines run by oracle/ref_driver.f90)
  *
@@ -1062,9 +1062,83 @@ int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
 /* sol_anal (condlim.f90:311-831) at one point, in the reference's evaluation order: Fortran's
  * unary minus applies to the whole product that follows it, X**2 is X*X, and the integer powers of
  * the test constants are exact. */
+/* The bump of tests 2, 3, 6, 7: MAX(0, 0.2*(x-8)**3*(12-x)**3/64) (condlim.f90:355 and the same
+ * expression in each of those cases). */
+static double bump_z(double x)
+{
+    const double a = x - 8.0, c = 12.0 - x;
+    return fmaxF(0.0, 0.2 * (a * a * a) * (c * c * c) / 64.0);
+}
+
+/* The real roots of the cubic of the steady bump flows (condlim.f90:364-370 and 480-505): R, Q,
+ * D = Q**3 + R**2, and for D < 0 the three trigonometric roots. */
+typedef struct { double R, Q, D; } orc_cubic;
+static orc_cubic cubic_of(double b, double d)
+{
+    orc_cubic c;
+    c.R = (-(27.0 * d) - 2.0 * (b * b * b)) / 54.0;
+    c.Q = -((b * b) / 9.0);
+    c.D = c.Q * c.Q * c.Q + c.R * c.R;
+    return c;
+}
+
 double orc_sol_anal(int type_test, int k, double x, double y, double t, double gravity, double htiny, int *ok)
 {
     *ok = 1;
+    if (type_test == 3) {            /* condlim.f90:346-384 */
+        const double q0 = 4.42, hL = 2.0;
+        if (k == 1) {
+            const double bathi = bump_z(x);
+            if (t <= 1.0e-14) return hL - bathi;
+            const double b = bathi - (q0 * q0) / (2.0 * gravity * (hL * hL)) - hL;
+            const double d = (q0 * q0) / (2.0 * gravity);
+            const orc_cubic c = cubic_of(b, d);
+            const double theta = acos(c.R / sqrt(-(c.Q * c.Q * c.Q)));
+            return 2.0 * sqrt(-c.Q) * cos(theta / 3.0) - b / 3.0;
+        }
+        if (k == 2) return (t <= 1.0e-14) ? 0.0 : q0;
+        return 0.0;
+    }
+    if (type_test == 6 || type_test == 7) {   /* condlim.f90:457-521, 522-605 */
+        const double x0 = 10.0;
+        const double q0 = (type_test == 6) ? 1.53 : 0.18;
+        const double tpio3 = 2.0 * acos(-1.0) / 3.0, fpio3 = 2.0 * tpio3;
+        const double d = (q0 * q0) / (2.0 * gravity);
+        const double eta = -pow(27.0 * d / 4.0, 1.0 / 3.0);
+        const double ber = 0.2 - eta;               /* bernoulli (6), Ber_Pre (7) */
+        double xshock = 0.0, bath_shock = 0.0, h_post = 0.0;
+        if (type_test == 7) {
+            xshock = 11.7;
+            if (8.0 < xshock && xshock < 12.0) bath_shock = bump_z(xshock);
+            const orc_cubic c = cubic_of(bath_shock - ber, d);
+            if (!(c.D >= 0.0)) {
+                const double b = bath_shock - ber;
+                const double theta = acos(c.R / sqrt(-(c.Q * c.Q * c.Q)));
+                const double x3 = 2.0 * sqrt(-c.Q) * cos(theta / 3.0 + fpio3) - b / 3.0;
+                h_post = (-x3 + sqrt(x3 * x3 + 8.0 * (q0 * q0) / (gravity * x3))) / 2.0;
+            }
+        }
+        if (k == 1) {
+            const double hL = 0.66;
+            const double bathi = bump_z(x);
+            if (t <= 1.0e-14) return (type_test == 6) ? hL - bathi : h_post - bathi;
+            const double b = bathi - ber;
+            const orc_cubic c = cubic_of(b, d);
+            if (c.D >= 0.0) {
+                const double S = -pow(-sqrt(c.D) - c.R, 1.0 / 3.0);
+                const double T = -pow(sqrt(c.D) - c.R, 1.0 / 3.0);
+                return -((S + T) / 2.0) - b / 3.0;
+            }
+            const double theta = acos(c.R / sqrt(-(c.Q * c.Q * c.Q)));
+            const double x1 = 2.0 * sqrt(-c.Q) * cos(theta / 3.0) - b / 3.0;
+            const double x3 = 2.0 * sqrt(-c.Q) * cos(theta / 3.0 + fpio3) - b / 3.0;
+            if (x < x0) return (x1 > x3) ? x1 : x3;
+            if (type_test == 6 || x < xshock) return (x1 < x3) ? x1 : x3;
+            return bath_shock + h_post - bathi;
+        }
+        if (k == 2) return (type_test == 6 && t <= 1.0e-14) ? 0.0 : q0;
+        return 0.0;
+    }
     if (type_test == 4) {            /* condlim.f90:385-428 */
         const double x0 = 5.0, hL = 0.005;
         const double speed = sqrt(gravity * hL);

@@ -68,7 +68,7 @@ int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
 /* sin(x) rounded to nearest (double-double evaluation, one final rounding), |x| < 2^29. */
 double orc_sin_rn(double x);
 
-/* sol_anal(k, (x, y), t) (SOURCES/condlim.f90:311-831) for the analytic tests 4, 5, 11 and 12,
+/* sol_anal(k, (x, y), t) (SOURCES/condlim.f90:311-831) for the analytic tests 3 to 7, 11 and 12,
  * components k = 1 (h), 2 (qx), 3 (qy); *ok = 0 for other tests. */
 double orc_sol_anal(int type_test, int k, double x, double y, double t, double gravity, double htiny, int *ok);
 

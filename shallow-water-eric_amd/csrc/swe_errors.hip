@@ -1,11 +1,11 @@
 // swe_errors.hip -- compute_errors (SOURCES/main.f90:198-263) on the device state: sol_anal
-// (SOURCES/condlim.f90:311-831, analytic tests 4, 5, 11 and 12) at the Gauss points of r_gauss
+// (SOURCES/condlim.f90:311-831, analytic tests 3 to 7, 11 and 12) at the Gauss points of r_gauss
 // (main.f90:303-322), and the quadrature sums of ns_anal_l1 / ns_anal_0
 // (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1679-1716, 1513-1553) of u(k,:) - exact and 0 - exact.
 //
 // Every summand is computed with the reference's expressions (the file is compiled without FMA
 // contraction; the factors of sol_anal that depend on t only are evaluated once on the host with the
-// C library's cos, sin and exp, as the reference's runtime does).  The reference sums the summands
+// C library's cos, sin, exp, acos and pow, as the reference's runtime does).  The reference sums the summands
 // sequentially in element order; here each sum is accumulated in double-double arithmetic (error-free
 // two-sums) over a fixed partition of the elements and reduced in a fixed order, which gives the
 // exactly rounded sum up to a relative 2^-100 or so, the same on every run.  It differs from the
@@ -34,7 +34,17 @@ struct Anal {
     double k05 = 0.0, cs = 0.0, sn = 0.0, cm = 0.0, cp = 0.0;                   // test 5 (429-456)
     double A1 = 0.0, cB = 0.0, P1 = 0.0, P2 = 0.0, gam = 0.0, E2 = 0.0;           // test 11 (649-700)
     double cT1 = 0.0, cT2 = 0.0, T3 = 0.0, delta = 0.0, eta = 0.0, w = 0.0;       // test 12 (701-750)
+    // the steady flows over the bump, tests 3 (346-384), 6 (457-521) and 7 (522-605)
+    double q0 = 0.0, b0 = 0.0, d = 0.0, ber = 0.0, fpio3 = 0.0, xshock = 0.0, bath_shock = 0.0, h_post = 0.0;
 };
+
+// The bump of tests 2, 3, 6, 7: MAX(0, 0.2*(x-8)**3*(12-x)**3/64) (condlim.f90:355).
+__host__ __device__ inline double bump_of(double x) {
+    const double a = x - 8.0, c = 12.0 - x;
+    const double v = 0.2 * (a * a * a) * (c * c * c) / 64.0;
+    return (0.0 > v) ? 0.0 : v;
+}
+inline double bump_host(double x) { return bump_of(x); }
 
 int make_anal(int test, double t, double g, double htiny, Anal& A) {
     A = Anal{};
@@ -51,6 +61,37 @@ int make_anal(int test, double t, double g, double htiny, Anal& A) {
         A.two_t = 2.0 * t;
         A.c49 = 4.0 / (9.0 * g);
         A.c827 = 8.0 / (27.0 * g);
+        return SWE_OK;
+    }
+    if (test == 3) {
+        A.small_t = (t <= 1.0e-14) ? 1 : 0;
+        A.q0 = 4.42;
+        A.hL = 2.0;
+        A.b0 = (A.q0 * A.q0) / (2.0 * g * (A.hL * A.hL));
+        A.d = (A.q0 * A.q0) / (2.0 * g);
+        return SWE_OK;
+    }
+    if (test == 6 || test == 7) {
+        A.small_t = (t <= 1.0e-14) ? 1 : 0;
+        A.x0 = 10.0;
+        A.hL = 0.66;
+        A.q0 = (test == 6) ? 1.53 : 0.18;
+        const double tpio3 = 2.0 * std::acos(-1.0) / 3.0;
+        A.fpio3 = 2.0 * tpio3;
+        A.d = (A.q0 * A.q0) / (2.0 * g);
+        const double eta = -std::pow(27.0 * A.d / 4.0, 1.0 / 3.0);
+        A.ber = 0.2 - eta;                                        // bernoulli (test 6), Ber_Pre (test 7)
+        if (test == 7) {
+            A.xshock = 11.7;
+            if (8.0 < A.xshock && A.xshock < 12.0) A.bath_shock = bump_host(A.xshock);
+            const double b = A.bath_shock - A.ber;
+            const double R = (-(27.0 * A.d) - 2.0 * (b * b * b)) / 54.0, Q = -((b * b) / 9.0);
+            if (!(Q * Q * Q + R * R >= 0.0)) {
+                const double theta = std::acos(R / std::sqrt(-(Q * Q * Q)));
+                const double x3 = 2.0 * std::sqrt(-Q) * std::cos(theta / 3.0 + A.fpio3) - b / 3.0;
+                A.h_post = (-x3 + std::sqrt(x3 * x3 + 8.0 * (A.q0 * A.q0) / (g * x3))) / 2.0;
+            }
+        }
         return SWE_OK;
     }
     if (test == 5) {
@@ -101,14 +142,46 @@ int make_anal(int test, double t, double g, double htiny, Anal& A) {
         A.eta = eta;
         return SWE_OK;
     }
-    return fail(SWE_EBADOPT, "compute_errors: no analytic solution for type_test %d (tests 4, 5, 11 and 12 have one)",
+    return fail(SWE_EBADOPT, "compute_errors: no analytic solution for type_test %d (tests 3 to 7, 11 and 12 have one)",
                 test);
 }
 
 __device__ __forceinline__ double fmaxF(double a, double b) { return (a > b) ? a : b; }
 
-// sol_anal(1..3) at one point, given the t-dependent factors.
+// sol_anal(1..3) at one point, given the t-dependent factors.  The steady bump flows take the roots
+// of a cubic per point with the device's acos, cos and pow, which may differ from the host C
+// library's in the last bit.
 __device__ __forceinline__ void sol_anal3(const Anal& A, double x, double y, double v[3]) {
+    if (A.test == 3 || A.test == 6 || A.test == 7) {
+        v[2] = 0.0;
+        const double bathi = bump_of(x);
+        if (A.small_t) {
+            v[0] = ((A.test == 7) ? A.h_post : A.hL) - bathi;
+            v[1] = (A.test == 7) ? A.q0 : 0.0;
+            return;
+        }
+        v[1] = A.q0;
+        const double b = (A.test == 3) ? bathi - A.b0 - A.hL : bathi - A.ber;
+        const double R = (-(27.0 * A.d) - 2.0 * (b * b * b)) / 54.0, Q = -((b * b) / 9.0);
+        const double D = Q * Q * Q + R * R;
+        if (A.test != 3 && D >= 0.0) {
+            const double S = -pow(-sqrt(D) - R, 1.0 / 3.0);
+            const double T = -pow(sqrt(D) - R, 1.0 / 3.0);
+            v[0] = -((S + T) / 2.0) - b / 3.0;
+            return;
+        }
+        const double theta = acos(R / sqrt(-(Q * Q * Q)));
+        const double x1 = 2.0 * sqrt(-Q) * cos(theta / 3.0) - b / 3.0;
+        if (A.test == 3) {
+            v[0] = x1;
+            return;
+        }
+        const double x3 = 2.0 * sqrt(-Q) * cos(theta / 3.0 + A.fpio3) - b / 3.0;
+        if (x < A.x0) v[0] = (x1 > x3) ? x1 : x3;
+        else if (A.test == 6 || x < A.xshock) v[0] = (x1 < x3) ? x1 : x3;
+        else v[0] = A.bath_shock + A.h_post - bathi;
+        return;
+    }
     if (A.test == 4) {
         v[2] = 0.0;
         if (A.small_t) {

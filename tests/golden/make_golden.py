@@ -18,6 +18,7 @@ links the reference modules) and the outputs it wrote.  Cases:
   par_friction.npz    test 11 (linear friction) on a 41x41 grid of the paraboloid, viscous + alpha
   errors.npz          the quadrature norms of compute_errors (SOURCES/main.f90:198-263) on states of the
                       fixtures above (tests 4, 5, 11, 12), by the reference's sol_anal, ns_anal_l1, ns_anal_0
+  errors_bump.npz     the same norms for the steady bump flows (tests 3, 6, 7) on a [0,25]x[0,1] strip
 """
 from __future__ import annotations
 
@@ -337,6 +338,20 @@ def gen_errors():
         out["%s|final|time" % name] = errs(case, z["final"], float(z["time"]))
         case.close()
     save("errors.npz", **out)
+    # the steady flows over a bump (tests 3, 6, 7) on a strip of [0,25]x[0,1], at their initial states,
+    # at t = 0 and at t = 1 (the analytic steady state)
+    raw = M.structured_mesh(101, 5, 0.0, 25.0, 0.0, 1.0)
+    bump = {}
+    for test in (3, 6, 7):
+        case = RefCase(raw, test=test, viscosity="viscous")
+        S = case.setup()
+        bump["jj"], bump["rr"] = S["jj"], S["rr"]
+        bump["u0_%d" % test] = S["u0"]
+        bump["scalars_%d" % test] = np.array([S["gravity"], S["htiny"]])
+        for t in (0.0, 1.0):
+            bump["errors_%d_t%d" % (test, int(t))] = errs(case, S["u0"], t)
+        case.close()
+    save("errors_bump.npz", **bump)
 
 
 def main():

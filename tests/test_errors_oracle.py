@@ -69,3 +69,13 @@ def test_error_of_the_exact_field_vanishes(oracle):
     assert out[0, 0] <= 8 * 2.0 ** -53 * out[0, 1] and out[0, 2] <= 8 * 2.0 ** -53 * out[0, 3]
     assert abs(out[0, 1] - 0.005 * 16.0) <= 1e-15 and abs(out[0, 3] - 0.005 * 4.0) <= 1e-15
     assert np.all(out[1:, :] == 0.0)
+
+
+@pytest.mark.parametrize("test", [3, 6, 7])
+def test_oracle_bump_flow_errors_are_the_references(oracle, test):
+    """Tests 3, 6, 7 (steady flows over the bump: roots of a cubic per point) at t = 0 and t = 1."""
+    z = golden("errors_bump.npz")
+    g, htiny = z["scalars_%d" % test]
+    for t in (0, 1):
+        out = oracle.compute_errors(z["jj"], z["rr"], z["u0_%d" % test], test, float(t), g, htiny)
+        assert np.array_equal(out, z["errors_%d_t%d" % (test, t)]), t

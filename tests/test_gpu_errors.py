@@ -133,3 +133,28 @@ def test_batch_instance_errors_equal_solver_alone():
         with pytest.raises(SweError):
             evd.sums(1.0)
         evd.close()
+
+
+@pytest.mark.parametrize("test", [3, 6, 7])
+def test_device_bump_flow_errors(oracle, test):
+    """Tests 3, 6, 7 on the reference's strip fixture: at t = 0 the summands are bitwise the oracle's
+    (2 ulp from math.fsum); at t = 1 the device's acos/cos/pow may move a summand by an ulp, so the sums
+    are held to 1e-12 relative of the reference's."""
+    from swe_amd.solver import Solver, assemble
+    z = golden("errors_bump.npz")
+    g, htiny = z["scalars_%d" % test]
+    ia, ja, _d, _m, lumped, cij = assemble(z["jj"], z["rr"])
+    s = Solver(ia, ja, cij, lumped, np.zeros(z["rr"].shape[0]), gravity=g, htiny=htiny, cfl=0.25,
+               max_water_h=2.0, viscosity="viscous", type_test=test)
+    s.set_state(z["u0_%d" % test])
+    ev = s.error_evaluator(z["jj"], z["rr"])
+    for t in (0, 1):
+        dev = ev.sums(float(t))
+        ref = z["errors_%d_t%d" % (test, t)]
+        if t == 0:
+            out, terms = oracle.compute_errors(z["jj"], z["rr"], z["u0_%d" % test], test, 0.0, g, htiny, terms=True)
+            _check(dev, out, terms, ref)
+        else:
+            assert np.all(np.abs(dev - ref) <= 1e-12 * np.maximum(np.abs(ref), 1e-300)), (dev, ref)
+    ev.close()
+    s.close()
