@@ -289,13 +289,13 @@ std::vector<int> rcm_order(const swe_mesh_desc& d) {
 // SWE_REORDER: 0 keeps the caller's order, 2 always renumbers, otherwise (default) the rows are
 // renumbered by reverse Cuthill-McKee when the caller's numbering scatters neighbours (mean offset
 // above 4*sqrt(np)) and the new order is better.
+}  // namespace
+
 int reorder_mode() {
     const char* v = std::getenv("SWE_REORDER");
     const int m = v ? std::atoi(v) : 1;
     return (m >= 0 && m <= 2) ? m : 1;
 }
-
-}  // namespace
 
 // Internal numbering: a base order of the rows (the caller's, or reverse Cuthill-McKee when the
 // caller's numbering scatters neighbours), then inside consecutive windows of `sort_window` rows

@@ -67,6 +67,9 @@ int validate_mesh(const swe_mesh_desc& d);
 int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L);
 // Rows sorted by length inside windows of this many rows: SWE_SORT_WINDOW, else 64.
 int sort_window_default();
+// SWE_REORDER: 0 keeps the caller's numbering as the base order, 1 (default) renumbers for locality
+// when the caller's numbering scatters neighbours (mean |i-j| above 4 sqrt(np)), 2 always.
+int reorder_mode();
 int make_consts(const swe_params& p, Consts& K);
 
 // ---- the pool ---------------------------------------------------------------------------------

@@ -13,8 +13,10 @@
 // A node may belong to at most kMaxElems elements (rows of at most 3*kMaxElems candidates);
 // larger fans are reported as SWE_EINVAL and can be assembled on the host.
 #include <hip/hip_runtime.h>
+#include <hipcub/device/device_radix_sort.hpp>
 #include <hipcub/device/device_scan.hpp>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 
@@ -173,6 +175,7 @@ struct swe_assembly {
     int* ja = nullptr;
     int* diag = nullptr;
     double *mass = nullptr, *lumped = nullptr, *cx = nullptr, *cy = nullptr;
+    double* rr = nullptr;     // node coordinates rr(2,np), kept for the locality order of the layout
 };
 
 namespace {
@@ -191,7 +194,7 @@ void dfree(void* p, hipStream_t s) {
 
 void release(swe_assembly* a) {
     if (!a) return;
-    void* bufs[] = {a->ia0, a->ja, a->diag, a->mass, a->lumped, a->cx, a->cy};
+    void* bufs[] = {a->ia0, a->ja, a->diag, a->mass, a->lumped, a->cx, a->cy, a->rr};
     for (void* b : bufs) dfree(b, a->stream);
     if (a->stream) {
         (void)hipStreamSynchronize(a->stream);
@@ -282,7 +285,8 @@ extern "C" int swe_assemble_device(int np, int me, const int* jj, const double* 
             if (e == hipSuccess) e = hipStreamSynchronize(s);
         }
     }
-    void* tmp[] = {djj, drr, elem, fill, flag, cnt, start};
+    a->rr = drr;
+    void* tmp[] = {djj, elem, fill, flag, cnt, start};
     for (void* b : tmp) dfree(b, s);
     if (e != hipSuccess) rc = swe::fail(SWE_EHIP, "device assembly: %s", hipGetErrorString(e));
     if (rc != SWE_OK) {
@@ -340,12 +344,15 @@ __global__ void k_lay_len(int np, const int64_t* __restrict__ ia0, int* __restri
     if (l > swe::kMaxRow || l < 1) atomicOr(bad, 1);
 }
 
-// One wave per window: each lane's rank among the window's valid rows by (length, position).
-__global__ void k_lay_sort(int np, int nwin, const int* __restrict__ lenc, int* __restrict__ perm) {
+// One wave per window of the base order (base[p], or p when base is null): each lane's rank among
+// the window's valid rows by (length, position in the base order).
+__global__ void k_lay_sort(int np, int nwin, const int* __restrict__ lenc, const int* __restrict__ base,
+                           int* __restrict__ perm) {
     const int w = (int)(blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
-    const int i = w * 64 + lane;
-    const bool valid = w < nwin && i < np;
+    const int pos = w * 64 + lane;
+    const bool valid = w < nwin && pos < np;
+    const int i = valid ? (base ? base[pos] : pos) : 0;
     const int li = valid ? lenc[i] : 0x7fffffff;
     int rank = 0;
     for (int m = 0; m < 64; ++m) {
@@ -354,6 +361,88 @@ __global__ void k_lay_sort(int np, int nwin, const int* __restrict__ lenc, int* 
         rank += (vm && (lm < li || (lm == li && m < lane))) ? 1 : 0;
     }
     if (valid) perm[w * 64 + rank] = i;
+}
+
+// ---- locality order: nodes sorted along a Morton (Z-order) curve of their coordinates ----------
+// Orderable image of a double: unsigned comparison of the images orders finite doubles.
+__device__ __forceinline__ unsigned long long ord_of(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+double double_of_ord(unsigned long long o) {
+    const unsigned long long b = (o >> 63) ? (o & ~0x8000000000000000ull) : ~o;
+    double v;
+    std::memcpy(&v, &b, sizeof v);
+    return v;
+}
+
+// box = {min x, max x, min y, max y} as orderable images (initialised to ~0, 0, ~0, 0).
+__global__ void k_bbox(int np, const double* __restrict__ rr, unsigned long long* __restrict__ box) {
+    unsigned long long v[4] = {~0ull, 0ull, ~0ull, 0ull};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long x = ord_of(rr[2 * i]), y = ord_of(rr[2 * i + 1]);
+        v[0] = (x < v[0]) ? x : v[0];
+        v[1] = (x > v[1]) ? x : v[1];
+        v[2] = (y < v[2]) ? y : v[2];
+        v[3] = (y > v[3]) ? y : v[3];
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 4; ++q) {
+            const unsigned long long o = __shfl_xor(v[q], off, 64);
+            v[q] = (q & 1) ? ((o > v[q]) ? o : v[q]) : ((o < v[q]) ? o : v[q]);
+        }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(box + 0, v[0]);
+        atomicMax(box + 1, v[1]);
+        atomicMin(box + 2, v[2]);
+        atomicMax(box + 3, v[3]);
+    }
+}
+
+__device__ __forceinline__ unsigned long long spread_bits(unsigned long long x) {
+    x &= 0xffffffffull;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+constexpr int kMortonBits = 20;   // per coordinate: 2^20 cells along each side of the bounding box
+
+__global__ void k_morton(int np, const double* __restrict__ rr, double x0, double sx, double y0, double sy,
+                         unsigned long long* __restrict__ key, int* __restrict__ idx) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= np) return;
+    const double top = (double)((1 << kMortonBits) - 1);
+    double fx = (rr[2 * (size_t)i] - x0) * sx, fy = (rr[2 * (size_t)i + 1] - y0) * sy;
+    fx = (fx > 0.0) ? ((fx < top) ? fx : top) : 0.0;
+    fy = (fy > 0.0) ? ((fy < top) ? fy : top) : 0.0;
+    key[i] = spread_bits((unsigned long long)fx) | (spread_bits((unsigned long long)fy) << 1);
+    idx[i] = i;
+}
+
+__global__ void k_inverse(int np, const int* __restrict__ order, int* __restrict__ pos) {
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p < np) pos[order[p]] = p;
+}
+
+// Sum over the off-diagonal entries of |pos(i) - pos(j)| (pos null: the caller's numbering).
+__global__ void k_offset_sum(int np, const int64_t* __restrict__ ia0, const int* __restrict__ ja,
+                             const int* __restrict__ pos, unsigned long long* __restrict__ sum) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    unsigned long long acc = 0;
+    if (i < np) {
+        const int pi = pos ? pos[i] : i;
+        for (int64_t p = ia0[i]; p < ia0[i + 1]; ++p) {
+            const int j = ja[p] - 1;
+            const int pj = pos ? pos[j] : j;
+            acc += (unsigned long long)((pi > pj) ? pi - pj : pj - pi);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(sum, acc);
 }
 
 __global__ void k_lay_iperm(int np, int np_pad, const int* __restrict__ perm, const int* __restrict__ lenc,
@@ -482,6 +571,82 @@ __global__ void k_lay_bc(int nbc, int nh, int nux, int64_t r0, int64_t b0, int i
 
 }  // namespace
 
+namespace {
+
+hipError_t read_u64(const unsigned long long* d, unsigned long long* h, int n, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(h, d, sizeof(unsigned long long) * (size_t)n, hipMemcpyDeviceToHost, s);
+    return (e == hipSuccess) ? hipStreamSynchronize(s) : e;
+}
+
+// The base order of the layout.  *base stays null for the caller's numbering; otherwise it lists the
+// nodes along a Morton (Z-order) curve of their coordinates, the device counterpart of the host
+// layout's reverse Cuthill-McKee order, under the same SWE_REORDER policy: 1 when the caller's
+// numbering scatters neighbours (mean |i-j| above 4 sqrt(np)) and the curve brings them closer, 2
+// always.  Only the processing order changes; results do not depend on it.
+hipError_t locality_order(const swe_assembly* a, hipStream_t s, int** base) {
+    *base = nullptr;
+    const int mode = swe::reorder_mode();
+    const int np = a->np;
+    if (mode == 0 || np < 2 || a->nnz <= np) return hipSuccess;
+    unsigned long long* acc = nullptr;   // [0] offset sum, caller's order; [1] curve order; [2..5] box
+    unsigned long long* keys = nullptr;
+    int *idx = nullptr, *order = nullptr, *pos = nullptr;
+    unsigned long long* keys2 = nullptr;
+    void* tmp = nullptr;
+    unsigned long long h[6] = {0, 0, ~0ull, 0ull, ~0ull, 0ull};
+    const unsigned bl = (unsigned)((np + 255) / 256);
+    const double offdiag = (double)(a->nnz - np);
+    hipError_t e = dalloc(&acc, 6, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(acc, h, sizeof h, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_offset_sum, dim3(bl), dim3(256), 0, s, np, (const int64_t*)a->ia0, (const int*)a->ja,
+                           (const int*)nullptr, acc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = read_u64(acc, h, 1, s);
+    const double mean0 = (double)h[0] / offdiag;
+    bool take = false;
+    if (e == hipSuccess && (mode == 2 || mean0 > 4.0 * std::sqrt((double)np))) {
+        hipLaunchKernelGGL(k_bbox, dim3(bl < 1024u ? bl : 1024u), dim3(256), 0, s, np, (const double*)a->rr, acc + 2);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = read_u64(acc + 2, h + 2, 4, s);
+        const double x0 = double_of_ord(h[2]), x1 = double_of_ord(h[3]);
+        const double y0 = double_of_ord(h[4]), y1 = double_of_ord(h[5]);
+        const double top = (double)((1 << kMortonBits) - 1);
+        const double sx = (x1 > x0) ? top / (x1 - x0) : 0.0, sy = (y1 > y0) ? top / (y1 - y0) : 0.0;
+        if (e == hipSuccess) e = dalloc(&keys, (size_t)np, s);
+        if (e == hipSuccess) e = dalloc(&keys2, (size_t)np, s);
+        if (e == hipSuccess) e = dalloc(&idx, (size_t)np, s);
+        if (e == hipSuccess) e = dalloc(&order, (size_t)np, s);
+        if (e == hipSuccess) e = dalloc(&pos, (size_t)np, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_morton, dim3(bl), dim3(256), 0, s, np, (const double*)a->rr, x0, sx, y0, sy, keys, idx);
+            e = hipGetLastError();
+        }
+        size_t bytes = 0;
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys, keys2, idx, order, np, 0, 2 * kMortonBits, s);
+        if (e == hipSuccess) e = hipMallocAsync(&tmp, bytes ? bytes : 1, s);
+        if (e == hipSuccess)   // a radix sort is stable: equal keys keep the caller's order
+            e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys2, idx, order, np, 0, 2 * kMortonBits, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_inverse, dim3(bl), dim3(256), 0, s, np, (const int*)order, pos);
+            hipLaunchKernelGGL(k_offset_sum, dim3(bl), dim3(256), 0, s, np, (const int64_t*)a->ia0,
+                               (const int*)a->ja, (const int*)pos, acc + 1);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = read_u64(acc + 1, h + 1, 1, s);
+        take = e == hipSuccess && (mode == 2 || h[1] < h[0]);
+    }
+    void* bufs[] = {acc, keys, keys2, idx, pos, tmp};
+    for (void* b : bufs) dfree(b, s);
+    if (take) *base = order;
+    else dfree(order, s);
+    return e;
+}
+
+}  // namespace
+
 namespace swe {
 
 void free_device_layout(DeviceLayout& DL) {
@@ -515,10 +680,13 @@ int build_layout_device(const swe_assembly* a, DeviceLayout& DL) {
     if (e == hipSuccess) e = hipMemsetAsync(flag, 0, 2 * sizeof(int), s);
     if (e == hipSuccess) e = hipMemsetAsync(slots + nsl, 0, sizeof(int64_t), s);
     if (e == hipSuccess) e = hipMemsetAsync(ecnt + nsl, 0, sizeof(int64_t), s);
+    int* base = nullptr;
+    if (e == hipSuccess) e = locality_order(a, s, &base);
     int hflag[2] = {0, 0};
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_lay_len, dim3(grid_of(np, 256)), dim3(256), 0, s, np, (const int64_t*)a->ia0, lenc, flag);
-        hipLaunchKernelGGL(k_lay_sort, dim3(grid_of(nsl, 4)), dim3(256), 0, s, np, nsl, (const int*)lenc, DL.perm);
+        hipLaunchKernelGGL(k_lay_sort, dim3(grid_of(nsl, 4)), dim3(256), 0, s, np, nsl, (const int*)lenc,
+                           (const int*)base, DL.perm);
         hipLaunchKernelGGL(k_lay_iperm, dim3(grid_of(np_pad, 256)), dim3(256), 0, s, np, np_pad, (const int*)DL.perm,
                            (const int*)lenc, DL.iperm, DL.len);
         hipLaunchKernelGGL(k_lay_slices, dim3(grid_of(nsl, 256)), dim3(256), 0, s, nsl, np, (const int*)DL.len, slots,
@@ -531,7 +699,7 @@ int build_layout_device(const swe_assembly* a, DeviceLayout& DL) {
     if (e == hipSuccess) e = hipMemcpyAsync(&DL.nslot, DL.sptr + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&DL.ne, DL.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    void* tmp[] = {lenc, flag, slots, ecnt};
+    void* tmp[] = {lenc, flag, slots, ecnt, base};
     for (void* b : tmp) dfree(b, s);
     if (e != hipSuccess) {
         free_device_layout(DL);
