@@ -126,3 +126,19 @@ def test_solver_from_device_assembly_on_tiny_meshes(n):
         out.append((s.get_state(), t, dts))
         s.close()
     assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][2], out[1][2]) and out[0][1] == out[1][1]
+
+
+def test_device_locality_order_forced(monkeypatch):
+    """SWE_REORDER=2 forces the Morton order of the device layout (and RCM on the host) on a mesh
+    whose numbering is already good: the results keep every bit."""
+    from swe_amd import problems as PR
+    p = PR.bumps(n_side=129)
+    out = []
+    for mode, dev in (("0", False), ("2", True), ("2", False)):
+        monkeypatch.setenv("SWE_REORDER", mode)
+        with p.solver(device_setup=dev) as s:
+            s.set_state(p.u0)
+            t, dts = s.run(20, time=0.0)
+            out.append((s.get_state(), t, dts))
+    for u, t, d in out[1:]:
+        assert same_bits(u, out[0][0]) and same_bits(d, out[0][2]) and t == out[0][1]
