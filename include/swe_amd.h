@@ -110,7 +110,7 @@ int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **ou
 /* swe_create over a mesh assembled by swe_assemble_device, without taking the matrices to the
  * host: the solver's layout is built on the GPU from the assembly (rows sorted by length inside
  * 64-row windows of a base order: the caller's numbering, or under SWE_REORDER -- see swe_create --
- * the nodes sorted along a Morton curve of their coordinates), bath and the Dirichlet
+ * the nodes sorted along a Hilbert curve of their coordinates), bath and the Dirichlet
  * lists (1-based, as in swe_mesh_desc) are copied from the host.  The solver does not keep the
  * assembly; results are bitwise those of swe_create on the same arrays.  This replaces, together
  * with swe_assemble_device, construct_matrices (SOURCES/update.f90:18-132, called at

@@ -363,7 +363,7 @@ __global__ void k_lay_sort(int np, int nwin, const int* __restrict__ lenc, const
     if (valid) perm[w * 64 + rank] = i;
 }
 
-// ---- locality order: nodes sorted along a Morton (Z-order) curve of their coordinates ----------
+// ---- locality order: nodes sorted along a Hilbert curve of their coordinates ------------------
 // Orderable image of a double: unsigned comparison of the images orders finite doubles.
 __device__ __forceinline__ unsigned long long ord_of(double v) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -399,27 +399,34 @@ __global__ void k_bbox(int np, const double* __restrict__ rr, unsigned long long
     }
 }
 
-__device__ __forceinline__ unsigned long long spread_bits(unsigned long long x) {
-    x &= 0xffffffffull;
-    x = (x | (x << 16)) & 0x0000ffff0000ffffull;
-    x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
-    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
-    x = (x | (x << 2)) & 0x3333333333333333ull;
-    x = (x | (x << 1)) & 0x5555555555555555ull;
-    return x;
-}
+constexpr int kCurveBits = 20;   // per coordinate: 2^20 cells along each side of the bounding box
 
-constexpr int kMortonBits = 20;   // per coordinate: 2^20 cells along each side of the bounding box
-
-__global__ void k_morton(int np, const double* __restrict__ rr, double x0, double sx, double y0, double sy,
-                         unsigned long long* __restrict__ key, int* __restrict__ idx) {
+// Position of each node along a Hilbert curve through the 2^20 x 2^20 cells of the bounding box
+// (the classic bit-by-bit conversion: quadrant, then rotate / reflect the lower bits).
+__global__ void k_curve_key(int np, const double* __restrict__ rr, double x0, double sx, double y0, double sy,
+                            unsigned long long* __restrict__ key, int* __restrict__ idx) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= np) return;
-    const double top = (double)((1 << kMortonBits) - 1);
+    const double top = (double)((1 << kCurveBits) - 1);
     double fx = (rr[2 * (size_t)i] - x0) * sx, fy = (rr[2 * (size_t)i + 1] - y0) * sy;
     fx = (fx > 0.0) ? ((fx < top) ? fx : top) : 0.0;
     fy = (fy > 0.0) ? ((fy < top) ? fy : top) : 0.0;
-    key[i] = spread_bits((unsigned long long)fx) | (spread_bits((unsigned long long)fy) << 1);
+    unsigned x = (unsigned)fx, y = (unsigned)fy;
+    unsigned long long d = 0;
+    for (unsigned sq = 1u << (kCurveBits - 1); sq > 0; sq >>= 1) {
+        const unsigned rx = (x & sq) ? 1u : 0u, ry = (y & sq) ? 1u : 0u;
+        d += (unsigned long long)sq * sq * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = (x & ~(sq - 1)) + (sq - 1 - (x & (sq - 1)));
+                y = (y & ~(sq - 1)) + (sq - 1 - (y & (sq - 1)));
+            }
+            const unsigned t = x;
+            x = y;
+            y = t;
+        }
+    }
+    key[i] = d;
     idx[i] = i;
 }
 
@@ -579,7 +586,7 @@ hipError_t read_u64(const unsigned long long* d, unsigned long long* h, int n, h
 }
 
 // The base order of the layout.  *base stays null for the caller's numbering; otherwise it lists the
-// nodes along a Morton (Z-order) curve of their coordinates, the device counterpart of the host
+// nodes along a Hilbert curve of their coordinates, the device counterpart of the host
 // layout's reverse Cuthill-McKee order, under the same SWE_REORDER policy: 1 when the caller's
 // numbering scatters neighbours (mean |i-j| above 4 sqrt(np)) and the curve brings them closer, 2
 // always.  Only the processing order changes; results do not depend on it.
@@ -612,7 +619,7 @@ hipError_t locality_order(const swe_assembly* a, hipStream_t s, int** base) {
         if (e == hipSuccess) e = read_u64(acc + 2, h + 2, 4, s);
         const double x0 = double_of_ord(h[2]), x1 = double_of_ord(h[3]);
         const double y0 = double_of_ord(h[4]), y1 = double_of_ord(h[5]);
-        const double top = (double)((1 << kMortonBits) - 1);
+        const double top = (double)((1 << kCurveBits) - 1);
         const double sx = (x1 > x0) ? top / (x1 - x0) : 0.0, sy = (y1 > y0) ? top / (y1 - y0) : 0.0;
         if (e == hipSuccess) e = dalloc(&keys, (size_t)np, s);
         if (e == hipSuccess) e = dalloc(&keys2, (size_t)np, s);
@@ -620,15 +627,15 @@ hipError_t locality_order(const swe_assembly* a, hipStream_t s, int** base) {
         if (e == hipSuccess) e = dalloc(&order, (size_t)np, s);
         if (e == hipSuccess) e = dalloc(&pos, (size_t)np, s);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_morton, dim3(bl), dim3(256), 0, s, np, (const double*)a->rr, x0, sx, y0, sy, keys, idx);
+            hipLaunchKernelGGL(k_curve_key, dim3(bl), dim3(256), 0, s, np, (const double*)a->rr, x0, sx, y0, sy, keys, idx);
             e = hipGetLastError();
         }
         size_t bytes = 0;
         if (e == hipSuccess)
-            e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys, keys2, idx, order, np, 0, 2 * kMortonBits, s);
+            e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys, keys2, idx, order, np, 0, 2 * kCurveBits, s);
         if (e == hipSuccess) e = hipMallocAsync(&tmp, bytes ? bytes : 1, s);
         if (e == hipSuccess)   // a radix sort is stable: equal keys keep the caller's order
-            e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys2, idx, order, np, 0, 2 * kMortonBits, s);
+            e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys2, idx, order, np, 0, 2 * kCurveBits, s);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_inverse, dim3(bl), dim3(256), 0, s, np, (const int*)order, pos);
             hipLaunchKernelGGL(k_offset_sum, dim3(bl), dim3(256), 0, s, np, (const int64_t*)a->ia0,

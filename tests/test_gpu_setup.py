@@ -129,7 +129,7 @@ def test_solver_from_device_assembly_on_tiny_meshes(n):
 
 
 def test_device_locality_order_forced(monkeypatch):
-    """SWE_REORDER=2 forces the Morton order of the device layout (and RCM on the host) on a mesh
+    """SWE_REORDER=2 forces the Hilbert-curve order of the device layout (and RCM on the host) on a mesh
     whose numbering is already good: the results keep every bit."""
     from swe_amd import problems as PR
     p = PR.bumps(n_side=129)
