@@ -543,6 +543,24 @@ def main():
     s.enable_kernel_timing(None)
     hmin = float(s.get_state()[:, 0].min())
     schedule = {k: list(v) for k, v in s.schedule.items()}
+    # every kernel of the step against the same roofline (3 more steps per kernel, after the timed ones)
+    per_kernel = {}
+    names = (["k_compute_dt", "k_alpha", "k_fct_lo", "k_fct_rminus", "k_fct_apply"] if fct else
+             ["k_compute_dt", "k_alpha", "k_stage_lo"] if alpha else ["k_compute_dt", "k_stage_lo"])
+    for name in names:
+        if name == dominant:
+            n_l, t_ms = nlaunch, kms
+        else:
+            s.enable_kernel_timing(name)
+            s.run_async(3)
+            s.sync()
+            n_l, t_ms = s.kernel_timing(name)
+            s.enable_kernel_timing(None)
+        if n_l:
+            ms_l = t_ms / n_l
+            gbs = kernel_bytes_per_node(name, k, alpha) * p.np / (ms_l / 1e3) / 1e9
+            per_kernel[name] = {"launch_ms": ms_l, "per_step": n_l / (args.steps if name == dominant else 3),
+                                "alg_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     rate = p.np * 3 * args.steps / el
     bpn = kernel_bytes_per_node(dominant, k, alpha)
     avg_s = kms / 1e3 / max(nlaunch, 1)
@@ -566,6 +584,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
                      "launch_ms": avg_s * 1e3, "bytes_per_node": bpn},
         "step_effective_GBs": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9,
+        "kernels": per_kernel,
     }
     s.close()
     if not args.no_cpu:
