@@ -110,6 +110,46 @@ __device__ __forceinline__ int64_t row_base(const Mesh& M, int i) {
     return M.sl[i >> 6].sbase + (i & 63);
 }
 
+// Per-entry arrays streamed once per pass (c_ij, the first-order viscosities; with SWE_NT_LOADS >= 2
+// also the dt pass's d_ij): their loads and stores carry the non-temporal hint, so they do not
+// displace the gathered node data from the caches (measured: 14.86 -> 14.70 ms per 16.8M-node step).
+#ifndef SWE_NT_LOADS
+#define SWE_NT_LOADS 2
+#endif
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_entry(const double2* p) {
+#if SWE_NT_LOADS
+    const dvec2 v = __builtin_nontemporal_load((const dvec2*)p);
+    return make_double2(v.x, v.y);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ double ld_entry1(const double* p) {
+#if SWE_NT_LOADS > 1
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void st_entry1(double* p, double v) {
+#if SWE_NT_LOADS > 1
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+__device__ __forceinline__ void st_entry(double2* p, double2 v) {
+#if SWE_NT_LOADS
+    dvec2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (dvec2*)p);
+#else
+    *p = v;
+#endif
+}
+
 // Column of slot p of row i: a 16-bit offset from the row (half the bytes of the 32-bit column,
 // which is read only where the offset does not fit).
 __device__ __forceinline__ int col_at(const Mesh& M, int64_t p, int i) {
@@ -268,10 +308,10 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
             const int64_t slot = S.sbase + off;
             const Node Ir = U[slice * 64 + r];
             const Node J = U[col_at(M, slot, slice * 64 + r)];
-            const double d = dij_from_c(M.c[slot], Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
+            const double d = dij_from_c(ld_entry(M.c + slot), Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
                                         sqrt(K.g * fabs(J.h)), K, err);
             lw[128 + off] = d;
-            if (d0) d0[slot] = d;
+            if (d0) st_entry1(d0 + slot, d);
         }
     }
     __syncthreads();
@@ -439,7 +479,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = col_at(M, p, i);
         const Node J = U[j];
-        const double2 c = M.c[p];
+        const double2 c = ld_entry(M.c + p);
         const double ovh_j = ovh_of(J.h, K.htiny);
         const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
         const double zmax = fmaxF(I.z, J.z);
@@ -574,7 +614,7 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = col_at(M, p, i);
         const Node J = U[j];
-        const double2 c = M.c[p];
+        const double2 c = ld_entry(M.c + p);
         const double ovh_j = ovh_of(J.h, K.htiny);
         const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
         const double zmax = fmaxF(I.z, J.z);
@@ -591,10 +631,10 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
             // D0: compute_dij at U comes from the dt pass (stage 1), so this kernel carries no
             // lambda code and needs fewer registers (5 waves per SIMD instead of 4)
             double dlam;
-            if constexpr (D0) dlam = d0[p];
+            if constexpr (D0) dlam = ld_entry1(d0 + p);
             else dlam = dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j, dlam, ai, alpha, j, d, mu);
-            dmu[p] = make_double2(d, mu);
+            st_entry(dmu + p, make_double2(d, mu));
         }
         double xm = 0.0;
         xm = xm - c.x * (v1j);
@@ -773,7 +813,7 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
         if (j == i) continue;    // f_ii is a zero, never < 0
         const Node J = U[j];
         double dH, muH, rij_, rji_;
-        const double2 lo = dmu[p];
+        const double2 lo = ld_entry(dmu + p);
         const double f = fct_flux(I, J, lo.x, lo.y, fmaxF(resi, res[j]), dH, muH, rij_, rji_);
         if (f < 0.0) P = P + f;
     }
@@ -811,7 +851,7 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         const int j = col_at(M, p, i);
         if (j == i) continue;    // contributes d*(0) + mu*(0)
         const Node J = U[j];
-        const double2 lo = dmu[p];
+        const double2 lo = ld_entry(dmu + p);
         const double dLp = lo.x, muLp = lo.y;
         double dH, muH, ratij, ratji;
         const double f = fct_flux(I, J, dLp, muLp, fmaxF(resi, res[j]), dH, muH, ratij, ratji);
@@ -872,7 +912,7 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = col_at(M, p, i);
         const Node J = U[j];
-        const double2 c = M.c[p];
+        const double2 c = ld_entry(M.c + p);
         const double ovh_j = ovh_of(J.h, K.htiny);
         const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
         const double zmax = fmaxF(I.z, J.z);
@@ -884,7 +924,7 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         double d = 0.0, mu = 0.0;
         if (off) {
             const double rij = fmaxF(resi, res[j]);
-            const double2 lo = dmu[p];
+            const double2 lo = ld_entry(dmu + p);
             d = fminF(lo.x, 1.0 * rij);
             mu = fminF(lo.y, 1.0 * rij);
         }
