@@ -113,6 +113,11 @@ __device__ __forceinline__ int64_t row_base(const Mesh& M, int i) {
 // Per-entry arrays streamed once per pass (c_ij, the first-order viscosities; with SWE_NT_LOADS >= 2
 // also the dt pass's d_ij): their loads and stores carry the non-temporal hint, so they do not
 // displace the gathered node data from the caches (measured: 14.86 -> 14.70 ms per 16.8M-node step).
+// SWE_NT_NODE: the same hint on the low-order solution (1: discharge, 2: depth too), written by
+// k_fct_lo and read back only at the row itself.
+#ifndef SWE_NT_NODE
+#define SWE_NT_NODE 2
+#endif
 #ifndef SWE_NT_LOADS
 #define SWE_NT_LOADS 2
 #endif
@@ -715,8 +720,16 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
         double hL;
         if (roundoff) hL = I.h * (1.0 + dt * fix / mi) + dt * r1;
         else hL = fmaxF(I.h + dt * r1, 0.0);
+#if SWE_NT_NODE > 1
+        st_entry1(ulo_h + i, hL);
+#else
         ulo_h[i] = hL;
+#endif
+#if SWE_NT_NODE > 0
+        st_entry(ulo_q + i, make_double2(I.qx + dt * r2, I.qy + dt * r3));
+#else
         ulo_q[i] = make_double2(I.qx + dt * r2, I.qy + dt * r3);
+#endif
         if (K.check_h && hL < 0.0) err |= FLAG_NEG_H;
     }
     // entropy_residual (update.f90:838-868)
@@ -863,8 +876,16 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         x3 = x3 + d * (J.qy * ratji - I.qy * ratij) + mu * (J.qy * (1.0 - ratji) - I.qy * (1.0 - ratij));
     }
     const double mi = M.m[i];
+#if SWE_NT_NODE > 1
+    const double hL = ld_entry1(ulo_h + i);   // low-order solution of k_fct_lo
+#else
     const double hL = ulo_h[i];       // low-order solution of k_fct_lo
+#endif
+#if SWE_NT_NODE > 0
+    const double2 qL = ld_entry(ulo_q + i);
+#else
     const double2 qL = ulo_q[i];
+#endif
     const double h = hL + dt * (x1 / mi);
     const double qx = qL.x + dt * (x2 / mi);
     const double qy = qL.y + dt * (x3 / mi);
