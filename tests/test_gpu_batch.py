@@ -271,3 +271,20 @@ def test_host_dropin_loop_equals_device_loop():
         u, t, dts = DropinLoop(s, p).run(p.u0, n)
     assert t == t_dev and np.array_equal(dts, dts_dev)
     assert same_bits(u, u_dev)
+
+
+def test_admission_from_elements_reports_no_room_and_bad_lists():
+    """add_elements: a mesh larger than the batch's free room is refused (None, nothing admitted),
+    a Dirichlet node out of range is an error; the batch still admits what fits afterwards."""
+    from swe_amd import problems
+    from swe_amd.solver import SweError, make_params
+    small, big = problems.dam_break(61, 13, seed=3), problems.bumps(n_side=65, L=1000.0, seed=11)
+    with new_batch([small], spare=1) as b:
+        assert b.add_problem(big, max_steps=5, device_setup=True) is None
+        with pytest.raises(SweError):
+            b.add_elements(small.mesh.jj, small.mesh.rr, small.bath, make_params(**small.params), small.u0,
+                           h_dir=[small.np])
+        k = b.add_problem(small, max_steps=5, device_setup=True)
+        assert k is not None
+        b.step(5)
+        assert same_bits(b.get_state(k), run_alone(small, 5)[0])
