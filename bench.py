@@ -23,6 +23,12 @@ sys.path.insert(0, os.path.join(REPO, "shallow-water-eric_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MESH_NAMES = {
+    "bumps": "Delaunay triangulation of a seeded Poisson-disk point set (unstructured)",
+    "bumps_lattice": "cell-split jittered lattice (structured topology)",
+    "dam_break": "cell-split jittered lattice 2241 x 449",
+    "paraboloid": "81 x 81 grid",
+}
 
 
 def kernel_bytes_per_node(kernel, k, alpha=True):
@@ -67,9 +73,11 @@ def measured_traffic(kernel, workload, nodes):
     return None
 
 
-def build_problem(name, size=None):
+def build_problem(name, size=None, host_assembly=True):
     from swe_amd import problems
     if name == "bumps":
+        return problems.bumps_unstructured(n_nodes=size or 16_785_409, host_assembly=host_assembly)
+    if name == "bumps_lattice":
         return problems.bumps(n_side=size or 4097)
     if name == "dam_break":
         nx = size or 2241
@@ -79,69 +87,103 @@ def build_problem(name, size=None):
     raise SystemExit("unknown workload %s" % name)
 
 
-def reference_baseline(name, p, seconds):
-    """The reference Fortran itself (oracle/_ref/ref_driver: the reference modules built with amdflang
-    -O2, its SSP-RK3 loop of SOURCES/main.f90:61-92), 1 thread, on the mesh of problem `p`: a short
-    run gives the rate, a second run as many steps as fit in about `seconds`; the time is the
-    driver's own clock around its time loop (set-up excluded)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """The cores this process may run on (the box's CPU share), at most 16."""
+    try:
+        cores = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = list(range(os.cpu_count() or 1))
+    return cores[:16]
+
+
+def reference_baseline(name, p, seconds, driver=None, all_cores=True):
+    """The reference Fortran (oracle/_ref: the reference modules built with amdflang -O2, their
+    SSP-RK3 loop of SOURCES/main.f90:61-92) on the mesh of problem `p`, timed by the driver's own
+    clock around its time loop (set-up excluded).  driver = REF_DRIVER_HOIST is the build whose
+    entropy_residual assigns its rescale array once instead of once per node (the same bits in O(np)
+    rather than O(np^2) per stage).  One process pinned to one core with taskset: a short run gives
+    the rate, a second run as many steps as fit in about `seconds`.  all_cores: then one process per
+    core of this process's CPU share (at most 16), each running the same steps at once, their
+    rates summed: the throughput of the whole host on independent meshes."""
     import shutil
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as O
-    from swe_amd import mesh as M
-    pr = p.params
     test = {"bumps": 4, "dam_break": 10, "paraboloid": 5}[name]
+    cores = host_cores()
+    pin = cores[0] if shutil.which("taskset") else None
     d = O.tmpdir()
     try:
-        M.write_fem(os.path.join(d, "mesh.FEM"), p.mesh)
-        O.data_file(os.path.join(d, "data"), mesh_dir=".", mesh_file="mesh.FEM", test=test, cfl=pr["cfl"],
-                    viscosity=pr["viscosity"], viscous_type="type%d" % pr["viscous_type"], alpha=pr["alpha_limit"],
-                    h_dir=p.h_sides, ux_dir=p.ux_sides, uy_dir=p.uy_sides,
-                    mannings=pr["mannings"] if test == 10 else None)
-        init = "none"
-        if not p.reference_init:
-            kind, amp, period, level = p.bc
-            O.write_custom_init(os.path.join(d, "init.bin"), p.bath, p.u0, gravity=pr["gravity"],
-                                max_water_h=pr["max_water_h"], bc_kind=kind, amp=amp, period=period, level=level)
-            init = "init.bin"
-
-        def run(n):
-            out = os.path.join(d, "run%d" % n)
-            os.makedirs(out, exist_ok=True)
-            O.run_ref(["run", "data", init, n, out], cwd=d, timeout=600)
-            return float(np.fromfile(os.path.join(out, "loop_seconds.f8"))[0])
-        el0 = run(1)
+        init = O.ref_case(d, p, test)
+        el0 = O.ref_time_loop(d, init, 1, driver=driver, pin=pin, out="one")[3]
         n = max(1, int(seconds / max(el0, 1e-6)))
-        el = run(n)
+        el = O.ref_time_loop(d, init, n, driver=driver, pin=pin, out="pinned")[3]
+        res = {"value": p.np * 3 * n / el, "unit": "node-stage updates/s", "cores": 1, "kind": "reference",
+               "cpu_model": cpu_model(), "pinned_core": pin,
+               "sample": "%s, %d nodes, %d steps (%.1f s of time loop), the reference Fortran (amdflang -O2 build of "
+                         "SOURCES/update.f90, fct.f90 and the RK3 loop%s), 1 thread%s"
+                         % (p.name, p.np, n, el, ", rescale loop of update.f90:865-867 hoisted" if driver else "",
+                            ", taskset -c %d" % pin if pin is not None else "")}
+        if all_cores and len(cores) > 1:
+            import subprocess as sp
+            procs = []
+            for q, c in enumerate(cores):
+                od = os.path.join(d, "all%d" % q)
+                os.makedirs(od, exist_ok=True)
+                cmd = ["taskset", "-c", str(c)] if pin is not None else []
+                cmd += [driver or O.REF_DRIVER, "run", "data", init, str(n), "all%d" % q]
+                procs.append(sp.Popen(cmd, cwd=d, stdout=sp.DEVNULL, stderr=sp.DEVNULL, preexec_fn=O._big_stack))
+            ok = all(pr.wait(timeout=900) == 0 for pr in procs)
+            if ok:
+                secs = [float(np.fromfile(os.path.join(d, "all%d" % q, "loop_seconds.f8"))[0]) for q in range(len(cores))]
+                res["all_cores_value"] = sum(p.np * 3 * n / x for x in secs)
+                res["all_cores_procs"] = len(cores)
+                res["all_cores_sample"] = ("%d processes of the same run at once, one per core (%s), rates summed"
+                                           % (len(cores), "pinned" if pin is not None else "not pinned"))
     finally:
         shutil.rmtree(d, ignore_errors=True)
-    return {"value": p.np * 3 * n / el, "unit": "node-stage updates/s", "cores": 1, "kind": "reference",
-            "sample": "%s, %d nodes, %d steps (%.1f s of time loop), the reference Fortran (amdflang -O2 "
-                      "build of SOURCES/update.f90, fct.f90 and the RK3 loop), 1 thread" % (name, p.np, n, el)}
+    return res
 
 
-def cpu_baseline(name, seconds=15.0):
-    """The CPU baseline on a bounded sample of the same workload family (a smaller mesh, as many steps
-    as fit in about `seconds`, 1 thread): the reference Fortran when its build is present
-    (oracle/_ref), else the C restatement of the reference step."""
+def cpu_baseline(name, seconds=12.0):
+    """The CPU baseline on a bounded sample of the same workload family (a smaller mesh of the same
+    generator, as many steps as fit in about `seconds`): the reference Fortran with the rescale loop
+    hoisted (the reference's own arithmetic, bitwise, without its O(np^2) loop) when its build is
+    present (oracle/_ref), pinned to one core, with the all-cores figure beside it; then the
+    unmodified reference on a smaller mesh and the C restatement for comparison.  Without the
+    reference build, the C restatement alone."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as O
-    size = {"bumps": 257, "dam_break": 561, "paraboloid": 81}[name]
+    size = {"bumps": 262_144, "dam_break": 561, "paraboloid": 81}[name]
     p = build_problem(name, size)
     port = port_baseline(name, p, seconds)
-    if O.ref_available():
-        try:
-            ref = reference_baseline(name, p, seconds)
-        except (RuntimeError, OSError, subprocess.SubprocessError) as e:
-            print("# reference baseline failed (%s); reporting the C restatement" % str(e)[:200],
-                  file=sys.stderr, flush=True)
-            return port
-        # the reference re-assigns its entropy rescale array once per node inside its node loop
-        # (SOURCES/update.f90:865-867), O(np^2) per stage, so its rate falls with the mesh size; the
-        # C restatement evaluates that assignment once (the same bits) and is reported beside it
-        ref["port_value"] = port["value"]
-        ref["port_sample"] = port["sample"]
-        return ref
-    return port
+    if not (O.ref_available() and os.path.exists(O.REF_DRIVER_HOIST)):
+        return port
+    try:
+        ref = reference_baseline(name, p, seconds, driver=O.REF_DRIVER_HOIST)
+        # the unmodified reference, whose rate falls with the mesh size (O(np^2) rescale loop), on a
+        # mesh a quarter the size, a few seconds
+        small = build_problem(name, {"bumps": 65_536, "dam_break": 281, "paraboloid": 81}[name])
+        plain = reference_baseline(name, small, seconds / 3, all_cores=False)
+    except (RuntimeError, OSError, subprocess.SubprocessError) as e:
+        print("# reference baseline failed (%s); reporting the C restatement" % str(e)[:200],
+              file=sys.stderr, flush=True)
+        return port
+    ref["unmodified_value"] = plain["value"]
+    ref["unmodified_sample"] = plain["sample"]
+    ref["port_value"] = port["value"]
+    ref["port_sample"] = port["sample"]
+    return ref
 
 
 def port_baseline(name, p, seconds):
@@ -167,7 +209,7 @@ def port_baseline(name, p, seconds):
     rate = p.np * 3 * n / el
     return {"value": rate, "unit": "node-stage updates/s", "cores": 1, "kind": "port",
             "sample": "%s, %d nodes, %d steps (%.1f s), CPU restatement of the reference step, 1 thread"
-                      % (name, p.np, n, el)}
+                      % (p.name, p.np, n, el)}
 
 
 def copy_calibration(gib=1, reps=3):
@@ -198,13 +240,15 @@ def _batch_caps(ps, spare_rows=0):
     return rows, entries, bnd
 
 
-def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=5000, nmax=500000, dropin_steps=3):
+def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=5000, nmax=500000, dropin_steps=3,
+                dropin_count=None):
     """Batched form: `count` meshes of the 16.8M-node workload's family, sizes log-uniform in
     [nmin, nmax], dam-break or still-water initial data, all resident in one batch and advanced
     together (one captured graph per step, autotuned schedules); against two plain per-instance
     loops over the same meshes one at a time: the device-resident loop (state kept on the device,
     one captured graph per step) and the drop-in loop (the reference's host time loop calling
-    compute_dij and euler per stage, `dropin_steps` steps per mesh)."""
+    compute_dij and euler per stage, `dropin_steps` steps per mesh, on the first `dropin_count`
+    meshes, a random sample of the family, when given)."""
     from swe_amd import problems
     from swe_amd.batch import Batch
     t0 = time.perf_counter()
@@ -274,7 +318,7 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
     # (compute_dij, euler) of each mesh, one mesh at a time, state over PCIe every stage
     from swe_amd.dropin import DropinLoop
     el_drop, work_drop = 0.0, 0
-    for p in ps:
+    for p in ps[:dropin_count]:
         with p.solver() as s:
             loop = DropinLoop(s, p)
             loop.run(p.u0, 1)
@@ -293,7 +337,8 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
             "node_stage_per_s": rate_b, "ms_per_step": el_graph / steps * 1e3,
             "node_stage_per_s_eager_timed": work / el, "plain_node_stage_per_s": rate_p,
             "plain_ms_total": el_plain * 1e3, "speedup_vs_plain": rate_b / rate_p,
-            "dropin_node_stage_per_s": rate_d, "dropin_steps": dropin_steps, "speedup_vs_dropin": rate_b / rate_d,
+            "dropin_node_stage_per_s": rate_d, "dropin_steps": dropin_steps,
+            "dropin_meshes": len(ps[:dropin_count]), "speedup_vs_dropin": rate_b / rate_d,
             "failed_instances": failed,
             "admission_s": t_admit,
             "roofline": {"bound": "hbm", "kernel": "k_fct_lo", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -342,6 +387,79 @@ def bench_serve(count, rate, workers, chunk=16, budget_gb=200.0, seed=20261202, 
             "plain_p50_s": pl["p50_s"], "plain_p99_s": pl["p99_s"], "plain_node_stage_per_s": pl["node_stage_per_s"],
             "speedup_vs_plain": s["node_stage_per_s"] / pl["node_stage_per_s"],
             "results_identical_to_plain": bool(same), "errors": s["errors"]}
+
+
+def replay_fifo(arrivals, durations):
+    """Finish times of a one-at-a-time FIFO server given arrival times and each request's solve
+    duration: request k starts at max(its arrival, the previous finish)."""
+    fin, t = [], 0.0
+    for a, d in zip(arrivals, durations):
+        t = max(a, t) + d
+        fin.append(t)
+    return fin
+
+
+def bench_serve_sweep(count, rates, workers, chunk=16, budget_gb=200.0, seed=20261202, median=100000, nmax=2000000):
+    """The served form over several arrival rates: the same `count` requests (sizes lognormal with
+    the given median, capped at nmax, final times of 1-20 wave periods, 20% dry-bed dam breaks; the
+    seed fixes them, the rate only scales the arrival gaps) served by continuous batching at each
+    rate against a 200 GB budget.  The plain loop serves the stream one request at a time once; its
+    per-request solve times (admission, steps, read-back) do not depend on arrivals, so its
+    finish times at every rate follow from them by the FIFO recurrence (replay_fifo).  Every
+    served result must equal the plain loop's."""
+    from swe_amd import problems
+    from swe_amd.batch import Batch
+    from swe_amd.serve import BatchEngine, Request, Server, run_plain, summarize
+    specs = {r: problems.serve_request_specs(count, r, seed=seed, median=median, nmax=nmax) for r in rates}
+    base = specs[rates[-1]]
+    t0 = time.perf_counter()
+    ps = problems.build_family(base, workers)
+    print("# serve: %d requests, %d nodes in all (max %d), generated in %.1f s"
+          % (count, sum(p.np for p in ps), max(p.np for p in ps), time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
+
+    def reqs(sp):
+        return [Request(rid=q, problem=p, tfinal=x["tfinal"], arrival=x["arrival"]) for q, (p, x) in
+                enumerate(zip(ps, sp))]
+    # the plain loop, at the fastest rate (no idle time between requests)
+    big = max(ps, key=lambda p: p.nnz)
+    r1, e1, b1 = _batch_caps([big])
+    one = Batch(r1, e1, max(b1, max(sum(x.size for x in p.dirichlet()) for p in ps) + 64), 1)
+    plain = run_plain(BatchEngine(one), reqs(base), chunk=chunk)
+    one.close()
+    plain_by_id = {r.rid: r for r in plain}
+    dur = [plain_by_id[q].finished - max(plain_by_id[q].admitted, 0.0) for q in range(count)]
+    work = float(sum(r.node_stages for r in plain))
+    budget = budget_gb * (1 << 30)
+    need = Batch.bytes_for(*_batch_caps(ps), count)
+    out = {"requests": count, "budget_gb": budget_gb, "chunk_steps": chunk, "nodes": int(sum(p.np for p in ps)),
+           "max_nodes": int(max(p.np for p in ps)),
+           "dry_bed_fraction": sum(1 for x in base if x["kind"] == "drydam") / float(count),
+           "plain_solve_s": float(sum(dur)), "rates": []}
+    for rate in rates:
+        sp = specs[rate]
+        b = Batch.for_budget(min(budget, 1.05 * need + (1 << 30)), max_instances=max(64, count))
+        served = Server(BatchEngine(b, device_setup=False), chunk=chunk).run(reqs(sp))
+        b.close()
+        same = all(a.steps == plain_by_id[a.rid].steps and a.time == plain_by_id[a.rid].time and
+                   np.array_equal(a.state, plain_by_id[a.rid].state, equal_nan=True) for a in served)
+        arr = [x["arrival"] for x in sp]
+        fin = replay_fifo(arr, dur)
+        lat = np.array([f - a for f, a in zip(fin, arr)])
+        span = max(fin) - min(arr)
+        sv = summarize(served)
+        row = {"rate_per_s": rate, "p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "mean_s": sv["mean_s"],
+               "node_stage_per_s": sv["node_stage_per_s"], "makespan_s": sv["makespan_s"],
+               "plain_p50_s": float(np.percentile(lat, 50)), "plain_p99_s": float(np.percentile(lat, 99)),
+               "plain_node_stage_per_s": work / span, "speedup_vs_plain": sv["node_stage_per_s"] / (work / span),
+               "results_identical_to_plain": bool(same), "errors": sv["errors"]}
+        out["rates"].append(row)
+        print("# served at %.0f/s: p50 %.2f s p99 %.2f s, %.3g node-stage/s (plain %.3g), identical %s"
+              % (rate, row["p50_s"], row["p99_s"], row["node_stage_per_s"], row["plain_node_stage_per_s"], same),
+              file=sys.stderr, flush=True)
+    out["speedup_vs_plain"] = min(r["speedup_vs_plain"] for r in out["rates"])
+    out["p99_s"] = max(r["p99_s"] for r in out["rates"])
+    return out
 
 
 def bench_setup(n_side=4097, reps=2):
@@ -459,7 +577,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="bumps",
-                    choices=["bumps", "dam_break", "paraboloid", "batch", "serve", "setup", "errors"])
+                    choices=["bumps", "bumps_lattice", "dam_break", "paraboloid", "batch", "serve", "setup",
+                             "errors"])
     ap.add_argument("--instances", type=int, default=256, help="batch: meshes in the batch")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
     ap.add_argument("--rates", default="20", help="serve: arrival rates (per s), comma separated")
@@ -519,15 +638,19 @@ def main():
                               "config": {"workload": "serve", **r}}), flush=True)
         return
     t0 = time.perf_counter()
-    p = build_problem(args.workload, args.size)
+    # the unstructured workload is assembled and laid out on the GPU (swe_create_from_assembly)
+    device_setup = args.workload == "bumps"
+    p = build_problem(args.workload, args.size, host_assembly=not device_setup)
+    t_gen = time.perf_counter() - t0
+    s = p.solver(device_setup=device_setup)
     t_setup = time.perf_counter() - t0
-    k = p.nnz / p.np
+    nnz = s.nnz
+    k = nnz / p.np
     fct = p.params["viscosity"] == "fct"
     alpha = p.params["alpha_limit"]
     dominant = "k_fct_lo" if fct else "k_stage_lo"
-    print("# workload %s: %d nodes, %d entries (k=%.3f), set-up %.1f s" % (args.workload, p.np, p.nnz, k, t_setup),
-          file=sys.stderr, flush=True)
-    s = p.solver()
+    print("# workload %s: %d nodes, %d entries (k=%.3f), mesh and data %.1f s, solver %.1f s"
+          % (p.name, p.np, nnz, k, t_gen, t_setup - t_gen), file=sys.stderr, flush=True)
     s.set_state(p.u0)
     s.time = 0.0
     if not args.no_tune:
@@ -578,8 +701,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": p.name, "nodes": p.np, "entries": p.nnz, "viscosity": p.params["viscosity"],
-                   "alpha_limit": bool(alpha), "steps": args.steps, "hmin_end": hmin, "schedule": schedule},
+        "config": {"workload": p.name, "mesh": MESH_NAMES.get(args.workload, ""), "nodes": p.np, "entries": nnz,
+                   "viscosity": p.params["viscosity"], "alpha_limit": bool(alpha), "steps": args.steps,
+                   "hmin_end": hmin, "schedule": schedule, "setup": "device" if device_setup else "host"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
                      "launch_ms": avg_s * 1e3, "bytes_per_node": bpn},
@@ -590,9 +714,10 @@ def main():
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
     if args.workload == "bumps" and args.size is None and not args.no_extra:
-        # the batched and served forms at reduced sizes, against the plain per-instance loop
-        res["batched"] = bench_batch(64, 50, 3, args.workers)
-        res["served"] = bench_serve(32, 20.0, args.workers)
+        # the batched form at its configuration (256 meshes of 5k-500k nodes) and the served form's
+        # arrival-rate sweep, each against the plain per-instance loop
+        res["batched"] = bench_batch(256, 30, 3, args.workers, dropin_count=32)
+        res["served"] = bench_serve_sweep(32, (5.0, 50.0, 200.0), args.workers)
     print(json.dumps(res))
 
 

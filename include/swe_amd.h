@@ -56,7 +56,11 @@ typedef struct swe_params {
     double cfl;           /* inputs%CFL */
     double max_water_h;   /* max_water_h (entropy-residual scaling, SOURCES/update.f90:866) */
     double mannings;      /* inputs%mannings (tests 9 and 10) */
-    int type_test;        /* inputs%type_test: 9/10 Manning friction, 11 linear friction, 12 Coriolis */
+    int type_test;        /* inputs%type_test, 1..12: 9/10 Manning friction, 11 linear friction, 12 Coriolis;
+                             test 9 also fixes htiny = 1e-3 in every call and dt = 2.5e-2 in the time loop
+                             (SOURCES/main.f90:66-69).  Other values give SWE_EBADOPT: outside 1..13
+                             compute_lambda_vacc STOPs (SOURCES/condlim.f90:952-954), and 13 (the
+                             5-component SGN system) is not programmed here. */
     int viscosity_type;   /* swe_viscosity */
     int viscous_type;     /* 1 = 'type1' (smb_1), 2 = 'type2' (smb_2 family) */
     int alpha_limit;      /* inputs%if_alpha_limit */

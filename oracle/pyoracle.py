@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 import subprocess
 import tempfile
 
@@ -15,6 +16,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_swe.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+# the same driver with entropy_residual's O(np^2) rescale loop reduced to its one assignment
+# (oracle/hoist_rescale.awk): the same bits, timed as the CPU baseline
+REF_DRIVER_HOIST = os.path.join(HERE, "_ref", "ref_driver_hoist")
 
 GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
 VISC_NAMES = {GALERKIN: "galerkin", VISCOUS: "viscous", ENTROPY_VISC: "entropy_visc", FCT: "fct"}
@@ -24,7 +28,7 @@ class OrcParams(C.Structure):
     _fields_ = [("gravity", C.c_double), ("htiny", C.c_double), ("cfl", C.c_double),
                 ("max_water_h", C.c_double), ("mannings", C.c_double),
                 ("type_test", C.c_int), ("viscosity_type", C.c_int), ("viscous_type", C.c_int),
-                ("alpha_limit", C.c_int), ("roundoff_fix", C.c_int)]
+                ("alpha_limit", C.c_int), ("roundoff_fix", C.c_int), ("lambda_sgn", C.c_double)]
 
 
 _lib = None
@@ -49,6 +53,9 @@ def lib():
         L.orc_create.argtypes = [C.c_int, C.c_int, P, P, P, P, P, C.POINTER(OrcParams)]
         L.orc_destroy.argtypes = [P]
         L.orc_set_params.argtypes = [P, C.POINTER(OrcParams)]
+        L.orc_syst_size.restype = C.c_int
+        L.orc_syst_size.argtypes = [P]
+        L.orc_set_coords.argtypes = [P, P]
         L.orc_lambda_batch.argtypes = [C.c_int, P, C.c_double, C.c_double, P, P]
         L.orc_fct_positivity.argtypes = [C.c_int, P, P, P, P, C.c_double, P]
         L.orc_fct_generic.argtypes = [C.c_int, P, P, P, P, P, P, P]
@@ -150,18 +157,23 @@ class Oracle:
 
     def __init__(self, ia, ja, cij, lumped, bath, *, gravity, htiny, cfl, max_water_h,
                  viscosity_type=VISCOUS, viscous_type=2, alpha_limit=False, type_test=5,
-                 mannings=0.0, roundoff_fix=True):
+                 mannings=0.0, roundoff_fix=True, lambda_sgn=0.0, rr=None):
+        """States are (np, nc): nc = 3, or 5 (h, qx, qy, h*eta, h*w) for the SGN model of
+        type_test 13.  rr (np, 2): node coordinates, for the reference's bdy (bc_kind 3)."""
         self.ia = np.ascontiguousarray(ia, np.int32)
         self.ja = np.ascontiguousarray(ja, np.int32)
         self.np = self.ia.size - 1
         self.nnz = self.ja.size
         self.prm = OrcParams(gravity, htiny, cfl, max_water_h, mannings, type_test,
-                             viscosity_type, viscous_type, int(alpha_limit), int(roundoff_fix))
+                             viscosity_type, viscous_type, int(alpha_limit), int(roundoff_fix), float(lambda_sgn))
         cij = np.ascontiguousarray(cij, np.float64).reshape(-1)
         self._keep = (self.ia, self.ja, cij, np.ascontiguousarray(lumped, np.float64),
                       np.ascontiguousarray(bath, np.float64))
         self.h = lib().orc_create(self.np, self.nnz, _p(self.ia), _p(self.ja), _p(self._keep[2]),
                                   _p(self._keep[3]), _p(self._keep[4]), C.byref(self.prm))
+        self.nc = lib().orc_syst_size(self.h)
+        if rr is not None:
+            lib().orc_set_coords(self.h, _p(np.ascontiguousarray(rr, np.float64)))
 
     def close(self):
         if self.h:
@@ -216,7 +228,7 @@ def ref_available() -> bool:
 
 
 def data_file(path, *, mesh_dir, mesh_file, test, tfinal=1.0, cfl=0.25, viscosity="viscous",
-              viscous_type="type2", alpha=False, h_dir=(), ux_dir=(), uy_dir=(), mannings=None):
+              viscous_type="type2", alpha=False, h_dir=(), ux_dir=(), uy_dir=(), mannings=None, lambda_sgn=None):
     """Write a reference `data` file (keys of SOURCES/read_data.f90:42-97)."""
     def lst(v):
         return " ".join(str(int(x)) for x in v) if len(v) else "1"
@@ -242,6 +254,8 @@ def data_file(path, *, mesh_dir, mesh_file, test, tfinal=1.0, cfl=0.25, viscosit
     ]
     if mannings is not None:
         lines += ["===Mannings coefficient===", repr(float(mannings))]
+    if lambda_sgn is not None:
+        lines += ["===Lambda for SGN model===", repr(float(lambda_sgn))]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -264,12 +278,45 @@ def _big_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
 
 
-def run_ref(args, cwd, timeout=3600):
-    r = subprocess.run([REF_DRIVER] + [str(a) for a in args], cwd=cwd, capture_output=True,
-                       text=True, timeout=timeout, preexec_fn=_big_stack)
+def run_ref(args, cwd, timeout=3600, driver=None, pin=None):
+    """Run the reference driver (or `driver`) in `cwd`; pin = CPU core to pin it to with taskset."""
+    cmd = [driver or REF_DRIVER] + [str(a) for a in args]
+    if pin is not None:
+        cmd = ["taskset", "-c", str(pin)] + cmd
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=timeout, preexec_fn=_big_stack)
     if r.returncode != 0 or "BUG" in r.stdout or "STOP" in r.stdout:
         raise RuntimeError("ref_driver %s failed (%d):\n%s\n%s" % (args[0], r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     return r.stdout
+
+
+def ref_case(d, problem, test, mesh_file="mesh.FEM"):
+    """Write the mesh (.FEM), `data` file and initial state of a swe_amd.problems.Problem into the
+    directory d for the reference driver; returns the init argument ("none" or the init file)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "shallow-water-eric_amd"))
+    from swe_amd import mesh as M
+    pr = problem.params
+    M.write_fem(os.path.join(d, mesh_file), problem.mesh)
+    data_file(os.path.join(d, "data"), mesh_dir=".", mesh_file=mesh_file, test=test, cfl=pr["cfl"],
+              viscosity=pr["viscosity"], viscous_type="type%d" % pr["viscous_type"], alpha=pr["alpha_limit"],
+              h_dir=problem.h_sides, ux_dir=problem.ux_sides, uy_dir=problem.uy_sides,
+              mannings=pr["mannings"] if test in (9, 10) else None)
+    if problem.reference_init:
+        return "none"
+    kind, amp, period, level = problem.bc
+    write_custom_init(os.path.join(d, "init.bin"), problem.bath, problem.u0, gravity=pr["gravity"],
+                      max_water_h=pr["max_water_h"], bc_kind=kind, amp=amp, period=period, level=level)
+    return "init.bin"
+
+
+def ref_time_loop(d, init, nsteps, driver=None, pin=None, out="run", timeout=3600):
+    """nsteps of the driver's SSP-RK3 loop on the case in d: (final (np,3), dts, end time, seconds of
+    the time loop by the driver's own clock, set-up excluded)."""
+    od = os.path.join(d, out)
+    os.makedirs(od, exist_ok=True)
+    run_ref(["run", "data", init, nsteps, out], cwd=d, timeout=timeout, driver=driver, pin=pin)
+    final = np.fromfile(os.path.join(od, "final.f8"), dtype="<f8").reshape(-1, 3)
+    return (final, np.fromfile(os.path.join(od, "dt.f8")), float(np.fromfile(os.path.join(od, "time.f8"))[0]),
+            float(np.fromfile(os.path.join(od, "loop_seconds.f8"))[0]))
 
 
 def read_setup(d):
@@ -281,12 +328,13 @@ def read_setup(d):
     dims = i4("dims.i4")
     npn, me, nnz = int(dims[0]), int(dims[1]), int(dims[2])
     sc = f8("scalars.f8")
-    return dict(np=npn, me=me, nnz=nnz, type_test=int(dims[6]),
+    nc = 5 if int(dims[6]) == 13 else 3
+    return dict(np=npn, me=me, nnz=nnz, type_test=int(dims[6]), syst_size=nc,
                 gravity=sc[0], htiny=sc[1], max_water_h=sc[2], cfl=sc[3], epsilon=sc[4], time0=sc[5],
                 jj=i4("jj.i4").reshape(me, 3) - 1, rr=f8("rr.f8").reshape(npn, 2),
                 ia=i4("ia.i4") - 1, ja=i4("ja.i4") - 1, mass=f8("mass.f8"), lumped=f8("lumped.f8"),
                 diag=i4("diag.i4") - 1, cij=f8("cij.f8").reshape(nnz, 2), bath=f8("bath.f8"),
-                u0=f8("u0.f8").reshape(npn, 3),
+                u0=f8("u0.f8").reshape(npn, nc),
                 h_js_D=i4("h_js_D.i4") - 1, ux_js_D=i4("ux_js_D.i4") - 1, uy_js_D=i4("uy_js_D.i4") - 1)
 
 

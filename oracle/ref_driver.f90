@@ -18,6 +18,7 @@
 !       one forward-Euler stage, euler(un,unext) (SOURCES/update.f90:134-289); writes unext
 !       and dij%aa as left by euler.
 !   ref_driver run <data> <init|none> <nsteps> <outdir> [dump_every]
+!       (states are un(syst_size,np): 3 components, 5 for the SGN model of test 13)
 !       nsteps SSP-RK3 steps as in the time loop of SOURCES/main.f90:61-92 (fixed step count
 !       instead of the Tfinal loop); writes dt per step, U after each stage of step 1, the final
 !       U, optional snapshots and the wall time of the loop.
@@ -152,6 +153,7 @@ CONTAINS
     CALL construct_mesh
     CALL construct_matrices
     inputs%syst_size = k_dim+1
+    IF (inputs%type_test==13) inputs%syst_size = k_dim+3   ! SGN model: h, hu, hv, h*eta, h*w
     ALLOCATE(un(inputs%syst_size, mesh%np))
     inputs%time = 0.d0
     IF (TRIM(initfile) == 'none') THEN
@@ -196,6 +198,10 @@ CONTAINS
        IF (SIZE(h_js_D).NE.0)  uu(1,h_js_D)  = sol_anal(1,mesh%rr(:,h_js_D),t)
        IF (SIZE(ux_js_D).NE.0) uu(2,ux_js_D) = sol_anal(2,mesh%rr(:,ux_js_D),t)
        IF (SIZE(uy_js_D).NE.0) uu(3,uy_js_D) = sol_anal(3,mesh%rr(:,uy_js_D),t)
+       IF (inputs%type_test==13) THEN
+          IF (SIZE(ux_js_D).NE.0) uu(4,ux_js_D) = sol_anal(4,mesh%rr(:,ux_js_D),t)
+          IF (SIZE(uy_js_D).NE.0) uu(5,uy_js_D) = sol_anal(5,mesh%rr(:,uy_js_D),t)
+       END IF
        RETURN
     END IF
     pi = ACOS(-1.d0)

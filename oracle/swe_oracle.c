@@ -183,6 +183,7 @@ int orc_assemble(int np, int me, const int *jj, const double *rr,
 
 struct orc_ctx {
     int np, nnz;
+    int nc;          /* inputs%syst_size: 3, or 5 for the SGN model of test 13 (main.f90:22-26) */
     int *ia, *ja, *diag;
     double *cij, *lumped, *bath;
     orc_params p;
@@ -191,6 +192,8 @@ struct orc_ctx {
     double *velocity, *one_over_h, *fix, *vv;
     double *dij, *muij, *dijL, *muijL, *resij, *fctmat, *lij;
     double *rk, *rk2, *unext2, *alpha, *res, *ent, *scaled;
+    double *sgn_p, *sgn_s;   /* pTilde and s of mSGN_RHS */
+    double *rr;              /* node coordinates (bc_kind 3), or NULL */
     int err;
 };
 
@@ -215,11 +218,12 @@ orc_ctx *orc_create(int np, int nnz, const int *ia, const int *ja, const double 
     memcpy(c->bath, bath, sizeof(double) * (size_t)np);
     for (int i = 0; i < np; i++) c->diag[i] = find_entry(c->ia, c->ja, i, i);
     c->p = *prm;
+    c->nc = (prm->type_test == 13) ? 5 : 3;
     size_t N = (size_t)np, E = (size_t)nnz;
     c->velocity = (double *)xmalloc(sizeof(double) * 2 * N);
     c->one_over_h = (double *)xmalloc(sizeof(double) * N);
     c->fix = (double *)xmalloc(sizeof(double) * N);
-    c->vv = (double *)xmalloc(sizeof(double) * 6 * N);
+    c->vv = (double *)xmalloc(sizeof(double) * 10 * N);
     c->dij = (double *)xmalloc(sizeof(double) * E);
     c->muij = (double *)xmalloc(sizeof(double) * E);
     c->dijL = (double *)xmalloc(sizeof(double) * E);
@@ -227,19 +231,35 @@ orc_ctx *orc_create(int np, int nnz, const int *ia, const int *ja, const double 
     c->resij = (double *)xmalloc(sizeof(double) * E);
     c->fctmat = (double *)xmalloc(sizeof(double) * E);
     c->lij = (double *)xmalloc(sizeof(double) * E);
-    c->rk = (double *)xmalloc(sizeof(double) * 3 * N);
-    c->rk2 = (double *)xmalloc(sizeof(double) * 3 * N);
-    c->unext2 = (double *)xmalloc(sizeof(double) * 3 * N);
+    c->rk = (double *)xmalloc(sizeof(double) * 5 * N);
+    c->rk2 = (double *)xmalloc(sizeof(double) * 5 * N);
+    c->unext2 = (double *)xmalloc(sizeof(double) * 5 * N);
     c->alpha = (double *)xmalloc(sizeof(double) * N);
     c->res = (double *)xmalloc(sizeof(double) * N);
-    c->ent = (double *)xmalloc(sizeof(double) * 3 * N);
+    c->ent = (double *)xmalloc(sizeof(double) * 5 * N);
     c->scaled = (double *)xmalloc(sizeof(double) * N);
+    c->sgn_p = (double *)xmalloc(sizeof(double) * N);
+    c->sgn_s = (double *)xmalloc(sizeof(double) * N);
     memset(c->dij, 0, sizeof(double) * E);
     memset(c->muij, 0, sizeof(double) * E);
     return c;
 }
 
-void orc_set_params(orc_ctx *c, const orc_params *prm) { c->p = *prm; }
+/* the number of components follows type_test, as in main.f90:22-26 */
+void orc_set_params(orc_ctx *c, const orc_params *prm)
+{
+    c->p = *prm;
+    c->nc = (prm->type_test == 13) ? 5 : 3;
+}
+
+int orc_syst_size(const orc_ctx *c) { return c->nc; }
+
+void orc_set_coords(orc_ctx *c, const double *rr)
+{
+    free(c->rr);
+    c->rr = (double *)xmalloc(sizeof(double) * 2 * (size_t)c->np);
+    memcpy(c->rr, rr, sizeof(double) * 2 * (size_t)c->np);
+}
 
 void orc_destroy(orc_ctx *c)
 {
@@ -248,7 +268,7 @@ void orc_destroy(orc_ctx *c)
     free(c->velocity); free(c->one_over_h); free(c->fix); free(c->vv);
     free(c->dij); free(c->muij); free(c->dijL); free(c->muijL); free(c->resij); free(c->fctmat);
     free(c->lij); free(c->rk); free(c->rk2); free(c->unext2); free(c->alpha); free(c->res);
-    free(c->ent); free(c->scaled);
+    free(c->ent); free(c->scaled); free(c->sgn_p); free(c->sgn_s); free(c->rr);
     free(c);
 }
 
@@ -344,25 +364,26 @@ void orc_lambda_batch(int n, const double *rec8, double gravity, double htiny, d
     }
 }
 
-/* flux (condlim.f90:21-77) for the 3-component system: velocity, one_over_h, vv(k,d,n) */
+/* flux (condlim.f90:21-77): velocity, one_over_h, vv(k,d,n) = velocity(d,n)*un(k,n).  For the
+ * SGN model (test 13, condlim.f90:52-75) the second velocity is v*0 (the model runs in x only)
+ * and vv has rows for h*eta and h*w too; velocity(3:4,n) of that case is never read on the step. */
 static void flux(orc_ctx *c, const double *un)
 {
+    const int nc = c->nc;
     double ht = c->p.htiny;
     for (int n = 0; n < c->np; n++) {
-        double h = un[3 * n];
+        double h = un[nc * n];
         double o = ovh_of(h, ht);
         c->one_over_h[n] = o;
-        double v1 = un[3 * n + 1] * o;
-        double v2 = un[3 * n + 2] * o;
+        double v1 = un[nc * n + 1] * o;
+        double v2 = (nc == 5) ? un[nc * n + 2] * o * 0.0 : un[nc * n + 2] * o;
         c->velocity[2 * n] = v1;
         c->velocity[2 * n + 1] = v2;
-        double *vv = c->vv + 6 * (size_t)n;   /* vv[2*k+d] = vv(k+1,d+1,n) */
-        vv[0] = v1 * h;
-        vv[1] = v2 * h;
-        vv[2] = v1 * un[3 * n + 1];
-        vv[3] = v2 * un[3 * n + 1];
-        vv[4] = v1 * un[3 * n + 2];
-        vv[5] = v2 * un[3 * n + 2];
+        double *vv = c->vv + 2 * (size_t)nc * (size_t)n;   /* vv[2*k+d] = vv(k+1,d+1,n) */
+        for (int k = 0; k < nc; k++) {
+            vv[2 * k] = v1 * un[nc * n + k];
+            vv[2 * k + 1] = v2 * un[nc * n + k];
+        }
     }
 }
 
@@ -373,6 +394,7 @@ static void flux(orc_ctx *c, const double *un)
 /* compute_dij (update.f90:291-326) */
 static int compute_dij(orc_ctx *c, const double *un)
 {
+    const int nc = c->nc;
     int err = 0;
     const double g = c->p.gravity, ht = c->p.htiny;
     for (int i = 0; i < c->np; i++) {
@@ -383,7 +405,7 @@ static int compute_dij(orc_ctx *c, const double *un)
                 double norm = sqrt(0.0 + nij[0] * nij[0] + nij[1] * nij[1]);
                 nij[0] = nij[0] / norm;
                 nij[1] = nij[1] / norm;
-                double lambda = orc_lambda_vacc(un + 3 * i, un + 3 * j, nij, g, ht, &err);
+                double lambda = orc_lambda_vacc(un + nc * i, un + nc * j, nij, g, ht, &err);
                 c->dij[p] = norm * lambda;
             } else {
                 c->dij[p] = 0.0;
@@ -401,18 +423,19 @@ static int compute_dij(orc_ctx *c, const double *un)
 /* compute_muij (update.f90:328-364) */
 static void compute_muij(orc_ctx *c, const double *un)
 {
+    const int nc = c->nc;
     const double ht = c->p.htiny;
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i];
+        double hi = un[nc * i];
         double oi = 2 * hi / (hi * hi + fmaxF(hi, ht) * fmaxF(hi, ht));
-        double ul[2] = {un[3 * i + 1] * oi, un[3 * i + 2] * oi};
+        double ul[2] = {un[nc * i + 1] * oi, un[nc * i + 2] * oi};
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
             int j = c->ja[p];
             if (i != j) {
                 double n1 = c->cij[2 * p], n2 = c->cij[2 * p + 1];
-                double hj = un[3 * j];
+                double hj = un[nc * j];
                 double oj = 2 * hj / (hj * hj + fmaxF(hj, ht) * fmaxF(hj, ht));
-                double ur[2] = {un[3 * j + 1] * oj, un[3 * j + 2] * oj};
+                double ur[2] = {un[nc * j + 1] * oj, un[nc * j + 2] * oj};
                 double sl = 0.0 + n1 * ul[0] + n2 * ul[1];
                 double sr = 0.0 + n1 * ur[0] + n2 * ur[1];
                 c->muij[p] = fmaxF(fmaxF(-sl, 0.0), fmaxF(sr, 0.0));
@@ -431,13 +454,14 @@ static void compute_muij(orc_ctx *c, const double *un)
 /* alpha_limit (update.f90:750-799); alpha**8 with an integer exponent is three squarings */
 static void alpha_limit(orc_ctx *c, const double *un)
 {
+    const int nc = c->nc;
     const double ht = c->p.htiny;
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i];
+        double hi = un[nc * i];
         if (hi <= ht) { c->alpha[i] = 1.0; continue; }
         double num = 0.0, denom = 0.0, a;
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
-            double hj = un[3 * c->ja[p]];
+            double hj = un[nc * c->ja[p]];
             num = num + hj - hi;
             denom = denom + fabs(hj - hi);
         }
@@ -470,9 +494,49 @@ static void alpha_limit(orc_ctx *c, const double *un)
 /* Right-hand sides                                                                               */
 /* ============================================================================================ */
 
-/* friction / my_friction / coriolis dispatch on type_test (update.f90:412-421,614-623,672-681) */
+/* mSGN_RHS (update.f90:488-551), the source of the SGN model (test 13, 5 components): per node
+ * paper_constant = lambdaSGN*g/(3*SQRT(lumped)), x = (h*eta)/h**2, psi = 12*(x-1),
+ * pTilde = paper_constant*h**3*(2 + 4*x**3 - 6*x**4), s = 3*paper_constant*((h*eta)/h)**2*psi; then
+ * per row the momenta lose pTilde(j)*c_ij over the row in CSR order (both directions, as the
+ * reference's k loop runs over k_dim), rk(4,i) gains lumped(i)*(h*w)_i and rk(5,i) loses
+ * lumped(i)*s(i).  The literal integer powers are repeated products from the left, as flang
+ * expands them (x**2 = x*x, x**3 = (x*x)*x, x**4 = ((x*x)*x)*x; pinned by the reference's outputs,
+ * tests/golden/sgn_run.npz), unlike alpha_limit's alpha**exponent with a variable exponent, which
+ * its runtime evaluates by squaring. */
+static void msgn_rhs(orc_ctx *c, const double *un, double *rk)
+{
+    const int nc = c->nc;
+    const double g = c->p.gravity, lam = c->p.lambda_sgn;
+    double *pt = c->sgn_p, *sv = c->sgn_s;
+    for (int n = 0; n < c->np; n++) {
+        double lms = sqrt(c->lumped[n]);
+        double pc = lam * g / (3.0 * lms);
+        double h = un[nc * n], he = un[nc * n + 3];
+        double x = he / (h * h);
+        double psi = 12.0 * (x - 1.0);
+        double x3 = x * x * x;
+        double x4 = x * x * x * x;
+        double h3 = h * h * h;
+        pt[n] = pc * h3 * (2.0 + 4.0 * x3 - 6.0 * x4);
+        double t = he / h;
+        sv[n] = 3.0 * pc * (t * t) * psi;
+    }
+    for (int i = 0; i < c->np; i++) {
+        for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
+            int j = c->ja[p];
+            rk[nc * i + 1] = rk[nc * i + 1] - pt[j] * c->cij[2 * p];
+            rk[nc * i + 2] = rk[nc * i + 2] - pt[j] * c->cij[2 * p + 1];
+        }
+        rk[nc * i + 3] = rk[nc * i + 3] + c->lumped[i] * un[nc * i + 4];
+        rk[nc * i + 4] = rk[nc * i + 4] - c->lumped[i] * sv[i];
+    }
+}
+
+/* friction / my_friction / coriolis / mSGN_RHS dispatch on type_test (update.f90:412-421,614-623,
+ * 672-681) */
 static void sources(orc_ctx *c, const double *un, double *rk)
 {
+    const int nc = c->nc;
     int tt = c->p.type_test;
     if (tt == 9 || tt == 10) {
         /* friction (update.f90:425-452) */
@@ -480,42 +544,45 @@ static void sources(orc_ctx *c, const double *un, double *rk)
         double fric = g * (c->p.mannings * c->p.mannings);
         const double gamma = 4.0 / 3.0;
         for (int i = 0; i < c->np; i++) {
-            double hs = pow(fmaxF(un[3 * i], c->p.htiny), gamma);
+            double hs = pow(fmaxF(un[nc * i], c->p.htiny), gamma);
             double v1 = c->velocity[2 * i], v2 = c->velocity[2 * i + 1];
             double vel = fric * sqrt(v1 * v1 + v2 * v2);
             double hloc = 1.0 * vel * c->dt;
             for (int k = 1; k < 3; k++)
-                rk[3 * i + k] = rk[3 * i + k] - c->lumped[i] * un[3 * i + k] * vel * 2 / (hs + fmaxF(hs, hloc));
+                rk[nc * i + k] = rk[nc * i + k] - c->lumped[i] * un[nc * i + k] * vel * 2 / (hs + fmaxF(hs, hloc));
         }
     } else if (tt == 11) {
         /* my_friction (update.f90:454-469) */
         const double kappa = 0.2;
         for (int i = 0; i < c->np; i++)
             for (int k = 1; k < 3; k++)
-                rk[3 * i + k] = rk[3 * i + k] - c->lumped[i] * kappa * un[3 * i + k];
+                rk[nc * i + k] = rk[nc * i + k] - c->lumped[i] * kappa * un[nc * i + k];
     } else if (tt == 12) {
         /* coriolis (update.f90:471-486) with syst_size = 3 */
         const double f = 2.0;
         for (int i = 0; i < c->np; i++) {
-            rk[3 * i + 1] = rk[3 * i + 1] + c->lumped[i] * f * un[3 * i + 2];
-            rk[3 * i + 2] = rk[3 * i + 2] - c->lumped[i] * f * un[3 * i + 1];
+            rk[nc * i + 1] = rk[nc * i + 1] + c->lumped[i] * f * un[nc * i + 2];
+            rk[nc * i + 2] = rk[nc * i + 2] - c->lumped[i] * f * un[nc * i + 1];
         }
+    } else if (tt == 13) {
+        msgn_rhs(c, un, rk);
     }
 }
 
 /* smb_2_roundoff (update.f90:554-624) */
 static void smb_2_roundoff(orc_ctx *c, const double *un, double *rk)
 {
+    const int nc = c->nc;
     const double g = c->p.gravity, ht = c->p.htiny;
     const double *z = c->bath;
     flux(c, un);
-    memset(rk, 0, sizeof(double) * 3 * (size_t)c->np);
+    memset(rk, 0, sizeof(double) * nc * (size_t)c->np);
     for (int i = 0; i < c->np; i++) c->fix[i] = 0.0;
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i], zi = z[i];
+        double hi = un[nc * i], zi = z[i];
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
             int j = c->ja[p];
-            double hj = un[3 * j], zj = z[j];
+            double hj = un[nc * j], zj = z[j];
             double zmax = fmaxF(zi, zj);
             double Hij = fmaxF(0.0, hi + zi - zmax);
             double Hji = fmaxF(0.0, hj + zj - zmax);
@@ -531,22 +598,22 @@ static void smb_2_roundoff(orc_ctx *c, const double *un, double *rk)
                 c->fix[i] = c->fix[i] + xx;
             } else {
                 c->fix[i] = c->fix[i] - mu + (dd - mu) * (-ratij);
-                rk[3 * i] = rk[3 * i] + (xx + mu) * hj + fmaxF((dd - mu) * Hji, 0.0);
+                rk[nc * i] = rk[nc * i] + (xx + mu) * hj + fmaxF((dd - mu) * Hji, 0.0);
             }
-            /* momentum */
-            for (int k = 1; k < 3; k++) {
-                const double *vvj = c->vv + 6 * (size_t)j + 2 * k;
+            /* momentum (and h*eta, h*w of the SGN model) */
+            for (int k = 1; k < nc; k++) {
+                const double *vvj = c->vv + 2 * (size_t)nc * (size_t)j + 2 * k;
                 xx = 0.0;
                 xx = xx - c1 * (vvj[0]);
                 xx = xx - c2 * (vvj[1]);
-                double ukj = un[3 * j + k], uki = un[3 * i + k];
-                rk[3 * i + k] = rk[3 * i + k] + xx + dd * (ukj * ratji - uki * ratij)
+                double ukj = un[nc * j + k], uki = un[nc * i + k];
+                rk[nc * i + k] = rk[nc * i + k] + xx + dd * (ukj * ratji - uki * ratij)
                               + mu * (ukj * (1.0 - ratji) - uki * (1.0 - ratij));
             }
             /* topography */
             double topo = g * hi * (hj + zj - hi - zi);
-            rk[3 * i + 1] = rk[3 * i + 1] - topo * c1;
-            rk[3 * i + 2] = rk[3 * i + 2] - topo * c2;
+            rk[nc * i + 1] = rk[nc * i + 1] - topo * c1;
+            rk[nc * i + 2] = rk[nc * i + 2] - topo * c2;
         }
     }
     sources(c, un, rk);
@@ -555,16 +622,17 @@ static void smb_2_roundoff(orc_ctx *c, const double *un, double *rk)
 /* smb_2 (update.f90:626-682) */
 static void smb_2(orc_ctx *c, const double *un, double *rk)
 {
+    const int nc = c->nc;
     const double g = c->p.gravity;
     const double *z = c->bath;
     flux(c, un);
-    memset(rk, 0, sizeof(double) * 3 * (size_t)c->np);
+    memset(rk, 0, sizeof(double) * nc * (size_t)c->np);
     for (int i = 0; i < c->np; i++) c->fix[i] = 0.0;
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i], zi = z[i];
+        double hi = un[nc * i], zi = z[i];
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
             int j = c->ja[p];
-            double hj = un[3 * j], zj = z[j];
+            double hj = un[nc * j], zj = z[j];
             double zmax = fmaxF(zi, zj);
             double Hij = fmaxF(0.0, hi + zi - zmax);
             double Hji = fmaxF(0.0, hj + zj - zmax);
@@ -572,18 +640,18 @@ static void smb_2(orc_ctx *c, const double *un, double *rk)
             double ratji = (hj <= 0) ? 0.0 : Hji / hj;
             double c1 = c->cij[2 * p], c2 = c->cij[2 * p + 1];
             double dd = c->dij[p], mu = c->muij[p];
-            for (int k = 0; k < 3; k++) {
-                const double *vvj = c->vv + 6 * (size_t)j + 2 * k;
+            for (int k = 0; k < nc; k++) {
+                const double *vvj = c->vv + 2 * (size_t)nc * (size_t)j + 2 * k;
                 double xx = 0.0;
                 xx = xx - c1 * (vvj[0]);
                 xx = xx - c2 * (vvj[1]);
-                double ukj = un[3 * j + k], uki = un[3 * i + k];
-                rk[3 * i + k] = rk[3 * i + k] + xx + dd * (ukj * ratji - uki * ratij)
+                double ukj = un[nc * j + k], uki = un[nc * i + k];
+                rk[nc * i + k] = rk[nc * i + k] + xx + dd * (ukj * ratji - uki * ratij)
                               + mu * (ukj * (1.0 - ratji) - uki * (1.0 - ratij));
             }
             double topo = g * hi * (hj + zj - hi - zi);
-            rk[3 * i + 1] = rk[3 * i + 1] - topo * c1;
-            rk[3 * i + 2] = rk[3 * i + 2] - topo * c2;
+            rk[nc * i + 1] = rk[nc * i + 1] - topo * c1;
+            rk[nc * i + 2] = rk[nc * i + 2] - topo * c2;
         }
     }
     sources(c, un, rk);
@@ -592,15 +660,16 @@ static void smb_2(orc_ctx *c, const double *un, double *rk)
 /* smb_1 (update.f90:366-423) */
 static void smb_1(orc_ctx *c, const double *un, double *rk)
 {
+    const int nc = c->nc;
     const double g = c->p.gravity, ht = c->p.htiny;
     const double *z = c->bath;
     flux(c, un);
-    memset(rk, 0, sizeof(double) * 3 * (size_t)c->np);
+    memset(rk, 0, sizeof(double) * nc * (size_t)c->np);
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i], zi = z[i];
+        double hi = un[nc * i], zi = z[i];
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
             int j = c->ja[p];
-            double hj = un[3 * j], zj = z[j];
+            double hj = un[nc * j], zj = z[j];
             double zmax = fmaxF(zi, zj);
             double Hij = fmaxF(0.0, hi + zi - zmax);
             double Hji = fmaxF(0.0, hj + zj - zmax);
@@ -608,17 +677,17 @@ static void smb_1(orc_ctx *c, const double *un, double *rk)
             double ratji = (hj <= ht) ? 0.0 : Hji / hj;
             double c1 = c->cij[2 * p], c2 = c->cij[2 * p + 1];
             double dd = c->dij[p];
-            for (int k = 0; k < 3; k++) {
-                const double *vvj = c->vv + 6 * (size_t)j + 2 * k;
-                const double *vvi = c->vv + 6 * (size_t)i + 2 * k;
+            for (int k = 0; k < nc; k++) {
+                const double *vvj = c->vv + 2 * (size_t)nc * (size_t)j + 2 * k;
+                const double *vvi = c->vv + 2 * (size_t)nc * (size_t)i + 2 * k;
                 double xx = 0.0;
                 xx = xx - c1 * (vvj[0] * ratji + vvi[0] * ratij);
                 xx = xx - c2 * (vvj[1] * ratji + vvi[1] * ratij);
-                rk[3 * i + k] = rk[3 * i + k] + xx + dd * (un[3 * j + k] * ratji - un[3 * i + k] * ratij);
+                rk[nc * i + k] = rk[nc * i + k] + xx + dd * (un[nc * j + k] * ratji - un[nc * i + k] * ratij);
             }
             double t = 0.5 * g * (Hji * Hji - Hij * Hij);
-            rk[3 * i + 1] = rk[3 * i + 1] - t * c1;
-            rk[3 * i + 2] = rk[3 * i + 2] - t * c2;
+            rk[nc * i + 1] = rk[nc * i + 1] - t * c1;
+            rk[nc * i + 2] = rk[nc * i + 2] - t * c2;
         }
     }
     sources(c, un, rk);
@@ -626,41 +695,44 @@ static void smb_1(orc_ctx *c, const double *un, double *rk)
 
 static void divide_by_lumped(orc_ctx *c, double *rk)
 {
+    const int nc = c->nc;
     for (int i = 0; i < c->np; i++)
-        for (int k = 0; k < 3; k++) rk[3 * i + k] = rk[3 * i + k] / c->lumped[i];
+        for (int k = 0; k < nc; k++) rk[nc * i + k] = rk[nc * i + k] / c->lumped[i];
 }
 
 /* check_hmin (update.f90:810-822) */
 static int check_hmin(orc_ctx *c, const double *u)
 {
+    const int nc = c->nc;
     int tt = c->p.type_test;
     if (tt < 1 || tt > 13) return 0;
     for (int i = 0; i < c->np; i++)
-        if (u[3 * i] < 0.0) return ORC_ENEG_H;
+        if (u[nc * i] < 0.0) return ORC_ENEG_H;
     return 0;
 }
 
 /* entropy_residual (update.f90:825-878) with maxmin (924-935) */
 static void entropy_residual(orc_ctx *c, const double *un)
 {
+    const int nc = c->nc;
     const double g = c->p.gravity, ht = c->p.htiny, dt = c->dt;
     double *rk = c->rk2;
     double *un2 = c->unext2;
     smb_2(c, un, rk);
     divide_by_lumped(c, rk);
-    for (size_t q = 0; q < 3 * (size_t)c->np; q++) un2[q] = un[q] + dt * rk[q];
+    for (size_t q = 0; q < nc * (size_t)c->np; q++) un2[q] = un[q] + dt * rk[q];
     double *Ent = c->ent;
     for (int n = 0; n < c->np; n++) {
-        double h = un[3 * n], hn = un2[3 * n];
+        double h = un[nc * n], hn = un2[nc * n];
         double on = ovh_of(hn, ht);
-        double vn1 = un2[3 * n + 1] * on, vn2 = un2[3 * n + 2] * on;
+        double vn1 = un2[nc * n + 1] * on, vn2 = un2[nc * n + 2] * on;
         double v1 = c->velocity[2 * n], v2 = c->velocity[2 * n + 1];
         double E = g * (h * h);
         double En = g * (hn * hn);
-        E = E + v1 * un[3 * n + 1];
-        En = En + vn1 * un2[3 * n + 1];
-        E = E + v2 * un[3 * n + 2];
-        En = En + vn2 * un2[3 * n + 2];
+        E = E + v1 * un[nc * n + 1];
+        En = En + vn1 * un2[nc * n + 1];
+        E = E + v2 * un[nc * n + 2];
+        En = En + vn2 * un2[nc * n + 2];
         E = 0.5 * E;
         En = 0.5 * En;
         double Ep0 = g * h;
@@ -668,9 +740,9 @@ static void entropy_residual(orc_ctx *c, const double *un)
         Ep0 = Ep0 - 0.5 * (v2 * v2);
         double m = c->lumped[n];
         double r = m * (En - E) / dt;
-        r = r - m * rk[3 * n] * Ep0;
-        r = r - m * rk[3 * n + 1] * v1;
-        r = r - m * rk[3 * n + 2] * v2;
+        r = r - m * rk[nc * n] * Ep0;
+        r = r - m * rk[nc * n + 1] * v1;
+        r = r - m * rk[nc * n + 2] * v2;
         Ent[n] = E;
         c->res[n] = r;
     }
@@ -694,12 +766,13 @@ static void entropy_residual(orc_ctx *c, const double *un)
 /* compute_fct_matrix (update.f90:721-748) */
 static void compute_fct_matrix(orc_ctx *c, const double *un)
 {
+    const int nc = c->nc;
     const double *z = c->bath;
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i], zi = z[i];
+        double hi = un[nc * i], zi = z[i];
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
             int j = c->ja[p];
-            double hj = un[3 * j], zj = z[j];
+            double hj = un[nc * j], zj = z[j];
             double zmax = fmaxF(zi, zj);
             double Hij = fmaxF(0.0, hi + zi - zmax);
             double Hji = fmaxF(0.0, hj + zj - zmax);
@@ -773,27 +846,26 @@ void orc_fct_generic(int np, const int *ia, const int *ja, const double *f, cons
 /* apply_viscosity (update.f90:684-718) */
 static void apply_viscosity(orc_ctx *c, const double *un, double *rk)
 {
+    const int nc = c->nc;
     const double *z = c->bath;
     for (int i = 0; i < c->np; i++) {
-        double hi = un[3 * i], zi = z[i];
-        double xx[3] = {0.0, 0.0, 0.0};
+        double hi = un[nc * i], zi = z[i];
+        double xx[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int p = c->ia[i]; p < c->ia[i + 1]; p++) {
             int j = c->ja[p];
-            double hj = un[3 * j], zj = z[j];
+            double hj = un[nc * j], zj = z[j];
             double zmax = fmaxF(zi, zj);
             double Hij = fmaxF(0.0, hi + zi - zmax);
             double Hji = fmaxF(0.0, hj + zj - zmax);
             double ratij = (hi <= 0.0) ? 0.0 : Hij / hi;
             double ratji = (hj <= 0.0) ? 0.0 : Hji / hj;
-            for (int k = 0; k < 3; k++) {
-                double ukj = un[3 * j + k], uki = un[3 * i + k];
+            for (int k = 0; k < nc; k++) {
+                double ukj = un[nc * j + k], uki = un[nc * i + k];
                 xx[k] = xx[k] + c->dij[p] * (ukj * ratji - uki * ratij)
                       + c->muij[p] * (ukj * (1.0 - ratji) - uki * (1.0 - ratij));
             }
         }
-        rk[3 * i] = xx[0];
-        rk[3 * i + 1] = xx[1];
-        rk[3 * i + 2] = xx[2];
+        for (int k = 0; k < nc; k++) rk[nc * i + k] = xx[k];
     }
 }
 
@@ -801,25 +873,34 @@ static void apply_viscosity(orc_ctx *c, const double *un, double *rk)
 /* euler (update.f90:134-289)                                                                     */
 /* ============================================================================================ */
 
+/* un(2:,:) + dt*rk(2:,:): every component after the depth */
+static void update_rest(orc_ctx *c, const double *un, const double *rk, double dt, double *unext)
+{
+    const int nc = c->nc;
+    for (int i = 0; i < c->np; i++)
+        for (int k = 1; k < nc; k++) unext[nc * i + k] = un[nc * i + k] + dt * rk[nc * i + k];
+}
+
 int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *dij_out)
 {
+    const int nc = c->nc;
     const int N = c->np;
     const double *L = c->lumped;
     double *rk = c->rk;
     int vt = c->p.viscosity_type, ty = c->p.viscous_type;
     int err = 0;
     c->dt = dt;
+    /* with the SGN model's 5 components entropy_residual reads Entprime(4:5,:), which the reference
+     * never sets (update.f90:853-862): only the galerkin and first-order schemes are defined */
+    if (nc == 5 && (vt == ORC_ENTROPY_VISC || vt == ORC_FCT)) return ORC_EBADOPT;
 
     if (vt == ORC_GALERKIN) {
         memset(c->dij, 0, sizeof(double) * (size_t)c->nnz);
         memset(c->muij, 0, sizeof(double) * (size_t)c->nnz);
         smb_2(c, un, rk);
         divide_by_lumped(c, rk);
-        for (int i = 0; i < N; i++) {
-            unext[3 * i] = fmaxF(un[3 * i] + dt * rk[3 * i], 0.0);
-            unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
-            unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
-        }
+        for (int i = 0; i < N; i++) unext[nc * i] = fmaxF(un[nc * i] + dt * rk[nc * i], 0.0);
+        update_rest(c, un, rk, dt, unext);
         goto done;
     }
 
@@ -837,19 +918,13 @@ int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *di
     divide_by_lumped(c, rk);
 
     if (ty == 1) {
-        for (size_t q = 0; q < 3 * (size_t)N; q++) unext[q] = un[q] + dt * rk[q];
+        for (size_t q = 0; q < nc * (size_t)N; q++) unext[q] = un[q] + dt * rk[q];
     } else if (c->p.roundoff_fix) {
-        for (int i = 0; i < N; i++) {
-            unext[3 * i] = un[3 * i] * (1 + dt * c->fix[i] / L[i]) + dt * rk[3 * i];
-            unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
-            unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
-        }
+        for (int i = 0; i < N; i++) unext[nc * i] = un[nc * i] * (1 + dt * c->fix[i] / L[i]) + dt * rk[nc * i];
+        update_rest(c, un, rk, dt, unext);
     } else {
-        for (int i = 0; i < N; i++) {
-            unext[3 * i] = fmaxF(un[3 * i] + dt * rk[3 * i], 0.0);
-            unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
-            unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
-        }
+        for (int i = 0; i < N; i++) unext[nc * i] = fmaxF(un[nc * i] + dt * rk[nc * i], 0.0);
+        update_rest(c, un, rk, dt, unext);
     }
     if (vt == ORC_VISCOUS) goto done;
 
@@ -869,15 +944,15 @@ int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *di
         divide_by_lumped(c, rk);
         if (c->p.roundoff_fix) {
             for (int i = 0; i < N; i++) {
-                unext[3 * i] = un[3 * i] * (1 + dt * c->fix[i] / L[i]) + dt * rk[3 * i];
-                unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
-                unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+                unext[nc * i] = un[nc * i] * (1 + dt * c->fix[i] / L[i]) + dt * rk[nc * i];
+                unext[nc * i + 1] = un[nc * i + 1] + dt * rk[nc * i + 1];
+                unext[nc * i + 2] = un[nc * i + 2] + dt * rk[nc * i + 2];
             }
         } else {
             for (int i = 0; i < N; i++) {
-                unext[3 * i] = fmaxF(un[3 * i] + dt * rk[3 * i], 0.0);
-                unext[3 * i + 1] = un[3 * i + 1] + dt * rk[3 * i + 1];
-                unext[3 * i + 2] = un[3 * i + 2] + dt * rk[3 * i + 2];
+                unext[nc * i] = fmaxF(un[nc * i] + dt * rk[nc * i], 0.0);
+                unext[nc * i + 1] = un[nc * i + 1] + dt * rk[nc * i + 1];
+                unext[nc * i + 2] = un[nc * i + 2] + dt * rk[nc * i + 2];
             }
         }
     } else if (vt == ORC_FCT) {
@@ -885,7 +960,7 @@ int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *di
         if (err) return err;
         if (ty != 2) return ORC_EBADOPT;
         compute_fct_matrix(c, un);
-        for (int i = 0; i < N; i++) c->scaled[i] = (L[i] / dt) * unext[3 * i];
+        for (int i = 0; i < N; i++) c->scaled[i] = (L[i] / dt) * unext[nc * i];
         orc_fct_positivity(N, c->ia, c->ja, c->fctmat, c->scaled, c->p.htiny, c->lij);
         for (int p = 0; p < c->nnz; p++) {
             c->dij[p] = (c->dij[p] - c->dijL[p]) * c->lij[p];
@@ -893,7 +968,7 @@ int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *di
         }
         apply_viscosity(c, un, rk);
         divide_by_lumped(c, rk);
-        for (size_t q = 0; q < 3 * (size_t)N; q++) unext[q] = unext[q] + dt * rk[q];
+        for (size_t q = 0; q < nc * (size_t)N; q++) unext[q] = unext[q] + dt * rk[q];
     } else {
         return ORC_EBADOPT;
     }
@@ -1001,20 +1076,52 @@ double orc_sin_rn(double x)
 }
 
 /* Dirichlet overwrite of bdy (main.f90:176-188) for the problems driven by this checker */
-static void bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double period, double level,
-                int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
+static int bdy(orc_ctx *c, double *u, double t, int bc_kind, double amp, double period, double level,
+               int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
 {
+    const int nc = c->nc;
+    if (bc_kind == 3) {
+        /* the reference's bdy: sol_anal at the listed nodes (main.f90:180-187) */
+        const int tt = c->p.type_test;
+        const double g = c->p.gravity, ht = c->p.htiny;
+        int ok = 1;
+        if (!c->rr) return ORC_EBADOPT;
+        for (int q = 0; q < nh; q++) {
+            int n = nodes_h[q];
+            u[nc * n] = orc_sol_anal(tt, 1, c->rr[2 * n], c->rr[2 * n + 1], t, g, ht, &ok);
+        }
+        for (int q = 0; q < nux; q++) {
+            int n = nodes_ux[q];
+            u[nc * n + 1] = orc_sol_anal(tt, 2, c->rr[2 * n], c->rr[2 * n + 1], t, g, ht, &ok);
+        }
+        for (int q = 0; q < nuy; q++) {
+            int n = nodes_uy[q];
+            u[nc * n + 2] = orc_sol_anal(tt, 3, c->rr[2 * n], c->rr[2 * n + 1], t, g, ht, &ok);
+        }
+        if (tt == 13) {
+            for (int q = 0; q < nux; q++) {
+                int n = nodes_ux[q];
+                u[nc * n + 3] = orc_sol_anal(tt, 4, c->rr[2 * n], c->rr[2 * n + 1], t, g, ht, &ok);
+            }
+            for (int q = 0; q < nuy; q++) {
+                int n = nodes_uy[q];
+                u[nc * n + 4] = orc_sol_anal(tt, 5, c->rr[2 * n], c->rr[2 * n + 1], t, g, ht, &ok);
+            }
+        }
+        return ok ? 0 : ORC_EBADOPT;
+    }
     if (bc_kind == 2) {
         const double pi = acos(-1.0);
         for (int q = 0; q < nh; q++) {
             int n = nodes_h[q];
-            u[3 * n] = fmaxF(level - c->bath[n] + amp * orc_sin_rn(2.0 * pi * t / period), 0.0);
+            u[nc * n] = fmaxF(level - c->bath[n] + amp * orc_sin_rn(2.0 * pi * t / period), 0.0);
         }
     }
     if (bc_kind >= 1) {
-        for (int q = 0; q < nux; q++) u[3 * nodes_ux[q] + 1] = 0.0;
-        for (int q = 0; q < nuy; q++) u[3 * nodes_uy[q] + 2] = 0.0;
+        for (int q = 0; q < nux; q++) u[nc * nodes_ux[q] + 1] = 0.0;
+        for (int q = 0; q < nuy; q++) u[nc * nodes_uy[q] + 2] = 0.0;
     }
+    return 0;
 }
 
 /* SSP-RK3 time loop body of main.f90:71-91, repeated nsteps times */
@@ -1023,30 +1130,39 @@ int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
             int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy)
 {
     if (bc_kind == 0 && (nh || nux || nuy)) return ORC_EBADOPT;
-    size_t n3 = 3 * (size_t)c->np;
+    size_t n3 = (size_t)c->nc * (size_t)c->np;
     double *uo = (double *)xmalloc(sizeof(double) * n3);
     double *ui = (double *)xmalloc(sizeof(double) * n3);
     double t = *time_io;
     int err = 0;
     for (int it = 0; it < nsteps && !err; it++) {
-        double dt = orc_compute_dt(c, un, &err);
+        double dt;
+        if (c->p.type_test == 9) {          /* main.f90:66-69: fixed htiny and dt, no COMPUTE_DT */
+            c->p.htiny = 1.0e-3;
+            dt = 2.5e-2;
+        } else {
+            dt = orc_compute_dt(c, un, &err);
+        }
         if (err) break;
         if (dts) dts[it] = dt;
         double to = t;
         memcpy(uo, un, sizeof(double) * n3);
         err = orc_euler(c, uo, un, dt, NULL);
         if (err) break;
-        bdy(c, un, t + dt, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        err = bdy(c, un, t + dt, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        if (err) break;
         t = to + dt;
         err = orc_euler(c, un, ui, dt, NULL);
         if (err) break;
         for (size_t q = 0; q < n3; q++) un[q] = (3 * uo[q] + ui[q]) / 4;
-        bdy(c, un, t + dt / 2, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        err = bdy(c, un, t + dt / 2, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        if (err) break;
         t = to + dt / 2;
         err = orc_euler(c, un, ui, dt, NULL);
         if (err) break;
         for (size_t q = 0; q < n3; q++) un[q] = (uo[q] + 2 * ui[q]) / 3;
-        bdy(c, un, t + dt, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        err = bdy(c, un, t + dt, bc_kind, amp, period, level, nh, nodes_h, nux, nodes_ux, nuy, nodes_uy);
+        if (err) break;
         t = to + dt;
     }
     *time_io = t;
@@ -1205,6 +1321,25 @@ double orc_sol_anal(int type_test, int k, double x, double y, double t, double g
         if (k == 1) return vv;
         if (k == 2) return -(vv * delta * eta * c);
         return vv * eta * sn / w;
+    }
+    if (type_test == 13) {           /* condlim.f90:751-827; the case sets gravity to 9.81 */
+        const double g = 9.81, h1 = 10.0 / 100.0, h2 = 11.0 / 100.0, x0 = 2.75;
+        const double D = sqrt(g * h2);
+        const double zz = sqrt((3.0 * (h2 - h1)) / (h2 * (h1 * h1)));
+        const double arg = 1.0 / 2.0 * zz * (x - x0 - D * t);
+        const double sech = 1.0 / cosh(arg);
+        const double htilde = h1 + (h2 - h1) * (sech * sech);
+        const double vv = fmaxF(htilde, 0.0);
+        (void)y;
+        if (k == 1) return vv;
+        if (k == 2) return vv * D - h1 * D;
+        if (k == 3) return vv * 0.0;
+        if (t <= 1.0e-10) {
+            if (k == 4) return vv * vv;
+            if (k == 5) return -(D * h1 * ((h1 - h2) * zz * (sech * sech) * tanh(arg)));
+        }
+        *ok = 0;             /* h*eta and h*w after t = 1e-10 are undefined in the reference */
+        return 0.0;
     }
     *ok = 0;
     return 0.0;

@@ -6,8 +6,9 @@
  * the product never links, loads or calls it.
  *
  * Conventions: 0-based CSR (row i holds entries ia[i]..ia[i+1]-1, columns ascending, diagonal
- * included, as st_csr builds them); state u is the reference's un(3,np) in Fortran order, i.e.
- * u[3*i+k] for node i and component k (h, qx, qy); c_ij is interleaved c[2*p+d].
+ * included, as st_csr builds them); state u is the reference's un(nc,np) in Fortran order, i.e.
+ * u[nc*i+k] for node i and component k (h, qx, qy; then h*eta, h*w for the SGN model of test 13,
+ * nc = 5, as main.f90:22-26 sets syst_size); c_ij is interleaved c[2*p+d].
  * Every floating-point expression keeps the reference's left-to-right evaluation order and is
  * compiled without FMA contraction.
  */
@@ -29,6 +30,7 @@ typedef struct orc_params {
     int viscous_type;     /* 1 = 'type1', 2 = 'type2' */
     int alpha_limit;      /* inputs%if_alpha_limit */
     int roundoff_fix;     /* if_roundoff_fix (SOURCES/update.f90:15 hard-codes .TRUE.) */
+    double lambda_sgn;    /* inputs%lambdaSGN (SOURCES/read_data.f90:92-95), test 13 only */
 } orc_params;
 
 typedef struct orc_ctx orc_ctx;
@@ -43,6 +45,10 @@ orc_ctx *orc_create(int np, int nnz, const int *ia, const int *ja, const double 
                     const double *lumped, const double *bath, const orc_params *prm);
 void orc_destroy(orc_ctx *c);
 void orc_set_params(orc_ctx *c, const orc_params *prm);
+/* number of components of the context's states (3, or 5 for type_test 13) */
+int orc_syst_size(const orc_ctx *c);
+/* node coordinates rr(2,np), needed by the bc_kind 3 boundary program of orc_run */
+void orc_set_coords(orc_ctx *c, const double *rr);
 
 /* pointwise pieces */
 double orc_lambda_vacc(const double *ul, const double *ur, const double *nij,
@@ -60,7 +66,9 @@ int orc_euler(orc_ctx *c, const double *un, double *unext, double dt, double *di
 
 /* Fixed-step SSP-RK3 loop of SOURCES/main.f90:61-92.  bc_kind 0: no Dirichlet lists allowed;
  * 1: walls (qx = 0 on nodes_ux, qy = 0 on nodes_uy); 2: walls plus the inflow
- * h = max(level - z + amp*orc_sin_rn(2*pi*t/period), 0) on nodes_h.  dts receives dt per step. */
+ * h = max(level - z + amp*orc_sin_rn(2*pi*t/period), 0) on nodes_h; 3: the reference's bdy
+ * (main.f90:176-188), sol_anal of the context's type_test at the listed nodes (orc_set_coords
+ * first).  dts receives dt per step. */
 int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
             int bc_kind, double amp, double period, double level,
             int nh, const int *nodes_h, int nux, const int *nodes_ux, int nuy, const int *nodes_uy);
@@ -68,8 +76,9 @@ int orc_run(orc_ctx *c, double *un, int nsteps, double *time_io, double *dts,
 /* sin(x) rounded to nearest (double-double evaluation, one final rounding), |x| < 2^29. */
 double orc_sin_rn(double x);
 
-/* sol_anal(k, (x, y), t) (SOURCES/condlim.f90:311-831) for the analytic tests 3 to 7, 11 and 12,
- * components k = 1 (h), 2 (qx), 3 (qy); *ok = 0 for other tests. */
+/* sol_anal(k, (x, y), t) (SOURCES/condlim.f90:311-831) for the analytic tests 3 to 7, 11, 12 and
+ * 13, components k = 1 (h), 2 (qx), 3 (qy) (and for test 13 k = 4 (h*eta), 5 (h*w) at t <= 1e-10,
+ * the only times the reference defines them); *ok = 0 for other cases. */
 double orc_sol_anal(int type_test, int k, double x, double y, double t, double gravity, double htiny, int *ok);
 
 /* The quadrature norms of compute_errors (SOURCES/main.f90:198-263): r_gauss (303-322),

@@ -1,4 +1,5 @@
-"""Build libswe_amd.so in place with hipcc for gfx950.
+"""Build libswe_amd.so in place with hipcc for gfx950 (and the host-only mesh generator
+libswe_meshgen.so with g++).
 
     python shallow-water-eric_amd/build_lib.py
 
@@ -35,15 +36,30 @@ def command(out=OUT, extra=()):
             "-I", os.path.join(REPO, "include"), "-o", out, *extra, *SOURCES]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = OUT + ".tmp"
-    cmd = command(tmp)
+# The mesh generator of the benchmark workloads (host C++ only, no GPU code): libswe_meshgen.so.
+MESHGEN_OUT = os.path.join(PKG, "libswe_meshgen.so")
+MESHGEN_SRC = os.path.join(PKG, "meshgen", "swe_meshgen.cpp")
+
+
+def meshgen_command(out=MESHGEN_OUT):
+    cxx = shutil.which("g++") or "g++"
+    return [cxx, "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-o", out, MESHGEN_SRC, "-lpthread"]
+
+
+def _run(cmd, tmp, out, verbose):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed (%d):\n%s\n%s" % (r.returncode, " ".join(cmd), r.stderr[-4000:]))
-    os.replace(tmp, OUT)
+        raise RuntimeError("%s failed (%d):\n%s\n%s" % (cmd[0], r.returncode, " ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, out)
     if verbose:
         print(" ".join(cmd))
+
+
+def build(verbose: bool = False) -> str:
+    tmp = OUT + ".tmp"
+    _run(command(tmp), tmp, OUT, verbose)
+    tmp = MESHGEN_OUT + ".tmp"
+    _run(meshgen_command(tmp), tmp, MESHGEN_OUT, verbose)
     return OUT
 
 

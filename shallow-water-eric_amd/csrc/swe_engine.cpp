@@ -103,9 +103,21 @@ int make_consts(const swe_params& p, Consts& K) {
         return fail(SWE_EBADOPT, "Bug: entropy viscosity programmed only with type 2");
     if (p.viscosity_type == SWE_FCT && p.viscous_type != 2)
         return fail(SWE_EBADOPT, "BUG: FCT programmed with type2 only");
+    // compute_lambda_vacc knows tests 1 to 13 and STOPs on any other (SOURCES/condlim.f90:897,952-954)
+    if (p.type_test < 1 || p.type_test > 13)
+        return fail(SWE_EBADOPT, "BUG in compute_lambda: type_test %d is not one of 1..13", p.type_test);
+    // test 13 is the 5-component SGN system (SOURCES/main.f90:24-26, update.f90:619-620); this
+    // library steps the 3-component shallow-water state only
+    if (p.type_test == 13)
+        return fail(SWE_EBADOPT, "type_test 13 (SGN model, 5 components) is not programmed for the GPU step");
     K = Consts{};
     K.g = p.gravity;
-    K.htiny = p.htiny;
+    // test 9: the driver sets htiny = 1e-3 and dt = 2.5e-2 at the top of every step instead of
+    // calling COMPUTE_DT (SOURCES/main.f90:67-69); the library uses that htiny in every call of this
+    // test and that dt in its own time loop (the only reference call with another htiny is the
+    // COMPUTE_DT before the loop, main.f90:40, whose dt the loop replaces)
+    K.htiny = (p.type_test == 9) ? 1.0e-3 : p.htiny;
+    K.dt_fixed = (p.type_test == 9) ? 2.5e-2 : 0.0;
     K.cfl = p.cfl;
     K.max_water_h = p.max_water_h;
     K.mannings = p.mannings;
@@ -119,7 +131,7 @@ int make_consts(const swe_params& p, Consts& K) {
     K.fric = p.gravity * (p.mannings * p.mannings);
     K.ent_cst = 1.0e-10 * p.gravity * (p.max_water_h * p.max_water_h);
     K.type_test = p.type_test;
-    K.check_h = (p.type_test >= 1 && p.type_test <= 13) ? 1 : 0;
+    K.check_h = 1;
     K.src = (p.type_test == 9 || p.type_test == 10) ? SRC_MANNING
             : (p.type_test == 11)                   ? SRC_LINEAR
             : (p.type_test == 12)                   ? SRC_CORIOLIS
@@ -1016,6 +1028,14 @@ int pool_tune(Pool* P, int steps, int trials) {
     const std::string name0 = P->timed_name;
     hipError_t e = snapshot();
     rc = (e == hipSuccess) ? SWE_OK : fail(SWE_EHIP, "tuning snapshot: %s", hipGetErrorString(e));
+    // true from the start of a trial run until its restore is queued: every exit after a good
+    // snapshot gives the instances back their states, times, step counts and flags
+    bool dirty = false;
+    // SWE_TUNE_FAIL_AT=n makes the n-th trial fail after its steps are queued (a host-side failure
+    // injection for the test of the restore path; unset or 0: never)
+    const char* fa = std::getenv("SWE_TUNE_FAIL_AT");
+    const int fail_at = fa ? std::atoi(fa) : 0;
+    int trial_no = 0;
     const int blocks[3] = {128, 256, 512};
     for (int kid : kids) {
         if (rc) break;
@@ -1030,7 +1050,12 @@ int pool_tune(Pool* P, int steps, int trials) {
                     P->timing = true;
                     P->timed_name = kKernelNames[kid];
                     P->tk.used = 0;
+                    dirty = true;
                     if ((rc = pool_queue_steps(P, steps))) break;
+                    if (++trial_no == fail_at) {
+                        rc = fail(SWE_EHIP, "tuning run: failure injected at trial %d (SWE_TUNE_FAIL_AT)", trial_no);
+                        break;
+                    }
                     if ((e = hipStreamSynchronize(P->stream)) != hipSuccess) {
                         rc = fail(SWE_EHIP, "tuning run: %s", hipGetErrorString(e));
                         break;
@@ -1043,6 +1068,7 @@ int pool_tune(Pool* P, int steps, int trials) {
                     }
                     ms_min = std::min(ms_min, sum);
                     if ((e = restore()) != hipSuccess) rc = fail(SWE_EHIP, "tuning restore: %s", hipGetErrorString(e));
+                    else dirty = false;
                 }
                 if (ms_min < best_ms) {
                     best_ms = ms_min;
@@ -1055,6 +1081,16 @@ int pool_tune(Pool* P, int steps, int trials) {
     P->timing = timing0;
     P->timed_name = name0;
     P->tk.used = 0;
+    if (dirty) {
+        // a trial failed part-way: wait for what it queued, then restore the snapshot; a failed
+        // restore is reported on its own (the tuning error is kept in the message)
+        (void)hipStreamSynchronize(P->stream);
+        const std::string why = last_error();
+        e = restore();
+        if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+        if (e != hipSuccess)
+            rc = fail(SWE_EHIP, "tuning restore after a failed trial (%s): %s", why.c_str(), hipGetErrorString(e));
+    }
     (void)hipStreamSynchronize(P->stream);
     (void)hipFree(tab_save);
     (void)hipFree(flags_save);

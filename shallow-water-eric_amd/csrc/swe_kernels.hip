@@ -384,7 +384,8 @@ __global__ void k_finalize_dt(InstDev* __restrict__ T, unsigned long long* __res
         D.state = flags[k] ? INST_FAILED : INST_DONE;
         return;
     }
-    const double dt = D.K.cfl * 1.0 / mx;
+    // test 9 takes the driver's fixed dt and skips COMPUTE_DT (main.f90:66-71)
+    const double dt = (D.K.dt_fixed > 0.0) ? D.K.dt_fixed : D.K.cfl * 1.0 / mx;
     const double to = D.time;
     D.dt = dt;
     if (D.bc_kind == SWE_BC_INFLOW) {

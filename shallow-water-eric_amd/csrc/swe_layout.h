@@ -32,8 +32,9 @@ struct Consts {
     double g16;         // 16*g
     double fric;        // g*mannings**2
     double ent_cst;     // 1e-10*g*max_water_h**2
+    double dt_fixed;    // > 0: the time loop's dt instead of the CFL one (test 9, SOURCES/main.f90:68)
     int type_test;
-    int check_h;        // check_hmin is active for type_test 1..13
+    int check_h;        // check_hmin (SOURCES/update.f90:810-822) after every stage
     int src;            // source term of type_test (SrcKind)
     int pad_;
 };

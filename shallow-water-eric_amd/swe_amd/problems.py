@@ -68,9 +68,14 @@ class Problem:
         return s
 
 
-def _finish(name, raw_mesh, bath_fn, u0_fn, params, **kw):
+def _finish(name, raw_mesh, bath_fn, u0_fn, params, host_assembly=True, **kw):
+    """host_assembly=False leaves the matrices (ia, ja, diag, lumped, cij) out: such a problem is
+    solved through solver(device_setup=True), which assembles on the GPU."""
     m = M.renumber_first_touch(raw_mesh)
-    ia, ja, diag, _mass, lumped, cij = assemble(m.jj, m.rr)
+    if host_assembly:
+        ia, ja, diag, _mass, lumped, cij = assemble(m.jj, m.rr)
+    else:
+        ia = ja = diag = lumped = cij = None
     bath = bath_fn(m.rr)
     u0 = u0_fn(m.rr, bath)
     return Problem(name=name, mesh=m, ia=ia, ja=ja, diag=diag, lumped=lumped, cij=cij, bath=bath,
@@ -142,10 +147,13 @@ def bump_field(rr, L, nbumps=64, seed=20261202):
 
 def bumps(n_side=4097, L=1000.0, nbumps=64, seed=20261202, cfl=0.25, amp=0.05, period=20.0,
           viscosity="fct", alpha=True) -> Problem:
-    """Still water of level 1 over 64 Gaussian bumps (dry islands where z > 1) on a Delaunay mesh
-    of n_side^2 jittered lattice points of [0,L]^2; free-surface inflow
-    h = max(1 - z + amp*sin(2*pi*t/period), 0) on the west side, walls elsewhere.
-    n_side = 4097 gives 16,785,409 nodes (the 16.8M-node workload), 200 steps."""
+    """Still water of level 1 over 64 Gaussian bumps (dry islands where z > 1) on a cell-split
+    jittered lattice of n_side^2 points of [0,L]^2 (each lattice cell split along the diagonal the
+    in-circle test picks: the Delaunay triangulation of the jittered points, with the lattice's
+    structured topology and uniform valence); free-surface inflow
+    h = max(1 - z + amp*sin(2*pi*t/period), 0) on the west side, walls elsewhere.  n_side = 4097
+    gives 16,785,409 nodes (the lattice form of the 16.8M-node workload; bumps_unstructured is the
+    unstructured one), 200 steps."""
     params = dict(gravity=9.81, htiny=1.0e-4 * 1.0, cfl=cfl, max_water_h=1.0, viscosity=viscosity,
                   viscous_type=2, alpha_limit=alpha, type_test=4, mannings=0.0)
 
@@ -158,11 +166,31 @@ def bumps(n_side=4097, L=1000.0, nbumps=64, seed=20261202, cfl=0.25, amp=0.05, p
                    h_sides=(4,), ux_sides=(2,), uy_sides=(1, 3), bc=(BC_INFLOW, amp, period, 1.0), nsteps=200)
 
 
+def bumps_unstructured(n_nodes=16_785_409, L=1000.0, nbumps=64, seed=20261202, cfl=0.25, amp=0.05, period=20.0,
+                       viscosity="fct", alpha=True, host_assembly=True) -> Problem:
+    """The 16.8M-node workload on an unstructured mesh: still water of level 1 over 64 Gaussian
+    bumps (dry islands where z > 1), the sinusoidal west inflow, walls elsewhere, FCT + alpha,
+    200 steps, on the Delaunay triangulation of a seeded Poisson-disk point set of about n_nodes
+    nodes (mesh.poisson_delaunay_mesh), renumbered by the reference loader's first-touch rule."""
+    params = dict(gravity=9.81, htiny=1.0e-4 * 1.0, cfl=cfl, max_water_h=1.0, viscosity=viscosity,
+                  viscous_type=2, alpha_limit=alpha, type_test=4, mannings=0.0)
+
+    def u0(rr, z):
+        u = np.zeros((rr.shape[0], 3))
+        u[:, 0] = np.maximum(1.0 - z, 0.0)
+        return u
+    return _finish("bumps_pd", M.poisson_delaunay_mesh(L, n_nodes=n_nodes, seed=seed),
+                   lambda rr: bump_field(rr, L, nbumps, seed), u0, params,
+                   h_sides=(4,), ux_sides=(2,), uy_sides=(1, 3), bc=(BC_INFLOW, amp, period, 1.0), nsteps=200,
+                   host_assembly=host_assembly)
+
+
 # ----------------------------------------------------------------------------------------------
 def family_member(n_nodes, seed, kind="bumps", L=1000.0, tfinal=None, amp=0.05, period=20.0, zmax=0.7,
-                  cfl=0.25) -> Problem:
-    """One mesh of the 16.8M-node workload's family at another size: a jittered-lattice Delaunay
-    mesh of about n_nodes nodes on [0,L]^2 with its own 64 Gaussian bumps (seeded), scaled down
+                  cfl=0.25, mesh="poisson") -> Problem:
+    """One mesh of the 16.8M-node workload's family at another size: an unstructured mesh of about
+    n_nodes nodes on [0,L]^2 (the Delaunay triangulation of a Poisson-disk sample, seeded; mesh =
+    "lattice" gives the cell-split jittered lattice instead) with its own 64 Gaussian bumps, scaled down
     where needed so that max z <= zmax: the bumps stay submerged (the reference scheme stops with
     'Min h<0' within a few hundred steps on many coarse meshes with emerging islands, so the long
     batched and served runs keep the islands out).  Kinds:
@@ -213,8 +241,11 @@ def family_member(n_nodes, seed, kind="bumps", L=1000.0, tfinal=None, amp=0.05, 
         bc = walls
     else:
         raise ValueError("unknown kind %r" % kind)
-    return _finish("%s_%d" % (kind, n_side * n_side), M.jittered_delaunay_mesh(n_side, L, jitter=0.25, seed=seed),
-                   bath, u0, params, nsteps=1000, tfinal=tfinal, **bc)
+    if mesh == "poisson":
+        raw = M.poisson_delaunay_mesh(L, n_nodes=n_nodes, seed=seed)
+    else:
+        raw = M.jittered_delaunay_mesh(n_side, L, jitter=0.25, seed=seed)
+    return _finish("%s_%d" % (kind, raw.np), raw, bath, u0, params, nsteps=1000, tfinal=tfinal, **bc)
 
 
 def batch_family_specs(count=256, nmin=5000, nmax=500000, seed=20261202, dam_fraction=0.5):
@@ -245,7 +276,7 @@ def serve_request_specs(count, rate, seed=20261202, median=100000, sigma=1.0, nm
 
 def build_family(specs, workers=1):
     """Problems of a list of specs (family_member keyword sets), in parallel processes if workers > 1."""
-    keys = ("n_nodes", "seed", "kind", "L", "tfinal", "period")
+    keys = ("n_nodes", "seed", "kind", "L", "tfinal", "period", "mesh")
     args = [{k: s[k] for k in keys if k in s} for s in specs]
     if workers <= 1 or len(args) < 2:
         return [family_member(**a) for a in args]
@@ -255,4 +286,5 @@ def build_family(specs, workers=1):
 
 
 def _family_member_kw(kw):
+    M.MESHGEN_THREADS = 1       # one generator thread per worker process
     return family_member(**kw)

@@ -12,6 +12,8 @@ links the reference modules) and the outputs it wrote.  Cases:
   par41_modes.npz     euler (SOURCES/update.f90:134-289) for every viscosity mode, test 5 on 41x41
   par81_run.npz       2000 SSP-RK3 steps of test 5 on the 81x81 grid (the first-order workload)
   dambreak_run.npz    FCT + alpha + Manning (type_test 10) dam break, jittered 101x21, walls
+  dambreak9_run.npz   the same scheme with test 9's fixed htiny and dt (SOURCES/main.f90:66-69), 51x11
+  sgn_run.npz         the SGN model of test 13 (5 components, mSGN_RHS) on MESHES/SGN_CANAL/rectangle.3.FEM
   bumps_run.npz       FCT + alpha, Gaussian bumps, sinusoidal inflow, 65x65 Delaunay mesh
   cav025_run.npz      the reference's own mesh MESHES/DAM_BREAK/cav_0.25.FEM, test 4 init, FCT + alpha
   cdisk_coriolis.npz  the reference's own mesh MESHES/PARABOLOID/cdisk.1_mesh.FEM, test 12 (Coriolis)
@@ -53,8 +55,8 @@ def fresh(d, sub):
     return p
 
 
-def read_state(path, npn):
-    return np.fromfile(path, dtype="<f8").reshape(npn, 3)
+def read_state(path, npn, nc=3):
+    return np.fromfile(path, dtype="<f8").reshape(npn, nc)
 
 
 # ----------------------------------------------------------------------------------------------
@@ -131,14 +133,15 @@ class RefCase:
 
     def __init__(self, raw_mesh=None, fem_path=None, *, test, viscosity="viscous", alpha=False, cfl=0.25,
                  viscous_type="type2", h_sides=(), ux_sides=(), uy_sides=(), mannings=None,
-                 init=None, bc=(0, 0.0, 1.0, 0.0)):
+                 init=None, bc=(0, 0.0, 1.0, 0.0), lambda_sgn=None):
+        self.nc = 5 if test == 13 else 3
         self.d = O.tmpdir()
         if fem_path:
             shutil.copy(fem_path, os.path.join(self.d, "mesh.FEM"))
         else:
             M.write_fem(os.path.join(self.d, "mesh.FEM"), raw_mesh)
         self.kw = dict(mesh_dir=".", mesh_file="mesh.FEM", test=test, cfl=cfl, viscous_type=viscous_type,
-                       h_dir=h_sides, ux_dir=ux_sides, uy_dir=uy_sides, mannings=mannings)
+                       h_dir=h_sides, ux_dir=ux_sides, uy_dir=uy_sides, mannings=mannings, lambda_sgn=lambda_sgn)
         self.write_data(viscosity, alpha)
         self.init = "none"
         if init is not None:
@@ -170,7 +173,8 @@ class RefCase:
             if "Min h<0, STOP" not in str(e):
                 raise
             return np.full(u.shape, np.nan), None, 1
-        return read_state(os.path.join(out, "unext.f8"), u.shape[0]), np.fromfile(os.path.join(out, "dij.f8")), 0
+        return (read_state(os.path.join(out, "unext.f8"), u.shape[0], self.nc), np.fromfile(os.path.join(out, "dij.f8")),
+                0)
 
     def dij(self, u):
         np.ascontiguousarray(u, dtype="<f8").tofile(os.path.join(self.d, "state.f8"))
@@ -184,14 +188,15 @@ class RefCase:
         npn = int(np.fromfile(os.path.join(self.d, "setup", "dims.i4"), dtype="<i4")[0]) if os.path.exists(
             os.path.join(self.d, "setup", "dims.i4")) else None
         u0 = np.fromfile(os.path.join(out, "u0.f8"))
-        npn = u0.size // 3
-        res = dict(final=read_state(os.path.join(out, "final.f8"), npn),
+        nc = self.nc
+        npn = u0.size // nc
+        res = dict(final=read_state(os.path.join(out, "final.f8"), npn, nc),
                    dts=np.fromfile(os.path.join(out, "dt.f8")),
                    time=np.fromfile(os.path.join(out, "time.f8"))[0],
-                   s1=np.stack([read_state(os.path.join(out, "s1_stage%d.f8" % k), npn) for k in (1, 2, 3)]))
+                   s1=np.stack([read_state(os.path.join(out, "s1_stage%d.f8" % k), npn, nc) for k in (1, 2, 3)]))
         snaps = sorted(f for f in os.listdir(out) if f.startswith("snap_"))
         if snaps:
-            res["snaps"] = np.stack([read_state(os.path.join(out, f), npn) for f in snaps])
+            res["snaps"] = np.stack([read_state(os.path.join(out, f), npn, nc) for f in snaps])
             res["snap_steps"] = np.array([int(f[5:11]) for f in snaps])
         return res
 
@@ -266,6 +271,50 @@ def gen_dambreak():
     save("dambreak_run.npz", **setup_arrays(S), final=r["final"], dts=r["dts"], time=r["time"], s1=r["s1"],
          snaps=r["snaps"], snap_steps=r["snap_steps"], euler_in=r["snaps"][0], euler_dt=dt, euler_out=eu,
          euler_dij=dij)
+    case.close()
+
+
+def gen_dambreak9():
+    """Test 9's branch of the time loop (SOURCES/main.f90:66-69: htiny = 1e-3, dt = 2.5e-2, no
+    COMPUTE_DT) with its Manning friction, on a coarse dam break (test 9's own Malpasset mesh is not
+    in the reference tree)."""
+    p = P.dam_break(51, 11, seed=9)
+    raw = M.structured_mesh(51, 11, 0.0, 100.0, 0.0, 20.0, jitter=0.2, seed=9)
+    case, S, r = custom_case(p, nsteps=120, every=40, test=9, mannings=0.02, raw=raw)
+    save("dambreak9_run.npz", **setup_arrays(S), final=r["final"], dts=r["dts"], time=r["time"], s1=r["s1"],
+         snaps=r["snaps"], snap_steps=r["snap_steps"])
+    case.close()
+
+
+def gen_sgn():
+    """The SGN model (test 13, 5 components: h, hu, hv, h*eta, h*w) on the reference's own mesh
+    MESHES/SGN_CANAL/rectangle.3.FEM with its data (lambdaSGN = 9, CFL 0.08, h Dirichlet on sides
+    1 and 3 from the solitary-wave sol_anal): 200 steps of the first-order scheme with the alpha
+    limiter, then euler in the schemes defined for 5 components (viscous with and without alpha,
+    type1, galerkin) at the initial and the final state."""
+    fem = os.path.join(REF, "MESHES", "SGN_CANAL", "rectangle.3.FEM")
+    case = RefCase(fem_path=fem, test=13, viscosity="viscous", alpha=True, cfl=0.08, h_sides=(1, 3),
+                   lambda_sgn=9.0)
+    S = case.setup()
+    r = case.run(200, every=50)
+    states = np.stack([S["u0"], r["final"]])
+    orc = O.Oracle(S["ia"], S["ja"], S["cij"], S["lumped"], S["bath"], gravity=S["gravity"], htiny=S["htiny"],
+                   cfl=S["cfl"], max_water_h=S["max_water_h"], type_test=13, lambda_sgn=9.0)
+    modes = [("viscous", True, "type2"), ("viscous", False, "type2"), ("viscous", True, "type1"),
+             ("galerkin", False, "type2")]
+    outs, dts = [], []
+    for u in states:
+        dt, _ = orc.compute_dt(u)
+        dts.append(dt)
+        for (v, a, t) in modes:
+            case.write_data(v, a, t)
+            un, dij, st = case.euler(u, dt)
+            assert st == 0
+            outs.append(un)
+        case.write_data("viscous", True, "type2")
+    save("sgn_run.npz", **setup_arrays(S), final=r["final"], dts=r["dts"], time=r["time"], s1=r["s1"],
+         snaps=r["snaps"], snap_steps=r["snap_steps"], lambda_sgn=9.0, states=states, euler_dts=np.array(dts),
+         modes=np.array(["%s|%d|%s" % m for m in modes]), unext=np.stack(outs).reshape(2, len(modes), -1, 5))
     case.close()
 
 
@@ -359,7 +408,7 @@ def main():
         sys.exit("oracle/_ref/ref_driver is not built: run `make -C oracle ref` first")
     only = set(sys.argv[1:])
     gens = [("lambda", gen_lambda), ("fct", gen_fct), ("par41", gen_par41_modes), ("par81", gen_par81_run),
-            ("dambreak", gen_dambreak), ("bumps", gen_bumps), ("cav025", gen_cav025),
+            ("dambreak", gen_dambreak), ("dambreak9", gen_dambreak9), ("sgn", gen_sgn), ("bumps", gen_bumps), ("cav025", gen_cav025),
             ("cdisk", gen_cdisk_coriolis), ("friction", gen_par_friction), ("errors", gen_errors)]
     for name, fn in gens:
         if not only or name in only:

@@ -85,6 +85,21 @@ def test_create_rejects_malformed_input_before_device_work():
     assert e.value.status == SWE_EBADOPT
 
 
+@pytest.mark.parametrize("tt", [0, 14, -3, 13])
+def test_create_rejects_type_tests_the_step_does_not_program(tt):
+    """compute_lambda_vacc STOPs with 'BUG in compute_lambda' outside tests 1..13
+    (SOURCES/condlim.f90:897,952-954); test 13 is the 5-component SGN system, which the GPU step does
+    not program: both are refused before any device work (batch admission checks its parameters with
+    the same function)."""
+    from swe_amd.solver import SWE_EBADOPT, Solver, SweError
+    z = golden("cav025_run.npz")
+    ok = dict(gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0)
+    with pytest.raises(SweError) as e:
+        Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], type_test=tt, **ok)
+    assert e.value.status == SWE_EBADOPT
+    assert ("compute_lambda" in str(e.value)) == (tt != 13)
+
+
 def test_assemble_rejects_bad_elements():
     from swe_amd.solver import SweError, assemble
     with pytest.raises(SweError):

@@ -166,6 +166,29 @@ def test_schedules_do_not_change_results():
             assert same_bits(s.get_state(), ref), (blocks, remap)
 
 
+def test_failed_tuning_trial_restores_the_state(monkeypatch):
+    """A tuning trial that fails after queuing its steps (failure injected on the host by
+    SWE_TUNE_FAIL_AT) still gives back the state, time and step count: the next steps equal a run
+    that was never tuned."""
+    from swe_amd import problems
+    from swe_amd.solver import SweError
+    p = problems.bumps(n_side=65, L=1000.0, seed=11)
+    ref, _, _ = run_alone(p, 20)
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t5, _ = s.run(5, time=0.0)
+        u5 = s.get_state()
+        for at in (1, 4):
+            monkeypatch.setenv("SWE_TUNE_FAIL_AT", str(at))
+            with pytest.raises(SweError) as e:
+                s.tune(steps=2)
+            assert "injected" in str(e.value)
+            monkeypatch.delenv("SWE_TUNE_FAIL_AT")
+            assert s.time == t5 and same_bits(s.get_state(), u5), at
+        s.run(15)
+        assert same_bits(s.get_state(), ref)
+
+
 def test_full_batch_reports_no_room():
     from swe_amd import problems
     from swe_amd.batch import Batch

@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-11
 RUNS = {
     "dambreak_run.npz": ("fct", True, 10, 0.02, 1),
+    "dambreak9_run.npz": ("fct", True, 9, 0.02, 1),
     "bumps_run.npz": ("fct", True, 4, 0.0, 2),
     "cav025_run.npz": ("fct", True, 4, 0.0, 0),
     "cdisk_coriolis.npz": ("viscous", True, 12, 0.0, 0),

@@ -21,6 +21,7 @@ def make_oracle(O, z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2)
 # name -> (viscosity, alpha, type_test, mannings, bc_kind)
 RUNS = {
     "dambreak_run.npz": ("fct", True, 10, 0.02, 1),
+    "dambreak9_run.npz": ("fct", True, 9, 0.02, 1),
     "bumps_run.npz": ("fct", True, 4, 0.0, 2),
     "cav025_run.npz": ("fct", True, 4, 0.0, 0),
     "cdisk_coriolis.npz": ("viscous", True, 12, 0.0, 0),
@@ -121,3 +122,62 @@ def test_oracle_setup_matches_reference(oracle, name):
     ia, ja, diag, mass, lumped, cij = oracle.assemble(z["jj"], z["rr"])
     assert same_bits(ia, z["ia"]) and same_bits(ja, z["ja"])
     assert same_bits(lumped, z["lumped"]) and same_bits(cij, z["cij"])
+
+
+def test_hoisted_reference_build_gives_the_same_bits(oracle):
+    """The CPU baseline's build of the reference (entropy_residual's rescale loop reduced to its one
+    assignment, oracle/hoist_rescale.awk) against the unmodified build: the same final state, dt
+    of every step and end time on an FCT + alpha run with inflow (the path through
+    entropy_residual) and on the Manning dam break."""
+    import os
+    import shutil
+    from swe_amd import problems
+    if not (os.path.exists(oracle.REF_DRIVER) and os.path.exists(oracle.REF_DRIVER_HOIST)):
+        pytest.skip("the reference build (make -C oracle ref) is not present")
+    for p, test in ((problems.bumps(n_side=65, L=1000.0, seed=11), 4), (problems.dam_break(61, 13, seed=3), 10)):
+        d = oracle.tmpdir()
+        try:
+            init = oracle.ref_case(d, p, test)
+            a = oracle.ref_time_loop(d, init, 30, out="plain")
+            b = oracle.ref_time_loop(d, init, 30, driver=oracle.REF_DRIVER_HOIST, out="hoist")
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        assert same_bits(a[0], b[0]) and same_bits(a[1], b[1]) and a[2] == b[2], p.name
+        assert a[0][:, 0].min() >= 0.0
+
+
+def test_sgn_model_matches_reference(oracle):
+    """The SGN model of test 13 (5 components, mSGN_RHS of update.f90:488-551, syst_size of
+    main.f90:22-26, bdy of main.f90:176-188 with the solitary wave of condlim.f90:751-827) on the
+    reference's own mesh MESHES/SGN_CANAL/rectangle.3.FEM: euler in every scheme defined for five
+    components at the initial state (x = 1, where pTilde vanishes) and at the state after 200 steps,
+    then the 200-step run (every dt, every snapshot, the end time), all bit for bit."""
+    z = golden("sgn_run.npz")
+    g, htiny, hmax, cfl = z["scalars"]
+    assert z["u0"].shape[1] == 5 and int(z["type_test"]) == 13
+    for k in range(2):
+        for m, mode in enumerate(z["modes"]):
+            visc, alpha, vt = str(mode).split("|")
+            orc = oracle.Oracle(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+                                max_water_h=hmax, viscosity_type=VISC[visc], viscous_type=1 if vt == "type1" else 2,
+                                alpha_limit=bool(int(alpha)), type_test=13, lambda_sgn=float(z["lambda_sgn"]))
+            un, _, err = orc.euler(z["states"][k], float(z["euler_dts"][k]))
+            assert err == 0 and same_bits(un, z["unext"][k, m]), (k, mode)
+    orc = oracle.Oracle(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+                        max_water_h=hmax, viscosity_type=oracle.VISCOUS, alpha_limit=True, type_test=13,
+                        lambda_sgn=float(z["lambda_sgn"]), rr=z["rr"])
+    kw = dict(bc_kind=3, nodes_h=z["h_dir"], nodes_ux=z["ux_dir"], nodes_uy=z["uy_dir"])
+    u, _, _, err = orc.run(z["u0"], 1, **kw)
+    assert err == 0 and same_bits(u, z["s1"][2])
+    u, t, done = z["u0"], 0.0, 0
+    for snap, step in zip(z["snaps"], z["snap_steps"]):
+        u, d, t, err = orc.run(u, int(step) - done, t0=t, **kw)
+        assert err == 0 and same_bits(d, z["dts"][done:int(step)]) and same_bits(u, snap), step
+        done = int(step)
+    assert same_bits(u, z["final"]) and t == float(z["time"])
+    # entropy viscosity and FCT read Entprime(4:5,:), which the reference never sets: refused
+    for visc in (oracle.ENTROPY_VISC, oracle.FCT):
+        orc = oracle.Oracle(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+                            max_water_h=hmax, viscosity_type=visc, alpha_limit=True, type_test=13, lambda_sgn=9.0)
+        assert orc.euler(z["u0"], 1e-3)[2] == 3
+
