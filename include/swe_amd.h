@@ -165,6 +165,19 @@ int swe_fct_positivity(swe_handle *h, const double *unext_scaled, const double *
 int swe_fct_positivity_csr(int np, int nnz, const int *ia, const int *ja, const double *unext_scaled,
                            const double *f, double htiny, double *lij);
 
+/* swe_fct_positivity_csr without any allocation, for callers that cannot allocate device memory
+ * (code under HIP stream capture, tight serving loops, harnesses that forbid hipMalloc).  It has
+ * no counterpart in the reference.  Every pointer is DEVICE memory: ia(np+1), ja(nnz) 1-based,
+ * unext_scaled(np), f(nnz) in, lij(nnz) out; the kernels are queued on `stream` (a hipStream_t,
+ * NULL for the null stream) and the call returns without waiting.  R- of the rows lives in a
+ * fixed device array of the library (4,194,304 rows, 32 MiB per GPU, held until the process
+ * ends): np above that is refused with SWE_EINVAL.  The array is shared by every call on one GPU,
+ * so calls that overlap on one GPU race; each call writes all of R- before reading it.  The
+ * pattern is not checked on the host: entries with a column outside 1..np get lij = NaN and
+ * row ranges are clipped to 1..nnz.  Results are bitwise those of swe_fct_positivity_csr. */
+int swe_fct_positivity_csr_noalloc(int np, int nnz, const int *ia, const int *ja, const double *unext_scaled,
+                                   const double *f, double htiny, double *lij, void *stream);
+
 /* FCT_generic (SOURCES/SAFE/fct.f90:3-43), the two-sided limiter the reference keeps but does not
  * call: with local bounds maxn(np), minn(np), fluxes f(nnz) on a 1-based CSR matrix and the lumped
  * mass dg(np), unext += (SUM_j l_ij f_ij)/dg, l_ij = MIN(R+_i, R-_j) for f_ij >= 0 and

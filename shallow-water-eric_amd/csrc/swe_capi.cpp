@@ -402,7 +402,7 @@ int swe_fct_positivity_csr(int np, int nnz, const int* ia, const int* ja, const 
         if (e == hipSuccess) e = hipMemcpy(df, f, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = fail(SWE_EHIP, "swe_fct_positivity_csr: %s", hipGetErrorString(e));
     }
-    if (!rc) rc = launch_fctpos_csr(np, dia, dja, df, dus, htiny, dl);
+    if (!rc) rc = launch_fctpos_csr(np, nnz, dia, dja, df, dus, htiny, dl);
     if (!rc) {
         e = hipMemcpy(lij, dl, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(SWE_EHIP, "swe_fct_positivity_csr: %s", hipGetErrorString(e));
@@ -411,6 +411,16 @@ int swe_fct_positivity_csr(int np, int nnz, const int* ia, const int* ja, const 
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     return rc;
+}
+
+int swe_fct_positivity_csr_noalloc(int np, int nnz, const int* ia, const int* ja, const double* us, const double* f,
+                                   double htiny, double* lij, void* stream) {
+    clear_error();
+    if (np <= 0 || nnz <= 0 || !ia || !ja || !us || !f || !lij) return fail(SWE_EINVAL, "null or empty argument");
+    if (np > kNoAllocRows)
+        return fail(SWE_EINVAL, "swe_fct_positivity_csr_noalloc: %d rows exceed its fixed scratch of %d rows", np,
+                    kNoAllocRows);
+    return launch_fctpos_csr_noalloc(np, nnz, ia, ja, f, us, htiny, lij, (hipStream_t)stream);
 }
 
 int swe_fct_generic_csr(int np, int nnz, const int* ia, const int* ja, double* unext, const double* maxn,

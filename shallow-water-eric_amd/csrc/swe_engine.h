@@ -260,8 +260,13 @@ struct PokeWords {
     uint32_t w[kPokeWords];
 };
 int launch_poke(Pool& P, void* dst, const void* src, size_t bytes);
-int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, const double* us, double htiny,
-                      double* lij);
+int launch_fctpos_csr(int np, int nnz, const int* ia, const int* ja, const double* f, const double* us,
+                      double htiny, double* lij);
+// The same with R- in the library's fixed device array (rows up to kNoAllocRows), queued on
+// `stream` without allocating or waiting.
+constexpr int kNoAllocRows = 1 << 22;
+int launch_fctpos_csr_noalloc(int np, int nnz, const int* ia, const int* ja, const double* f, const double* us,
+                              double htiny, double* lij, hipStream_t stream);
 int launch_fctgen_csr(int np, const int* ia, const int* ja, const double* f, const double* maxn, const double* minn,
                       const double* dg, double* u);
 

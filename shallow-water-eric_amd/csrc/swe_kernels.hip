@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "swe_engine.h"
@@ -1129,14 +1130,18 @@ __global__ void k_fctpos_l(Mesh M, const double* __restrict__ f, const double* _
     }
 }
 
-// The same on a plain 1-based CSR matrix (swe_fct_positivity_csr).
-__global__ void k_fctpos_csr_r(int np, const int* __restrict__ ia, const double* __restrict__ f,
+// The same on a plain 1-based CSR matrix (swe_fct_positivity_csr and its variant without
+// allocations).  Row ranges are clipped to [0, nnz) and columns outside 1..np give l_ij = NaN, so
+// a malformed pattern that the host could not check (device arrays) cannot make a kernel read
+// out of bounds.
+__global__ void k_fctpos_csr_r(int np, int nnz, const int* __restrict__ ia, const double* __restrict__ f,
                                const double* __restrict__ us, double* __restrict__ Rm, double htiny) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= np) return;
     const double Q = 0.0 - us[i];
     double P = -1.0e-15;
-    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) {
+    const int p0 = max(ia[i] - 1, 0), p1 = min(ia[i + 1] - 1, nnz);
+    for (int p = p0; p < p1; ++p) {
         const double fij = f[p];
         if (fij < 0.0) P = P + fij;
     }
@@ -1145,11 +1150,15 @@ __global__ void k_fctpos_csr_r(int np, const int* __restrict__ ia, const double*
     Rm[i] = R;
 }
 
-__global__ void k_fctpos_csr_l(int np, const int* __restrict__ ia, const int* __restrict__ ja,
+__global__ void k_fctpos_csr_l(int np, int nnz, const int* __restrict__ ia, const int* __restrict__ ja,
                                const double* __restrict__ f, const double* __restrict__ Rm, double* __restrict__ lij) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= np) return;
-    for (int p = ia[i] - 1; p < ia[i + 1] - 1; ++p) lij[p] = (f[p] >= 0.0) ? Rm[ja[p] - 1] : Rm[i];
+    const int p0 = max(ia[i] - 1, 0), p1 = min(ia[i + 1] - 1, nnz);
+    for (int p = p0; p < p1; ++p) {
+        const int j = ja[p] - 1;
+        lij[p] = (j < 0 || j >= np) ? __builtin_nan("") : (f[p] >= 0.0) ? Rm[j] : Rm[i];
+    }
 }
 
 // FCT_generic (SOURCES/SAFE/fct.f90:3-43), the two-sided limiter on a plain 1-based CSR matrix:
@@ -1454,14 +1463,14 @@ void launch_fill_bc(Pool& P, int64_t b0, int64_t n) {
     hipLaunchKernelGGL(k_fill_bc, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, P.bc, b0, n);
 }
 
-int launch_fctpos_csr(int np, const int* ia, const int* ja, const double* f, const double* us, double htiny,
-                      double* lij) {
+int launch_fctpos_csr(int np, int nnz, const int* ia, const int* ja, const double* f, const double* us,
+                      double htiny, double* lij) {
     double* dR = nullptr;
     hipError_t e = hipMalloc((void**)&dR, sizeof(double) * (size_t)np);
     if (e != hipSuccess) return fail(SWE_ENOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
     const unsigned g = blocks_of(np, 256);
-    hipLaunchKernelGGL(k_fctpos_csr_r, dim3(g), dim3(256), 0, 0, np, ia, f, us, dR, htiny);
-    hipLaunchKernelGGL(k_fctpos_csr_l, dim3(g), dim3(256), 0, 0, np, ia, ja, f, (const double*)dR, lij);
+    hipLaunchKernelGGL(k_fctpos_csr_r, dim3(g), dim3(256), 0, 0, np, nnz, ia, f, us, dR, htiny);
+    hipLaunchKernelGGL(k_fctpos_csr_l, dim3(g), dim3(256), 0, 0, np, nnz, ia, ja, f, (const double*)dR, lij);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(dR);
@@ -1482,6 +1491,51 @@ int launch_fctgen_csr(int np, const int* ia, const int* ja, const double* f, con
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(dR);
     if (e != hipSuccess) return fail(SWE_EHIP, "FCT_generic kernels: %s", hipGetErrorString(e));
+    return SWE_OK;
+}
+
+// ---- FCT_positivity without allocations -----------------------------------------------------
+// R- of every row lives in `scratch`, a fixed array in the code object: each GPU the library runs
+// on holds its own copy (kNoAllocRows doubles, 32 MiB) until the process ends, and every call on
+// that GPU uses it, so calls that overlap on one GPU (two streams, two host threads) race.  Each
+// call writes R- of all its rows before any row reads it, so nothing carries over between calls.
+// In host code `scratch` names a host-side stand-in, not device memory: kernels get the device
+// address that hipGetSymbolAddress returns, looked up once per device on the first call there.
+static __device__ __align__(256) double scratch[kNoAllocRows];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(double** out) {
+    int dev = -1;
+    SWE_HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SWE_EINVAL, "device %d is beyond the %d this library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SWE_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SWE_OK;
+}
+}  // namespace
+
+int launch_fctpos_csr_noalloc(int np, int nnz, const int* ia, const int* ja, const double* f, const double* us,
+                              double htiny, double* lij, hipStream_t stream) {
+    double* R = nullptr;
+    int rc = scratch_of_current_device(&R);
+    if (rc) return rc;
+    const unsigned g = blocks_of(np, 256);
+    // two launches on one stream: every R- is written before any row forms its l_ij
+    hipLaunchKernelGGL(k_fctpos_csr_r, dim3(g), dim3(256), 0, stream, np, nnz, ia, f, us, R, htiny);
+    hipLaunchKernelGGL(k_fctpos_csr_l, dim3(g), dim3(256), 0, stream, np, nnz, ia, ja, f, (const double*)R, lij);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SWE_EHIP, "FCT_positivity kernels: %s", hipGetErrorString(e));
     return SWE_OK;
 }
 

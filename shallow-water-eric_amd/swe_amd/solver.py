@@ -76,6 +76,7 @@ SIGNATURES = {
     "swe_euler": (_I, [_P, _P, _P, _D]),
     "swe_fct_positivity": (_I, [_P, _P, _P, _P]),
     "swe_fct_positivity_csr": (_I, [_I, _I, _P, _P, _P, _P, _D, _P]),
+    "swe_fct_positivity_csr_noalloc": (_I, [_I, _I, _P, _P, _P, _P, _D, _P, _P]),
     "swe_set_state": (_I, [_P, _P]),
     "swe_get_state": (_I, [_P, _P]),
     "swe_set_time": (_I, [_P, _D]),
