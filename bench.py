@@ -212,6 +212,16 @@ def port_baseline(name, p, seconds):
                       % (p.name, p.np, n, el)}
 
 
+def hbm_ceiling(gib=2, reps=5):
+    """The device's achievable HBM bandwidth (GB/s): streaming read and streaming copy kernels of the
+    library (swe_stream_bandwidth), the second ceiling the kernels are quoted against."""
+    import ctypes
+    from swe_amd.solver import _check, load_library
+    r, c = ctypes.c_double(), ctypes.c_double()
+    _check(load_library().swe_stream_bandwidth(gib << 30, reps, ctypes.byref(r), ctypes.byref(c)))
+    return {"read_GBs": r.value, "copy_GBs": c.value, "bytes": gib << 30}
+
+
 def copy_calibration(gib=1, reps=3):
     """A plain device-to-device copy of a known size through the HIP runtime, to calibrate the
     FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes `gib` GiB)."""
@@ -688,6 +698,10 @@ def main():
     bpn = kernel_bytes_per_node(dominant, k, alpha)
     avg_s = kms / 1e3 / max(nlaunch, 1)
     achieved = bpn * p.np / avg_s / 1e9
+    ceiling = hbm_ceiling()
+    achievable = max(ceiling["read_GBs"], ceiling["copy_GBs"])
+    for v in per_kernel.values():
+        v["frac_of_achievable"] = v["alg_GBs"] / achievable
     res = {
         "metric": "node-stage updates/s (FP64, SSP-RK3, FCT + alpha)" if fct else "node-stage updates/s (FP64, SSP-RK3)",
         "value": rate,
@@ -706,8 +720,12 @@ def main():
                    "hmin_end": hmin, "schedule": schedule, "setup": "device" if device_setup else "host"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
-                     "launch_ms": avg_s * 1e3, "bytes_per_node": bpn},
+                     "launch_ms": avg_s * 1e3, "bytes_per_node": bpn, "achievable_GBs": achievable,
+                     "frac_of_achievable": achieved / achievable},
+        "hbm_ceiling": ceiling,
         "step_effective_GBs": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9,
+        "step_frac": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9 / HBM_PEAK_GBS,
+        "step_frac_of_achievable": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9 / achievable,
         "kernels": per_kernel,
     }
     s.close()

@@ -6,7 +6,10 @@
  * COMPUTE_DT, the SSP-RK3 combinations and bdy).  Every entry point takes plain pointers and
  * sizes; arrays keep the reference's layout: 1-based CSR (ia(np+1), ja(nnz), columns ascending,
  * diagonal included, as st_csr builds it), state un(3,np) in Fortran order (h, qx, qy per node),
- * c_ij as two arrays cij(1)%aa and cij(2)%aa.
+ * c_ij as two arrays cij(1)%aa and cij(2)%aa.  With type_test 13 (the SGN model) every state
+ * array is un(5,np) (h, qx, qy, h*eta, h*w), as SOURCES/main.f90:22-26 sets syst_size; such
+ * solvers take h Dirichlet lists only (bdy of h*eta and h*w, main.f90:184-187, uses sol_anal(4:5),
+ * which the reference defines only at t <= 1e-10) and no FCT entry points.
  *
  * All arithmetic is IEEE binary64 in the reference's evaluation order; results agree bit for bit
  * with the reference CPU step except where a source term calls pow (Manning friction), where
@@ -56,15 +59,18 @@ typedef struct swe_params {
     double cfl;           /* inputs%CFL */
     double max_water_h;   /* max_water_h (entropy-residual scaling, SOURCES/update.f90:866) */
     double mannings;      /* inputs%mannings (tests 9 and 10) */
-    int type_test;        /* inputs%type_test, 1..12: 9/10 Manning friction, 11 linear friction, 12 Coriolis;
-                             test 9 also fixes htiny = 1e-3 in every call and dt = 2.5e-2 in the time loop
-                             (SOURCES/main.f90:66-69).  Other values give SWE_EBADOPT: outside 1..13
-                             compute_lambda_vacc STOPs (SOURCES/condlim.f90:952-954), and 13 (the
-                             5-component SGN system) is not programmed here. */
+    int type_test;        /* inputs%type_test, 1..13: 9/10 Manning friction, 11 linear friction, 12 Coriolis,
+                             13 the SGN model (5 components, mSGN_RHS); test 9 also fixes htiny = 1e-3 in
+                             every call and dt = 2.5e-2 in the time loop (SOURCES/main.f90:66-69).
+                             Outside 1..13 compute_lambda_vacc STOPs (SOURCES/condlim.f90:952-954): such
+                             values give SWE_EBADOPT, and so does test 13 with 'entropy_visc' or 'fct',
+                             whose entropy_residual reads Entprime(4:5,:) that the reference never sets
+                             (SOURCES/update.f90:853-862). */
     int viscosity_type;   /* swe_viscosity */
     int viscous_type;     /* 1 = 'type1' (smb_1), 2 = 'type2' (smb_2 family) */
     int alpha_limit;      /* inputs%if_alpha_limit */
     int roundoff_fix;     /* if_roundoff_fix (SOURCES/update.f90:15; the reference fixes .TRUE.) */
+    double lambda_sgn;    /* inputs%lambdaSGN (SOURCES/read_data.f90:92-95), used by test 13 only */
 } swe_params;
 
 typedef struct swe_mesh_desc {
@@ -223,6 +229,12 @@ void *swe_stream(swe_handle *h);
  * number of launches since timing was enabled and their summed duration in milliseconds. */
 int swe_enable_kernel_timing(swe_handle *h, const char *kernel);
 int swe_kernel_timing(swe_handle *h, const char *kernel, int *launches, double *ms);
+
+/* The achievable HBM bandwidth of the current device, the ceiling the kernels' roofline fractions
+ * are quoted against besides the published peak: the best of `reps` streaming reads and streaming
+ * copies of `bytes` bytes (16-byte non-temporal loads and stores), in GB/s of bytes moved.  It
+ * allocates and frees two buffers of `bytes`.  No reference counterpart (a measurement utility). */
+int swe_stream_bandwidth(unsigned long long bytes, int reps, double *read_gbs, double *copy_gbs);
 
 /* Launch schedules: per kernel, the workgroup size (128, 256 or 512) and whether workgroups are
  * ordered XCD-aware (1) or in dispatch order (0).  Kernels, in this order: k_compute_dt, k_alpha,

@@ -168,6 +168,7 @@ int swe_create(const swe_mesh_desc* d, const swe_params* prm, swe_handle** out) 
     HostLayout L;
     Mode md;
     if ((rc = mode_of(*prm, md))) return rc;
+    if ((rc = check_sgn_lists(md, d->n_ux_dir, d->n_uy_dir))) return rc;
     if ((rc = build_layout(*d, sort_window_default(), L))) return rc;
     PoolCaps caps;
     caps.rows = L.np_pad;
@@ -209,6 +210,7 @@ int swe_create_from_assembly(const swe_assembly* a, const double* bath, int n_h_
     int rc;
     Mode md;
     if ((rc = mode_of(*prm, md))) return rc;
+    if ((rc = check_sgn_lists(md, n_ux_dir, n_uy_dir))) return rc;
     DeviceLayout DL;
     if ((rc = build_layout_device(a, DL))) return rc;
     if (DL.nnz > INT_MAX) {
@@ -421,6 +423,12 @@ int swe_fct_positivity_csr_noalloc(int np, int nnz, const int* ia, const int* ja
         return fail(SWE_EINVAL, "swe_fct_positivity_csr_noalloc: %d rows exceed its fixed scratch of %d rows", np,
                     kNoAllocRows);
     return launch_fctpos_csr_noalloc(np, nnz, ia, ja, f, us, htiny, lij, (hipStream_t)stream);
+}
+
+int swe_stream_bandwidth(unsigned long long bytes, int reps, double* read_gbs, double* copy_gbs) {
+    clear_error();
+    if (!read_gbs || !copy_gbs) return fail(SWE_EINVAL, "null argument");
+    return stream_bandwidth((size_t)bytes, reps, read_gbs, copy_gbs);
 }
 
 int swe_fct_generic_csr(int np, int nnz, const int* ia, const int* ja, double* unext, const double* maxn,

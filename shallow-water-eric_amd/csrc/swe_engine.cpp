@@ -106,10 +106,12 @@ int make_consts(const swe_params& p, Consts& K) {
     // compute_lambda_vacc knows tests 1 to 13 and STOPs on any other (SOURCES/condlim.f90:897,952-954)
     if (p.type_test < 1 || p.type_test > 13)
         return fail(SWE_EBADOPT, "BUG in compute_lambda: type_test %d is not one of 1..13", p.type_test);
-    // test 13 is the 5-component SGN system (SOURCES/main.f90:24-26, update.f90:619-620); this
-    // library steps the 3-component shallow-water state only
-    if (p.type_test == 13)
-        return fail(SWE_EBADOPT, "type_test 13 (SGN model, 5 components) is not programmed for the GPU step");
+    // test 13 is the 5-component SGN system (SOURCES/main.f90:24-26): with entropy viscosity or FCT
+    // the reference's entropy_residual reads Entprime(4:5,:), which it never sets
+    // (SOURCES/update.f90:853-862), so only the galerkin and first-order schemes are defined for it
+    if (p.type_test == 13 && (p.viscosity_type == SWE_ENTROPY_VISC || p.viscosity_type == SWE_FCT))
+        return fail(SWE_EBADOPT, "type_test 13 (SGN model): entropy_residual reads Entprime(4:5,:), which the "
+                                 "reference never sets; use galerkin or viscous");
     K = Consts{};
     K.g = p.gravity;
     // test 9: the driver sets htiny = 1e-3 and dt = 2.5e-2 at the top of every step instead of
@@ -132,9 +134,11 @@ int make_consts(const swe_params& p, Consts& K) {
     K.ent_cst = 1.0e-10 * p.gravity * (p.max_water_h * p.max_water_h);
     K.type_test = p.type_test;
     K.check_h = 1;
+    K.lambda_sgn = p.lambda_sgn;
     K.src = (p.type_test == 9 || p.type_test == 10) ? SRC_MANNING
             : (p.type_test == 11)                   ? SRC_LINEAR
             : (p.type_test == 12)                   ? SRC_CORIOLIS
+            : (p.type_test == 13)                   ? SRC_SGN
                                                     : SRC_NONE;
     return SWE_OK;
 }
@@ -147,12 +151,13 @@ int mode_of(const swe_params& p, Mode& m) {
     m.viscous_type = p.viscous_type;
     m.alpha = p.alpha_limit ? 1 : 0;
     m.roundoff = p.roundoff_fix ? 1 : 0;
+    m.sgn = (p.type_test == 13) ? 1 : 0;
     return SWE_OK;
 }
 
 bool same_mode(const Mode& a, const Mode& b) {
     return a.viscosity_type == b.viscosity_type && a.viscous_type == b.viscous_type && a.alpha == b.alpha &&
-           a.roundoff == b.roundoff;
+           a.roundoff == b.roundoff && a.sgn == b.sgn;
 }
 
 // ---- host layout ----------------------------------------------------------------------------
@@ -447,7 +452,7 @@ void free_pool(Pool* P) {
     void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->dcol,  P->c,
                     P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dmu,
                     P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
-                    P->flags, P->bc,    P->bcv};
+                    P->flags, P->bc,    P->bcv,  P->X[0], P->X[1], P->X[2], P->X[3], P->stage5, P->xscratch};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (P->act) (void)hipFree(P->act);
@@ -604,6 +609,36 @@ void pool_destroy(Pool* P) {
     free_pool(P);
 }
 
+NodeX* ext_of(const Pool& P, const Node* s) {
+    for (int q = 0; q < 3; ++q)
+        if (s == P.S[q]) return P.X[q];
+    return (s == P.scratch) ? P.X[3] : nullptr;
+}
+
+// The SGN model's h*eta and h*w arrays and its staging buffer, made when the first SGN instance is
+// admitted (pools that never hold one do without them).
+int ensure_sgn_buffers(Pool* P) {
+    if (P->stage5) return SWE_OK;
+    const size_t R = (size_t)P->rows_cap;
+    int rc;
+    for (int q = 0; q < 4; ++q)
+        if ((rc = dalloc(&P->X[q], R))) return rc;
+    if ((rc = dalloc(&P->xscratch, R))) return rc;
+    if ((rc = dalloc(&P->stage5, 5 * R))) return rc;
+    for (int q = 0; q < 4; ++q) SWE_HIPCHK(hipMemset(P->X[q], 0, sizeof(NodeX) * R));
+    return SWE_OK;
+}
+
+// bdy of the SGN model writes h*eta and h*w on the ux and uy lists with sol_anal(4) and (5)
+// (SOURCES/main.f90:184-187), which the reference defines only at t <= 1e-10
+// (SOURCES/condlim.f90:801-826): SGN instances take h lists only.
+int check_sgn_lists(const Mode& md, int nux, int nuy) {
+    if (md.sgn && (nux > 0 || nuy > 0))
+        return fail(SWE_EBADOPT, "type_test 13 (SGN model): bdy of h*eta and h*w on the ux/uy lists uses sol_anal(4:5), "
+                                 "undefined after t = 1e-10 in the reference; give h Dirichlet nodes only");
+    return SWE_OK;
+}
+
 int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const double* un, double t0, double tfinal,
              long long max_steps, int* id, HostLayout* keep) {
     Mode md;
@@ -614,6 +649,8 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
                     P->mode.viscosity_type, P->mode.viscous_type, P->mode.alpha, P->mode.roundoff);
     Consts K;
     if ((rc = make_consts(prm, K))) return rc;
+    if ((rc = check_sgn_lists(md, d.n_ux_dir, d.n_uy_dir))) return rc;
+    if (md.sgn && (rc = ensure_sgn_buffers(P))) return rc;
     HostLayout Lbuf;
     HostLayout& L = keep ? *keep : Lbuf;
     if ((rc = build_layout(d, P->sort_window, L))) return rc;
@@ -668,7 +705,10 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     if (e == hipSuccess) e = up(P->elist + e0, L.elist.data(), sizeof(uint16_t) * L.elist.size());
     if (e == hipSuccess) e = up(P->bc + b0, bce.data(), sizeof(BcEntry) * bce.size());
     if (e == hipSuccess) e = up(P->sl + s0, sls.data(), sizeof(SliceInfo) * sls.size());
-    if (e == hipSuccess && un) e = up(P->stage3 + 3 * r0, un, sizeof(double) * 3 * (size_t)d.np);
+    if (e == hipSuccess && un) {
+        if (md.sgn) e = up(P->stage5 + 5 * r0, un, sizeof(double) * 5 * (size_t)d.np);
+        else e = up(P->stage3 + 3 * r0, un, sizeof(double) * 3 * (size_t)d.np);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(cs);
     if (e != hipSuccess) {
         P->a_slices.release(s0, L.nsl);
@@ -693,6 +733,7 @@ int pool_add(Pool* P, const swe_mesh_desc& d, const swe_params& prm, const doubl
     I.nh = (int)L.lists[0].size();
     I.nux = (int)L.lists[1].size();
     I.nuy = (int)L.lists[2].size();
+    I.nc = md.sgn ? 5 : 3;
     if ((rc = pool_activate(P, k, I, prm, un != nullptr, t0, tfinal, max_steps))) return rc;
     *id = k;
     return SWE_OK;
@@ -718,7 +759,11 @@ int pool_activate(Pool* P, int k, const InstHost& layout, const swe_params& prm,
         P->mode_set = true;
     }
     // the state, then the table entry: from the next queued step on the instance is live
-    if (packed_state) launch_pack(*P, I, P->stage3 + 3 * r0, P->S[0]);
+    I.nc = md.sgn ? 5 : 3;
+    if (packed_state) {
+        if (md.sgn) launch_pack5(*P, I, P->stage5 + 5 * r0, P->S[0], P->X[0]);
+        else launch_pack(*P, I, P->stage3 + 3 * r0, P->S[0]);
+    }
     InstDev D;
     std::memset(&D, 0, sizeof D);
     D.K = K;
@@ -820,6 +865,12 @@ int pool_set_bc(Pool* P, int k, int kind, double amp, double period, double leve
 
 int pool_upload_state(Pool* P, int k, const double* un, Node* dst) {
     const InstHost& I = P->inst[(size_t)k];
+    if (I.nc == 5) {
+        double* st5 = P->stage5 + 5 * I.slice0 * 64;
+        SWE_HIPCHK(hipMemcpyAsync(st5, un, sizeof(double) * 5 * (size_t)I.np, hipMemcpyHostToDevice, P->stream));
+        launch_pack5(*P, I, st5, dst, ext_of(*P, dst));
+        return SWE_OK;
+    }
     double* st = P->stage3 + 3 * I.slice0 * 64;
     SWE_HIPCHK(hipMemcpyAsync(st, un, sizeof(double) * 3 * (size_t)I.np, hipMemcpyHostToDevice, P->stream));
     launch_pack(*P, I, st, dst);
@@ -828,6 +879,13 @@ int pool_upload_state(Pool* P, int k, const double* un, Node* dst) {
 
 int pool_download_state(Pool* P, int k, const Node* src, double* un) {
     const InstHost& I = P->inst[(size_t)k];
+    if (I.nc == 5) {
+        double* st5 = P->stage5 + 5 * I.slice0 * 64;
+        launch_unpack5(*P, I, src, ext_of(*P, src), st5);
+        SWE_HIPCHK(hipMemcpyAsync(un, st5, sizeof(double) * 5 * (size_t)I.np, hipMemcpyDeviceToHost, P->stream));
+        SWE_HIPCHK(hipStreamSynchronize(P->stream));
+        return SWE_OK;
+    }
     double* st = P->stage3 + 3 * I.slice0 * 64;
     launch_unpack(*P, I, src, st);
     SWE_HIPCHK(hipMemcpyAsync(un, st, sizeof(double) * 3 * (size_t)I.np, hipMemcpyDeviceToHost, P->stream));
@@ -962,6 +1020,13 @@ int pool_finished(Pool* P, long long ticket, int* ids, int cap, int* n) {
 int pool_result(Pool* P, int k, double* un) {
     const InstHost& I = P->inst[(size_t)k];
     if (!I.final_known) return fail(SWE_EINVAL, "instance %d has not been reported finished", k);
+    if (I.nc == 5) {
+        double* st5 = P->stage5 + 5 * I.slice0 * 64;
+        launch_unpack5(*P, I, P->S[0], P->X[0], st5, P->copy);
+        SWE_HIPCHK(hipMemcpyAsync(un, st5, sizeof(double) * 5 * (size_t)I.np, hipMemcpyDeviceToHost, P->copy));
+        SWE_HIPCHK(hipStreamSynchronize(P->copy));
+        return SWE_OK;
+    }
     double* st = P->stage3 + 3 * I.slice0 * 64;
     launch_unpack(*P, I, P->S[0], st, P->copy);
     SWE_HIPCHK(hipMemcpyAsync(un, st, sizeof(double) * 3 * (size_t)I.np, hipMemcpyDeviceToHost, P->copy));
@@ -999,12 +1064,16 @@ int pool_tune(Pool* P, int steps, int trials) {
     }
     auto snapshot = [&]() -> hipError_t {
         hipError_t e = hipMemcpyAsync(P->scratch, P->S[0], sizeof(Node) * R, hipMemcpyDeviceToDevice, P->stream);
+        if (e == hipSuccess && P->xscratch)
+            e = hipMemcpyAsync(P->xscratch, P->X[0], sizeof(NodeX) * R, hipMemcpyDeviceToDevice, P->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(tab_save, P->tab, sizeof(InstDev) * n, hipMemcpyDeviceToDevice, P->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(flags_save, P->flags, sizeof(int) * n, hipMemcpyDeviceToDevice, P->stream);
         return e;
     };
     auto restore = [&]() -> hipError_t {
         hipError_t e = hipMemcpyAsync(P->S[0], P->scratch, sizeof(Node) * R, hipMemcpyDeviceToDevice, P->stream);
+        if (e == hipSuccess && P->xscratch)
+            e = hipMemcpyAsync(P->X[0], P->xscratch, sizeof(NodeX) * R, hipMemcpyDeviceToDevice, P->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(P->tab, tab_save, sizeof(InstDev) * n, hipMemcpyDeviceToDevice, P->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(P->flags, flags_save, sizeof(int) * n, hipMemcpyDeviceToDevice, P->stream);
         if (e == hipSuccess) e = hipMemsetAsync(P->vmax, 0, sizeof(unsigned long long) * n, P->stream);

@@ -82,11 +82,13 @@ struct InstHost {
     int np = 0, nnz = 0, np_pad = 0, wmax = 0;
     int64_t slice0 = 0, nslices = 0, slot0 = 0, nslots = 0, e0 = 0, ne = 0, bc0 = 0, nbc = 0;
     int nh = 0, nux = 0, nuy = 0;
+    int nc = 3;                       // components of the state: 3, or 5 for the SGN model
     swe_params prm{};
 };
 
 struct Mode {
     int viscosity_type = SWE_VISCOUS, viscous_type = 2, alpha = 0, roundoff = 1;
+    int sgn = 0;                      // the 5-component SGN model (type_test 13)
 };
 int mode_of(const swe_params& p, Mode& m);
 bool same_mode(const Mode& a, const Mode& b);
@@ -141,6 +143,11 @@ struct Pool {
     // states and work arrays
     Node* S[3] = {nullptr, nullptr, nullptr};   // S[0] holds U^n between steps
     Node* scratch = nullptr;
+    // SGN pools only (allocated at the first SGN admission): h*eta, h*w beside S[0], S[1], S[2] and
+    // scratch, and a 5-doubles-per-row staging buffer for un(5,np)
+    NodeX* X[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* stage5 = nullptr;
+    NodeX* xscratch = nullptr;        // the tuner's copy of X[0]
     double* ulo_h = nullptr;          // FCT low-order solution: depth
     double2* ulo_q = nullptr;         // and discharge
     double *d0 = nullptr, *ework = nullptr;
@@ -193,6 +200,9 @@ int pool_set_params(Pool* P, int id, const swe_params& prm);
 int pool_set_bc(Pool* P, int id, int kind, double amp, double period, double level);
 int pool_set_field(Pool* P, int id, size_t offset, const void* src, size_t bytes);
 int pool_upload_state(Pool* P, int id, const double* un, Node* dst);
+// SGN pools: make the h*eta/h*w arrays at the first SGN admission; SGN instances take h lists only.
+int ensure_sgn_buffers(Pool* P);
+int check_sgn_lists(const Mode& md, int nux, int nuy);
 int pool_download_state(Pool* P, int id, const Node* src, double* un);
 int pool_queue_steps(Pool* P, int nsteps);
 // The parts of one step, launched on the compute stream: the dt pass and its bookkeeping, then the
@@ -248,6 +258,12 @@ void launch_bdy(Pool& P, Node* u, int stage);
 void launch_dij_full(Pool& P, const Node* U, double* dij);
 void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst);
 void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3, hipStream_t stream = nullptr);
+// The same for un(5,np) of the SGN model: (h, qx, qy) into a Node buffer, (h*eta, h*w) into its X.
+void launch_pack5(Pool& P, const InstHost& I, const double* un5, Node* dst, NodeX* xdst);
+void launch_unpack5(Pool& P, const InstHost& I, const Node* src, const NodeX* xsrc, double* un5,
+                    hipStream_t stream = nullptr);
+// X array paired with a state buffer (S[0..2], scratch), or null when the pool has none.
+NodeX* ext_of(const Pool& P, const Node* s);
 void launch_perm_nodes(Pool& P, const InstHost& I, const double* src, double* dst);
 void launch_csr_to_sell(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz);
 void launch_sell_to_csr(Pool& P, const double* src, const int64_t* map, double* dst, int64_t nnz);
@@ -260,6 +276,8 @@ struct PokeWords {
     uint32_t w[kPokeWords];
 };
 int launch_poke(Pool& P, void* dst, const void* src, size_t bytes);
+// Achievable HBM bandwidth: best of `reps` streaming reads and copies of `bytes` (GB/s).
+int stream_bandwidth(size_t bytes, int reps, double* read_gbs, double* copy_gbs);
 int launch_fctpos_csr(int np, int nnz, const int* ia, const int* ja, const double* f, const double* us,
                       double htiny, double* lij);
 // The same with R- in the library's fixed device array (rows up to kNoAllocRows), queued on

@@ -31,6 +31,7 @@
 // sign of a zero.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <type_traits>
@@ -585,6 +586,151 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 }
 
 // =============================================================================================
+// First-order stage of the SGN model (type_test 13: 5 components h, qx, qy, h*eta, h*w; 'viscous'
+// or 'galerkin').  The same sums as k_stage_lo over every component (smb_2_roundoff, smb_2, smb_1 of
+// update.f90:366-423, 554-682 with syst_size = 5), with the model's flux (condlim.f90:52-75: the
+// second velocity is v*0, so only x-fluxes remain; lambda and mu keep the full discharge), then
+// mSGN_RHS (update.f90:488-551) as a second pass over the row, the update of every component and
+// the RK combination.  The meshes of this model are small: one kernel, no launch tuning.
+// =============================================================================================
+// pTilde (update.f90:515-518) at node n: paper_constant = lambdaSGN*g/(3*SQRT(lumped)),
+// x = (h*eta)/h**2; the literal integer powers are products from the left, as flang expands them.
+__device__ __forceinline__ double sgn_ptilde(double h, double he, double m, const Consts& K) {
+    const double pc = K.lambda_sgn * K.g / (3.0 * sqrt(m));
+    const double x = he / (h * h);
+    return pc * (h * h * h) * (2.0 + 4.0 * (x * x * x) - 6.0 * (x * x * x * x));
+}
+
+template <int BS, int SMB, bool ALPHA>
+__global__ void __launch_bounds__(BS)
+k_stage_sgn(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const NodeX* __restrict__ UX,
+            const Node* __restrict__ Uo, const NodeX* __restrict__ UoX, Node* __restrict__ out,
+            NodeX* __restrict__ outX, const double* __restrict__ d0, const double* __restrict__ alpha, int stage,
+            int* __restrict__ flags_out) {
+    const int slice = wave_slice<BS>(M, 0);
+    if (slice < 0) return;
+    const int inst = slice_inst(M, slice);
+    if (inst < 0 || !T[inst].go) return;
+    const int i = slice * 64 + lane_of();
+    const int len = M.len[i];
+    if (len == 0) return;
+    const Consts K = T[inst].K;
+    const double dt = T[inst].dt;
+    int err = 0;
+    const Node I = U[i];
+    const NodeX IX = UX[i];
+    const double sql = sqrt(K.g * fabs(I.h));
+    const double mi = M.m[i];
+    const double ovh_i = ovh_of(I.h, K.htiny);
+    const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
+    const double v2fi = I.qy * ovh_i * 0.0;   // velocity(2,i) of the model's flux
+    const double ui[5] = {I.h, I.qx, I.qy, IX.he, IX.hw};
+    const double ai = ALPHA ? alpha[i] : 1.0;
+    const double thr = (SMB == SMB_ROUNDOFF || SMB == SMB_1) ? K.htiny : 0.0;
+    double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double fix = 0.0;
+    const int64_t pb = row_base(M, i);
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = col_at(M, p, i);
+        const Node J = U[j];
+        const NodeX JX = UX[j];
+        const double uj[5] = {J.h, J.qx, J.qy, JX.he, JX.hw};
+        const double2 c = M.c[p];
+        const double ovh_j = ovh_of(J.h, K.htiny);
+        const double v1j = J.qx * ovh_j, v2j = J.qy * ovh_j;
+        const double v2fj = J.qy * ovh_j * 0.0;
+        const double zmax = fmaxF(I.z, J.z);
+        const double Hij = fmaxF(0.0, I.h + I.z - zmax);
+        const double Hji = fmaxF(0.0, J.h + J.z - zmax);
+        const double ratij = (I.h <= thr) ? 0.0 : Hij / I.h;
+        const double ratji = (J.h <= thr) ? 0.0 : Hji / J.h;
+        double d = 0.0, mu = 0.0;
+        const bool off = (j != i);
+        if (SMB != SMB_GALERKIN && off)
+            first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
+                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   ai, alpha, j, d, mu);
+        if constexpr (SMB == SMB_1) {
+            for (int k = 0; k < 5; ++k) {
+                double xx = 0.0;
+                xx = xx - c.x * (v1j * uj[k] * ratji + v1i * ui[k] * ratij);
+                xx = xx - c.y * (v2fj * uj[k] * ratji + v2fi * ui[k] * ratij);
+                r[k] = r[k] + xx;
+                if (off) r[k] = r[k] + d * (uj[k] * ratji - ui[k] * ratij);
+            }
+            if (off) {
+                const double t = 0.5 * K.g * (Hji * Hji - Hij * Hij);
+                r[1] = r[1] - t * c.x;
+                r[2] = r[2] - t * c.y;
+            }
+        } else {
+            int k0 = 0;
+            if constexpr (SMB == SMB_ROUNDOFF) {
+                double xm = 0.0;
+                xm = xm - c.x * (v1j);
+                xm = xm - c.y * (v2fj);
+                if (!off) fix = fix + xm;
+                else {
+                    fix = fix - mu + (d - mu) * (-ratij);
+                    r[0] = r[0] + (xm + mu) * J.h + fmaxF((d - mu) * Hji, 0.0);
+                }
+                k0 = 1;
+            }
+            for (int k = k0; k < 5; ++k) {
+                double xx = 0.0;
+                xx = xx - c.x * (v1j * uj[k]);
+                xx = xx - c.y * (v2fj * uj[k]);
+                r[k] = r[k] + xx;
+                if (SMB != SMB_GALERKIN && off)
+                    r[k] = r[k] + d * (uj[k] * ratji - ui[k] * ratij) + mu * (uj[k] * (1.0 - ratji) - ui[k] * (1.0 - ratij));
+            }
+            const double topo = K.g * I.h * (J.h + J.z - I.h - I.z);
+            r[1] = r[1] - topo * c.x;
+            r[2] = r[2] - topo * c.y;
+        }
+    }
+    // mSGN_RHS: the momenta lose pTilde(j)*c_ij over the row in CSR order, then the h*eta and h*w sources
+    for (int q = 0; q < len; ++q) {
+        const int64_t p = pb + 64 * (int64_t)q;
+        const int j = col_at(M, p, i);
+        const double2 c = M.c[p];
+        const double pt = sgn_ptilde(U[j].h, UX[j].he, M.m[j], K);
+        r[1] = r[1] - pt * c.x;
+        r[2] = r[2] - pt * c.y;
+    }
+    {
+        const double pc = K.lambda_sgn * K.g / (3.0 * sqrt(mi));
+        const double x = IX.he / (I.h * I.h);
+        const double psi = 12.0 * (x - 1.0);
+        const double t = IX.he / I.h;
+        const double s = 3.0 * pc * (t * t) * psi;
+        r[3] = r[3] + mi * IX.hw;
+        r[4] = r[4] - mi * s;
+    }
+    for (int k = 0; k < 5; ++k) r[k] = r[k] / mi;
+    double o[5];
+    if constexpr (SMB == SMB_1) o[0] = I.h + dt * r[0];
+    else if constexpr (SMB == SMB_ROUNDOFF) o[0] = I.h * (1.0 + dt * fix / mi) + dt * r[0];
+    else o[0] = fmaxF(I.h + dt * r[0], 0.0);
+    for (int k = 1; k < 5; ++k) o[k] = ui[k] + dt * r[k];
+    if (stage != 0) {
+        const Node P = Uo[i];
+        const NodeX PX = UoX[i];
+        const double po[5] = {P.h, P.qx, P.qy, PX.he, PX.hw};
+        for (int k = 0; k < 5; ++k) o[k] = rk_combine(stage, po[k], o[k]);
+    }
+    Node w;
+    w.h = o[0];
+    w.qx = o[1];
+    w.qy = o[2];
+    w.z = I.z;
+    out[i] = w;
+    outX[i] = NodeX{o[3], o[4]};
+    raise_flag(flags_out, inst, err);
+}
+
+// =============================================================================================
 // FCT / entropy-viscosity stage, pass 1: first-order viscosities (kept in dL, muL), the
 // low-order solution (smb_2_roundoff, update) and the entropy residual res_i.
 // =============================================================================================
@@ -1052,6 +1198,35 @@ __global__ void k_pack(const double* __restrict__ un3, const double* __restrict_
     u[r] = o;
 }
 
+// un(5,np) of the SGN model <-> Node records and their (h*eta, h*w) beside them
+__global__ void k_pack5(const double* __restrict__ un5, const double* __restrict__ z, const int* __restrict__ perm,
+                        Node* __restrict__ u, NodeX* __restrict__ x, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np) return;
+    const size_t k = (size_t)perm[r];
+    Node o;
+    o.h = un5[5 * k];
+    o.qx = un5[5 * k + 1];
+    o.qy = un5[5 * k + 2];
+    o.z = z[r];
+    u[r] = o;
+    x[r] = NodeX{un5[5 * k + 3], un5[5 * k + 4]};
+}
+
+__global__ void k_unpack5(const Node* __restrict__ u, const NodeX* __restrict__ x, const int* __restrict__ perm,
+                          double* __restrict__ un5, int np) {
+    const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= np) return;
+    const size_t k = (size_t)perm[r];
+    const Node o = u[r];
+    const NodeX e = x[r];
+    un5[5 * k] = o.h;
+    un5[5 * k + 1] = o.qx;
+    un5[5 * k + 2] = o.qy;
+    un5[5 * k + 3] = e.he;
+    un5[5 * k + 4] = e.hw;
+}
+
 __global__ void k_unpack(const Node* __restrict__ u, const int* __restrict__ perm, double* __restrict__ un3, int np) {
     const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (r >= np) return;
@@ -1334,7 +1509,35 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
                                                   M, T, U, P.alpha, L.remap))
         timed_end(P, KID_ALPHA);
     }
-    if (md.viscosity_type == SWE_GALERKIN) {
+    if (md.sgn) {
+        // the SGN model (first-order and galerkin schemes only)
+        const NodeX* UX = ext_of(P, U);
+        const NodeX* UoX = ext_of(P, Uo);
+        NodeX* outX = ext_of(P, out);
+        if (!UX || !UoX || !outX) return fail(SWE_EINVAL, "SGN stage on a buffer without h*eta, h*w");
+        constexpr int BS = 256;
+        const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
+        const int smb = (md.viscosity_type == SWE_GALERKIN) ? SMB_GALERKIN
+                        : (md.viscous_type == 1)             ? SMB_1
+                        : (md.roundoff ? SMB_ROUNDOFF : SMB_2);
+        timed_begin(P, KID_STAGE_LO);
+#define SWE_SGN_LAUNCH(S, A)                                                                                       \
+    hipLaunchKernelGGL((k_stage_sgn<BS, S, A>), g, dim3(BS), 0, P.stream, M, T, U, UX, Uo, UoX, out, outX,          \
+                       (S == SMB_GALERKIN) ? nullptr : d0, (const double*)P.alpha, stage, P.flags)
+        if (smb == SMB_GALERKIN) SWE_SGN_LAUNCH(SMB_GALERKIN, false);
+        else if (smb == SMB_1) {
+            if (alpha) SWE_SGN_LAUNCH(SMB_1, true);
+            else SWE_SGN_LAUNCH(SMB_1, false);
+        } else if (smb == SMB_ROUNDOFF) {
+            if (alpha) SWE_SGN_LAUNCH(SMB_ROUNDOFF, true);
+            else SWE_SGN_LAUNCH(SMB_ROUNDOFF, false);
+        } else {
+            if (alpha) SWE_SGN_LAUNCH(SMB_2, true);
+            else SWE_SGN_LAUNCH(SMB_2, false);
+        }
+#undef SWE_SGN_LAUNCH
+        timed_end(P, KID_STAGE_LO);
+    } else if (md.viscosity_type == SWE_GALERKIN) {
         launch_stage_lo<SMB_GALERKIN, false>(P, U, Uo, out, nullptr, stage);
     } else if (md.viscosity_type == SWE_VISCOUS) {
         const int smb = (md.viscous_type == 1) ? SMB_1 : (md.roundoff ? SMB_ROUNDOFF : SMB_2);
@@ -1406,6 +1609,18 @@ void launch_dij_full(Pool& P, const Node* U, double* dij) {
     if (M.nact == 0) return;
     hipLaunchKernelGGL((k_dij_full<256>), dim3(blocks_of((int64_t)M.nact * 64, 256)), dim3(256), 0, P.stream, M,
                        (const InstDev*)P.tab, U, dij, P.flags);
+}
+
+void launch_pack5(Pool& P, const InstHost& I, const double* un5, Node* dst, NodeX* xdst) {
+    const int64_t r0 = I.slice0 * 64;
+    hipLaunchKernelGGL(k_pack5, dim3(blocks_of(I.np, 256)), dim3(256), 0, P.stream, un5, (const double*)(P.z + r0),
+                       (const int*)(P.perm + r0), dst + r0, xdst + r0, I.np);
+}
+
+void launch_unpack5(Pool& P, const InstHost& I, const Node* src, const NodeX* xsrc, double* un5, hipStream_t stream) {
+    const int64_t r0 = I.slice0 * 64;
+    hipLaunchKernelGGL(k_unpack5, dim3(blocks_of(I.np, 256)), dim3(256), 0, stream ? stream : P.stream, src + r0,
+                       xsrc + r0, (const int*)(P.perm + r0), un5, I.np);
 }
 
 void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst) {
@@ -1491,6 +1706,78 @@ int launch_fctgen_csr(int np, const int* ia, const int* ja, const double* f, con
     if (e == hipSuccess) e = hipDeviceSynchronize();
     (void)hipFree(dR);
     if (e != hipSuccess) return fail(SWE_EHIP, "FCT_generic kernels: %s", hipGetErrorString(e));
+    return SWE_OK;
+}
+
+// ---- HBM streaming probes -------------------------------------------------------------------------
+// The achievable bandwidth of the device, the second ceiling the step's kernels are quoted against:
+// a streaming read (16-byte non-temporal loads, grid-stride, a per-block sum written out so the loads
+// stay live) and a streaming copy (16-byte non-temporal loads and stores).
+namespace {
+__global__ void __launch_bounds__(256) k_stream_read(const dvec2* __restrict__ a, size_t n2, double* __restrict__ out) {
+    double s = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n2; q += stride) {
+        const dvec2 v = __builtin_nontemporal_load(a + q);
+        s += v.x + v.y;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_stream_copy(const dvec2* __restrict__ a, dvec2* __restrict__ b, size_t n2) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n2; q += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(a + q), b + q);
+}
+}  // namespace
+
+int stream_bandwidth(size_t bytes, int reps, double* read_gbs, double* copy_gbs) {
+    bytes = bytes / 16 * 16;
+    if (bytes == 0 || reps < 1) return fail(SWE_EINVAL, "stream probe: empty size or no repetitions");
+    const size_t n2 = bytes / 16;
+    const int blocks = 8192;
+    dvec2 *a = nullptr, *b = nullptr;
+    double* out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void**)&a, bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&b, bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&out, sizeof(double) * blocks);
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best_r = 1e30f, best_c = 1e30f;
+    for (int r = 0; r <= reps && e == hipSuccess; ++r) {     // the first pass warms up
+        float ms = 0.0f;
+        hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(256), 0, 0, (const dvec2*)a, n2, out);
+        e = hipEventRecord(e0, 0);
+        hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(256), 0, 0, (const dvec2*)a, n2, out);
+        if (e == hipSuccess) e = hipEventRecord(e1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) best_r = std::min(best_r, ms);
+        if (e == hipSuccess) e = hipEventRecord(e0, 0);
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const dvec2*)a, b, n2);
+        if (e == hipSuccess) e = hipEventRecord(e1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) best_c = std::min(best_c, ms);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return fail(SWE_EHIP, "stream probe: %s", hipGetErrorString(e));
+    *read_gbs = (double)bytes / (best_r * 1e-3) / 1e9;
+    *copy_gbs = 2.0 * (double)bytes / (best_c * 1e-3) / 1e9;
     return SWE_OK;
 }
 

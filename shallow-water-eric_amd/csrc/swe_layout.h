@@ -19,6 +19,11 @@ namespace swe {
 struct alignas(32) Node {
     double h, qx, qy, z;
 };
+// The two further components of the SGN model (test 13): h*eta and h*w, in arrays beside the
+// Node buffers of pools that run that model.
+struct alignas(16) NodeX {
+    double he, hw;
+};
 
 // Scalars fixed per parameter set, computed on the host with the same expressions the
 // reference evaluates at run time (host sqrt is correctly rounded).
@@ -33,13 +38,14 @@ struct Consts {
     double fric;        // g*mannings**2
     double ent_cst;     // 1e-10*g*max_water_h**2
     double dt_fixed;    // > 0: the time loop's dt instead of the CFL one (test 9, SOURCES/main.f90:68)
+    double lambda_sgn;  // inputs%lambdaSGN of the SGN model (test 13)
     int type_test;
     int check_h;        // check_hmin (SOURCES/update.f90:810-822) after every stage
     int src;            // source term of type_test (SrcKind)
     int pad_;
 };
 
-enum SrcKind : int { SRC_NONE = 0, SRC_MANNING = 1, SRC_LINEAR = 2, SRC_CORIOLIS = 3 };
+enum SrcKind : int { SRC_NONE = 0, SRC_MANNING = 1, SRC_LINEAR = 2, SRC_CORIOLIS = 3, SRC_SGN = 4 };
 enum : int { FLAG_NEG_H = 1, FLAG_LAMBDA = 2 };
 
 // One 64-row slice (one wave).

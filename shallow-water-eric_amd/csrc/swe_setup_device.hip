@@ -728,6 +728,8 @@ int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, c
     int rc = mode_of(prm, md);
     if (rc) return rc;
     if (P->mode_set && !same_mode(md, P->mode)) return fail(SWE_EBADOPT, "this instance's scheme differs from the pool's");
+    if ((rc = check_sgn_lists(md, nlist[1], nlist[2]))) return rc;
+    if (md.sgn && (rc = ensure_sgn_buffers(P))) return rc;
     const int np = DL.np;
     for (int c = 0; c < 3; ++c) {
         if (nlist[c] < 0 || (nlist[c] > 0 && !lists[c])) return fail(SWE_EINVAL, "Dirichlet list %d is malformed", c);
@@ -763,8 +765,11 @@ int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, c
             e = hipMemcpyAsync(dnodes + q, lists[c], sizeof(int) * (size_t)nlist[c], hipMemcpyHostToDevice, s);
     // ranges freed by a removal are rewritten only after the removal's kernels
     if (e == hipSuccess) e = hipStreamWaitEvent(s, P->released, 0);
-    if (e == hipSuccess && un)
-        e = hipMemcpyAsync(P->stage3 + 3 * r0, un, sizeof(double) * 3 * (size_t)np, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && un) {
+        const int nc = md.sgn ? 5 : 3;
+        double* st = md.sgn ? P->stage5 : P->stage3;
+        e = hipMemcpyAsync(st + nc * r0, un, sizeof(double) * nc * (size_t)np, hipMemcpyHostToDevice, s);
+    }
     if (e == hipSuccess) {
         LayOut o{P->len, P->dpos, P->perm, P->m, P->z, P->col, P->dcol, P->c};
         hipLaunchKernelGGL(k_lay_fill, dim3(grid_of(np_pad, 256)), dim3(256), 0, s, np, np_pad, r0, p0,

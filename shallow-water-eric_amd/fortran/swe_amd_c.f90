@@ -16,6 +16,7 @@ MODULE swe_amd_c
   TYPE, BIND(C) :: swe_params
      REAL(c_double)    :: gravity, htiny, cfl, max_water_h, mannings
      INTEGER(c_int)    :: type_test, viscosity_type, viscous_type, alpha_limit, roundoff_fix
+     REAL(c_double)    :: lambda_sgn
   END TYPE swe_params
 
   TYPE, BIND(C) :: swe_batch_caps

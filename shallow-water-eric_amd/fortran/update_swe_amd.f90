@@ -111,6 +111,8 @@ CONTAINS
     END SELECT
     prm%alpha_limit = MERGE(1, 0, inputs%if_alpha_limit)
     prm%roundoff_fix = 1
+    prm%lambda_sgn = 0.d0
+    IF (inputs%type_test == 13) prm%lambda_sgn = inputs%lambdaSGN
   END FUNCTION current_params
 
   SUBROUTINE ensure_handle
@@ -120,8 +122,8 @@ CONTAINS
     TYPE(swe_mesh_desc) :: desc
     TYPE(swe_params) :: prm
     REAL(KIND=8), DIMENSION(:), ALLOCATABLE, TARGET :: z
-    IF (inputs%syst_size /= 3) THEN
-       WRITE(*,*) ' BUG: the GPU step is programmed for syst_size = 3'
+    IF (inputs%syst_size /= MERGE(5, 3, inputs%type_test == 13)) THEN
+       WRITE(*,*) ' BUG: the GPU step is programmed for syst_size = 3 (5 for the SGN model)'
        STOP
     END IF
     IF (.NOT. inputs%if_lumped) THEN

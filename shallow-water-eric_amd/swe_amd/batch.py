@@ -14,7 +14,7 @@ import math
 import numpy as np
 
 from .solver import (BC_NONE, INSTANCE_DONE, INSTANCE_FAILED, INSTANCE_RUNNING, KERNELS, SWE_EFULL, BatchCaps,
-                     MeshArrays, SweError, _check, _ptr, load_library, make_params)
+                     MeshArrays, SweError, _check, _ptr, load_library, make_params, syst_size)
 
 __all__ = ["Batch", "INSTANCE_RUNNING", "INSTANCE_DONE", "INSTANCE_FAILED"]
 
@@ -69,8 +69,9 @@ class Batch:
             bc=(BC_NONE, 0.0, 1.0, 0.0)):
         """Admit a mesh; returns its id, or None when the batch has no room now."""
         u0 = np.ascontiguousarray(u0, np.float64)
-        if u0.shape != (mesh.np, 3):
-            raise SweError(5, "state must have shape (%d, 3)" % mesh.np)
+        nc = syst_size(params.type_test)
+        if u0.shape != (mesh.np, nc):
+            raise SweError(5, "state must have shape (%d, %d)" % (mesh.np, nc))
         k = C.c_int(-1)
         lib = load_library()
         rc = lib.swe_batch_add(self.h, C.byref(mesh.desc), C.byref(params), _ptr(u0), float(t0), float(tfinal),
@@ -80,7 +81,7 @@ class Batch:
         _check(rc)
         if bc[0] != BC_NONE:
             _check(lib.swe_batch_set_bc(self.h, k.value, int(bc[0]), float(bc[1]), float(bc[2]), float(bc[3])))
-        self._np[k.value] = mesh.np
+        self._np[k.value] = (mesh.np, nc)
         return k.value
 
     def add_elements(self, jj, rr, bath, params, u0, *, h_dir=(), ux_dir=(), uy_dir=(), t0=0.0, tfinal=math.inf,
@@ -94,8 +95,9 @@ class Batch:
         npn = rr.shape[0]
         bath = np.ascontiguousarray(bath, np.float64)
         u0 = np.ascontiguousarray(u0, np.float64)
-        if bath.shape != (npn,) or u0.shape != (npn, 3):
-            raise SweError(5, "bath must have shape (%d,) and the state (%d, 3)" % (npn, npn))
+        nc = syst_size(params.type_test)
+        if bath.shape != (npn,) or u0.shape != (npn, nc):
+            raise SweError(5, "bath must have shape (%d,) and the state (%d, %d)" % (npn, npn, nc))
         lists = [np.ascontiguousarray(x, np.int32) + 1 for x in (h_dir, ux_dir, uy_dir)]
         a = C.c_void_p()
         n = C.c_longlong()
@@ -112,7 +114,7 @@ class Batch:
         _check(rc)
         if bc[0] != BC_NONE:
             _check(lib.swe_batch_set_bc(self.h, k.value, int(bc[0]), float(bc[1]), float(bc[2]), float(bc[3])))
-        self._np[k.value] = npn
+        self._np[k.value] = (npn, nc)
         return k.value
 
     def add_problem(self, problem, *, tfinal=math.inf, max_steps=None, t0=0.0, device_setup=False):
@@ -163,7 +165,7 @@ class Batch:
         return ErrorEvaluator(self, jj, rr, k=k)
 
     def get_state(self, k):
-        out = np.zeros((self._np[k], 3))
+        out = np.zeros(self._np[k])
         _check(load_library().swe_batch_get_state(self.h, int(k), _ptr(out)))
         return out
 
@@ -185,7 +187,7 @@ class Batch:
     def result(self, k):
         """(state, (np, 3) final state, time, steps, error) of an instance reported by
         :meth:`finished`, read while later steps run."""
-        out = np.zeros((self._np[k], 3))
+        out = np.zeros(self._np[k])
         st, err = C.c_int(), C.c_int()
         t = C.c_double()
         n = C.c_longlong()

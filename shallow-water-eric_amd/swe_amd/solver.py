@@ -35,7 +35,7 @@ class Params(C.Structure):
     _fields_ = [("gravity", C.c_double), ("htiny", C.c_double), ("cfl", C.c_double),
                 ("max_water_h", C.c_double), ("mannings", C.c_double),
                 ("type_test", C.c_int), ("viscosity_type", C.c_int), ("viscous_type", C.c_int),
-                ("alpha_limit", C.c_int), ("roundoff_fix", C.c_int)]
+                ("alpha_limit", C.c_int), ("roundoff_fix", C.c_int), ("lambda_sgn", C.c_double)]
 
 
 class BatchCaps(C.Structure):
@@ -92,6 +92,7 @@ SIGNATURES = {
     "swe_set_schedule": (_I, [_P, _P, _P]),
     "swe_use_graph": (_I, [_P, _I]),
     "swe_fct_generic_csr": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "swe_stream_bandwidth": (_I, [C.c_ulonglong, _I, C.POINTER(_D), C.POINTER(_D)]),
     "swe_step_begin": (_I, [_P, C.POINTER(_D)]),
     "swe_step_finish": (_I, [_P, _P]),
     "swe_batch_bytes": (C.c_ulonglong, [C.POINTER(BatchCaps)]),
@@ -216,10 +217,16 @@ def fct_generic_csr(ia, ja, unext, maxn, minn, f, dg):
 
 
 def make_params(*, gravity, htiny, cfl, max_water_h, viscosity="viscous", viscous_type=2, alpha_limit=False,
-                type_test=5, mannings=0.0, roundoff_fix=True) -> Params:
+                type_test=5, mannings=0.0, roundoff_fix=True, lambda_sgn=0.0) -> Params:
     vt = VISCOSITY[viscosity] if isinstance(viscosity, str) else int(viscosity)
     return Params(gravity, htiny, cfl, max_water_h, mannings, type_test, vt, viscous_type,
-                  int(bool(alpha_limit)), int(bool(roundoff_fix)))
+                  int(bool(alpha_limit)), int(bool(roundoff_fix)), float(lambda_sgn))
+
+
+def syst_size(type_test):
+    """Components of the state (inputs%syst_size, SOURCES/main.f90:22-26): 5 for the SGN model of
+    test 13 (h, qx, qy, h*eta, h*w), 3 otherwise."""
+    return 5 if int(type_test) == 13 else 3
 
 
 class MeshArrays:
@@ -253,14 +260,16 @@ class Solver:
 
     def __init__(self, ia, ja, cij, lumped, bath, *, gravity, htiny, cfl, max_water_h,
                  viscosity="viscous", viscous_type=2, alpha_limit=False, type_test=5, mannings=0.0,
-                 roundoff_fix=True, h_dir=(), ux_dir=(), uy_dir=()):
+                 roundoff_fix=True, lambda_sgn=0.0, h_dir=(), ux_dir=(), uy_dir=()):
         lib = load_library()
         self.mesh = MeshArrays(ia, ja, cij, lumped, bath, h_dir, ux_dir, uy_dir)
         self.np = self.mesh.np
         self.nnz = self.mesh.nnz
+        self.nc = syst_size(type_test)
         self.params = make_params(gravity=gravity, htiny=htiny, cfl=cfl, max_water_h=max_water_h,
                                   viscosity=viscosity, viscous_type=viscous_type, alpha_limit=alpha_limit,
-                                  type_test=type_test, mannings=mannings, roundoff_fix=roundoff_fix)
+                                  type_test=type_test, mannings=mannings, roundoff_fix=roundoff_fix,
+                                  lambda_sgn=lambda_sgn)
         h = C.c_void_p()
         _check(lib.swe_create(C.byref(self.mesh.desc), C.byref(self.params), C.byref(h)))
         self.h = h
@@ -281,6 +290,7 @@ class Solver:
         self.h = None
         self.mesh = None
         self.params = make_params(**params)
+        self.nc = syst_size(self.params.type_test)
         a = C.c_void_p()
         n = C.c_longlong()
         _check(lib.swe_assemble_device(rr.shape[0], jj1.shape[0], _ptr(jj1), _ptr(rr), C.byref(a), C.byref(n)))
@@ -332,8 +342,8 @@ class Solver:
     # -- reference routines on host arrays ----------------------------------------------------------
     def _state(self, un):
         un = np.ascontiguousarray(un, np.float64)
-        if un.shape != (self.np, 3):
-            raise SweError(SWE_EINVAL, "state must have shape (%d, 3), got %s" % (self.np, un.shape))
+        if un.shape != (self.np, self.nc):
+            raise SweError(SWE_EINVAL, "state must have shape (%d, %d), got %s" % (self.np, self.nc, un.shape))
         return un
 
     def compute_dij(self, un):
@@ -367,7 +377,7 @@ class Solver:
         _check(load_library().swe_set_state(self.h, _ptr(self._state(un))))
 
     def get_state(self):
-        out = np.zeros((self.np, 3))
+        out = np.zeros((self.np, self.nc))
         _check(load_library().swe_get_state(self.h, _ptr(out)))
         return out
 
@@ -453,7 +463,7 @@ class ErrorEvaluator:
     def __init__(self, owner, jj, rr, k=None):
         jj1 = np.ascontiguousarray(jj, np.int32) + 1
         rr = np.ascontiguousarray(rr, np.float64)
-        npn = owner.np if k is None else owner._np[k]
+        npn = owner.np if k is None else owner._np[k][0]
         if rr.shape != (npn, 2) or jj1.ndim != 2 or jj1.shape[1] != 3:
             raise SweError(SWE_EINVAL, "jj must be (me, 3) and rr (%d, 2)" % npn)
         self.owner = owner                      # the handle must outlive the evaluator

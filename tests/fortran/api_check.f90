@@ -52,6 +52,7 @@ PROGRAM api_check
   prm%viscous_type = ints(3)
   prm%alpha_limit = ints(4)
   prm%roundoff_fix = ints(5)
+  prm%lambda_sgn = 0.d0
 
   CALL check(swe_assemble_device(np, me, jj, rr, asm, nnz), 'swe_assemble_device')
   CALL check(swe_create_from_assembly(asm, bath, nh, hd, nux, uxd, nuy, uyd, prm, h), 'swe_create_from_assembly')

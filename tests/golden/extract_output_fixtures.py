@@ -1,7 +1,8 @@
-"""Copies two output files the reference holds, as data, for the format tests of
-swe_amd/output.py: the first 401 lines of PARABOLOID/h.plt (plot_scalar_field output: header and
-99 elements) and SOURCES/BACKUP/h_1.vtk (vtk_2d output of a 508-node mesh).  Run where the
-reference sources are (they are not needed afterwards):
+"""Copies data files the reference holds, for the tests: the first 401 lines of PARABOLOID/h.plt
+(plot_scalar_field output: header and 99 elements) and SOURCES/BACKUP/h_1.vtk (vtk_2d output of a
+508-node mesh) for the format tests of swe_amd/output.py, and the mesh
+MESHES/SGN_CANAL/rectangle.3.FEM (365 nodes) that the SGN drop-in test runs the reference program
+on.  Run where the reference sources are (they are not needed afterwards):
 
     python tests/golden/extract_output_fixtures.py /root/reference
 """
@@ -18,3 +19,7 @@ with open(os.path.join(ref, "SOURCES", "BACKUP", "h_1.vtk")) as f:
     vtk = f.read()
 with open(os.path.join(here, "ref_h_1_vtk.txt"), "w") as f:
     f.write(vtk)
+with open(os.path.join(ref, "MESHES", "SGN_CANAL", "rectangle.3.FEM"), "rb") as f:
+    fem = f.read()
+with open(os.path.join(here, "sgn_rectangle3.FEM"), "wb") as f:
+    f.write(fem)

@@ -85,17 +85,20 @@ def test_create_rejects_malformed_input_before_device_work():
     assert e.value.status == SWE_EBADOPT
 
 
-@pytest.mark.parametrize("tt", [0, 14, -3, 13])
-def test_create_rejects_type_tests_the_step_does_not_program(tt):
-    """compute_lambda_vacc STOPs with 'BUG in compute_lambda' outside tests 1..13
-    (SOURCES/condlim.f90:897,952-954); test 13 is the 5-component SGN system, which the GPU step does
-    not program: both are refused before any device work (batch admission checks its parameters with
-    the same function)."""
+@pytest.mark.parametrize("tt,viscosity,ux", [(0, "viscous", ()), (14, "fct", ()), (-3, "viscous", ()),
+                                             (13, "fct", ()), (13, "entropy_visc", ()), (13, "viscous", (0, 1))])
+def test_create_rejects_what_the_reference_leaves_undefined(tt, viscosity, ux):
+    """Refused with SWE_EBADOPT before any device work (batch admission checks its parameters with
+    the same functions): type_test outside 1..13, where compute_lambda_vacc STOPs with 'BUG in
+    compute_lambda' (SOURCES/condlim.f90:897,952-954); the SGN model (test 13) with entropy viscosity
+    or FCT, whose entropy_residual reads Entprime(4:5,:) that the reference never sets
+    (update.f90:853-862); and SGN Dirichlet nodes for h*eta, h*w, whose sol_anal the reference
+    defines only at t <= 1e-10 (condlim.f90:801-826)."""
     from swe_amd.solver import SWE_EBADOPT, Solver, SweError
     z = golden("cav025_run.npz")
     ok = dict(gravity=9.81, htiny=1e-4, cfl=0.25, max_water_h=1.0)
     with pytest.raises(SweError) as e:
-        Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], type_test=tt, **ok)
+        Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], type_test=tt, viscosity=viscosity, ux_dir=ux, **ok)
     assert e.value.status == SWE_EBADOPT
     assert ("compute_lambda" in str(e.value)) == (tt != 13)
 
