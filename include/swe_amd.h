@@ -115,7 +115,7 @@ int swe_assembly_destroy(swe_assembly *a);
  * The environment variable SWE_SORT_WINDOW (default 64) sets the window of rows inside which
  * the internal numbering sorts rows by length; SWE_REORDER (default 1) renumbers rows internally by
  * reverse Cuthill-McKee when the caller's numbering scatters neighbours (mean |i-j| above
- * 4*sqrt(np)), 0 never, 2 always.  These are layout choices; results do not depend on them. */
+ * 2*sqrt(np)), 0 never, 2 always.  These are layout choices; results do not depend on them. */
 int swe_create(const swe_mesh_desc *mesh, const swe_params *prm, swe_handle **out);
 /* swe_create over a mesh assembled by swe_assemble_device, without taking the matrices to the
  * host: the solver's layout is built on the GPU from the assembly (rows sorted by length inside
@@ -245,6 +245,13 @@ int swe_stream_bandwidth(unsigned long long bytes, int reps, double *read_gbs, d
 int swe_tune(swe_handle *h, int steps);
 int swe_get_schedule(swe_handle *h, int *blocks, int *remap);
 int swe_set_schedule(swe_handle *h, const int *blocks, const int *remap);
+/* The schedule kind per kernel (same order), a third axis of swe_tune: k_fct_lo in stages 2 and 3
+ * has kind 0 (lambda_ij evaluated per entry inside each lane's row loop) and kind 1 (lambda_ij
+ * evaluated first entry-parallel, one off-diagonal entry per lane, into LDS, then read by the row
+ * loop); every other kernel has kind 0 only.  swe_set_schedule_kinds returns SWE_EINVAL for a kind
+ * a kernel does not have.  Results do not depend on the kind. */
+int swe_get_schedule_kinds(swe_handle *h, int *kinds);
+int swe_set_schedule_kinds(swe_handle *h, const int *kinds);
 /* Queue steps as one captured HIP graph per step (1, the default) or as individual launches (0).
  * Timed kernels (swe_enable_kernel_timing) always launch individually. */
 int swe_use_graph(swe_handle *h, int on);

@@ -138,6 +138,22 @@ int get_schedule(Pool* P, int* blocks, int* remap) {
     return SWE_OK;
 }
 
+int get_schedule_kinds(Pool* P, int* kinds) {
+    for (int k = 0; k < KID_COUNT; ++k) kinds[k] = P->tune.k[k].kind;
+    return SWE_OK;
+}
+
+int set_schedule_kinds(Pool* P, const int* kinds) {
+    Tuning t = P->tune;
+    for (int k = 0; k < KID_COUNT; ++k) {
+        if (kinds[k] < 0 || kinds[k] >= kinds_of(k))
+            return fail(SWE_EINVAL, "%s has no schedule kind %d", kKernelNames[k], kinds[k]);
+        t.k[k].kind = kinds[k];
+    }
+    P->tune = t;
+    return SWE_OK;
+}
+
 int set_schedule(Pool* P, const int* blocks, const int* remap) {
     Tuning t = P->tune;
     for (int k = 0; k < KID_COUNT; ++k) {
@@ -591,6 +607,16 @@ int swe_get_schedule(swe_handle* h, int* blocks, int* remap) {
 int swe_set_schedule(swe_handle* h, const int* blocks, const int* remap) {
     if (!h) return fail(SWE_EINVAL, "null handle");
     return set_schedule(h->P, blocks, remap);
+}
+
+int swe_get_schedule_kinds(swe_handle* h, int* kinds) {
+    if (!h || !kinds) return fail(SWE_EINVAL, "null argument");
+    return get_schedule_kinds(h->P, kinds);
+}
+
+int swe_set_schedule_kinds(swe_handle* h, const int* kinds) {
+    if (!h || !kinds) return fail(SWE_EINVAL, "null argument");
+    return set_schedule_kinds(h->P, kinds);
 }
 
 int swe_use_graph(swe_handle* h, int on) {

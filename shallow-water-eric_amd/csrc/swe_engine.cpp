@@ -305,7 +305,8 @@ std::vector<int> rcm_order(const swe_mesh_desc& d) {
 
 // SWE_REORDER: 0 keeps the caller's order, 2 always renumbers, otherwise (default) the rows are
 // renumbered by reverse Cuthill-McKee when the caller's numbering scatters neighbours (mean offset
-// above 4*sqrt(np)) and the new order is better.
+// above 2*sqrt(np): a lattice numbered row by row has about 0.7*sqrt(np), the loader's first-touch
+// numbering of an unstructured mesh about 3.4*sqrt(np)) and the new order is better.
 }  // namespace
 
 int reorder_mode() {
@@ -333,7 +334,7 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L) {
     std::vector<int> base((size_t)np);
     for (int i = 0; i < np; ++i) base[(size_t)i] = i;
     const int mode = reorder_mode();
-    if (mode == 2 || (mode == 1 && mean_offset(d, nullptr) > 4.0 * std::sqrt((double)np))) {
+    if (mode == 2 || (mode == 1 && mean_offset(d, nullptr) > 2.0 * std::sqrt((double)np))) {
         std::vector<int> rcm = rcm_order(d);
         std::vector<int> pos((size_t)np);
         for (int r = 0; r < np; ++r) pos[(size_t)rcm[(size_t)r]] = r;
@@ -925,7 +926,7 @@ int record_step(Pool* P) {
 
 int64_t tuning_hash(const Tuning& t) {
     int64_t h = 0;
-    for (int k = 0; k < KID_COUNT; ++k) h = h * 1315423911ll + t.k[k].block * 2 + t.k[k].remap;
+    for (int k = 0; k < KID_COUNT; ++k) h = h * 1315423911ll + (t.k[k].block * 2 + t.k[k].remap) * 4 + t.k[k].kind;
     return h;
 }
 
@@ -1110,10 +1111,11 @@ int pool_tune(Pool* P, int steps, int trials) {
         if (rc) break;
         LaunchCfg best = P->tune.k[kid];
         double best_ms = std::numeric_limits<double>::infinity();
-        for (int b : blocks) {
-            for (int remap = 0; remap <= 1 && !rc; ++remap) {
+        for (int cand = 0; cand < 3 * 2 * kinds_of(kid) && !rc; ++cand) {
+            const int b = blocks[cand % 3], remap = (cand / 3) % 2, kind = cand / 6;
+            {
                 if (!block_supported(kid, b)) continue;
-                P->tune.k[kid] = LaunchCfg{b, remap};
+                P->tune.k[kid] = LaunchCfg{b, remap, kind};
                 double ms_min = std::numeric_limits<double>::infinity();
                 for (int t = 0; t < trials && !rc; ++t) {
                     P->timing = true;

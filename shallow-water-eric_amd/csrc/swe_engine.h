@@ -33,7 +33,11 @@ extern const char* const kKernelNames[KID_COUNT];
 struct LaunchCfg {
     int block = 256;    // threads per workgroup (whole waves; 4 slices at 256)
     int remap = 1;      // XCD-aware workgroup order
+    int kind = 0;       // schedule kind (k_fct_lo in stages 2-3: 0 lambda per row inline,
+                        // 1 lambda entry-parallel in LDS); 0 for the other kernels
 };
+// Schedule kinds a kernel has (1 or 2).
+int kinds_of(int kid);
 struct Tuning {
     LaunchCfg k[KID_COUNT];   // defaults: 256 threads, XCD-aware order (the tuner refines per mesh)
 };
@@ -68,7 +72,7 @@ int build_layout(const swe_mesh_desc& d, int sort_window, HostLayout& L);
 // Rows sorted by length inside windows of this many rows: SWE_SORT_WINDOW, else 64.
 int sort_window_default();
 // SWE_REORDER: 0 keeps the caller's numbering as the base order, 1 (default) renumbers for locality
-// when the caller's numbering scatters neighbours (mean |i-j| above 4 sqrt(np)), 2 always.
+// when the caller's numbering scatters neighbours (mean |i-j| above 2 sqrt(np)), 2 always.
 int reorder_mode();
 int make_consts(const swe_params& p, Consts& K);
 

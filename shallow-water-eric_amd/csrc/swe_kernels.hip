@@ -734,12 +734,20 @@ k_stage_sgn(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
 // FCT / entropy-viscosity stage, pass 1: first-order viscosities (kept in dL, muL), the
 // low-order solution (smb_2_roundoff, update) and the entropy residual res_i.
 // =============================================================================================
-template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC, bool D0>
+// Where the row loop takes d_ij = |c_ij| lambda_ij (compute_dij at the stage's state) from:
+enum LamSrc : int {
+    LAM_INLINE = 0,   // evaluated per entry inside the row loop (lane per row)
+    LAM_D0 = 1,       // the dt pass's d_ij at U (stage 1), read from HBM
+    LAM_LDS = 2,      // evaluated entry-parallel into the wave's LDS before the row loop
+};
+
+template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC, int LAM>
 __device__ __forceinline__ void
 fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
             double2* __restrict__ ulo_q,
             double2* __restrict__ dmu, double* __restrict__ res, const double* __restrict__ d0,
-            const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
+            const double* __restrict__ lam, const double* __restrict__ alpha, int roundoff,
+            int* __restrict__ flags_out, int remap) {
     const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
@@ -781,10 +789,11 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
         const bool off = (j != i);
         double d = 0.0, mu = 0.0;
         if (off) {
-            // D0: compute_dij at U comes from the dt pass (stage 1), so this kernel carries no
-            // lambda code and needs fewer registers (5 waves per SIMD instead of 4)
+            // LAM_D0 / LAM_LDS: the row loop carries no lambda code and needs fewer registers
+            // (5 waves per SIMD instead of 4)
             double dlam;
-            if constexpr (D0) dlam = ld_entry1(d0 + p);
+            if constexpr (LAM == LAM_D0) dlam = ld_entry1(d0 + p);
+            else if constexpr (LAM == LAM_LDS) dlam = lam[64 * q + lane_of()];
             else dlam = dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j, dlam, ai, alpha, j, d, mu);
             st_entry(dmu + p, make_double2(d, mu));
@@ -916,8 +925,8 @@ __global__ void SWE_LB_LO(BS)
 k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
          double2* __restrict__ ulo_q, double2* __restrict__ dmu, double* __restrict__ res,
          const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
-    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, false>(M, T, U, ulo_h, ulo_q, dmu, res, nullptr, alpha, roundoff,
-                                                     flags_out, remap);
+    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, LAM_INLINE>(M, T, U, ulo_h, ulo_q, dmu, res, nullptr, nullptr, alpha,
+                                                          roundoff, flags_out, remap);
 }
 
 template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
@@ -926,8 +935,58 @@ k_fct_lo1(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, dou
           double2* __restrict__ ulo_q, double2* __restrict__ dmu, double* __restrict__ res,
           const double* __restrict__ d0, const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out,
           int remap) {
-    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, true>(M, T, U, ulo_h, ulo_q, dmu, res, d0, alpha, roundoff, flags_out,
-                                                    remap);
+    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, LAM_D0>(M, T, U, ulo_h, ulo_q, dmu, res, d0, nullptr, alpha, roundoff,
+                                                      flags_out, remap);
+}
+
+// Stages 2 and 3 with lambda entry-parallel (schedule kind 1): the wave first evaluates
+// d_ij = |c_ij| lambda_ij of its slice's off-diagonal entries, one entry per lane over the slice's
+// compact entry list as the dt pass does, into its LDS; then the row loop runs as in k_fct_lo1 with
+// d_ij read from LDS.  Every d_ij is the same function of the same operands as in the inline form,
+// so the results are the same bits; lambda no longer idles the lanes of diagonal and padding slots.
+template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
+__global__ void SWE_LB_LO1(BS)
+k_fct_lo_lds(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ ulo_h,
+             double2* __restrict__ ulo_q, double2* __restrict__ dmu, double* __restrict__ res,
+             const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
+    extern __shared__ double smem[];
+    double* lw = wave_lds(smem, M.wmax);
+    const int slice = wave_slice<BS>(M, remap);
+    const int lane = lane_of();
+    int inst = -1;
+    bool live = false;
+    if (slice >= 0) {
+        inst = slice_inst(M, slice);
+        live = inst >= 0 && T[inst].go;
+    }
+    if (live) {
+        const int i = slice * 64 + lane;
+        if (M.len[i] > 0) {
+            const double h = U[i].h;
+            lw[lane] = ovh_of(h, T[inst].K.htiny);
+            lw[64 + lane] = sqrt(T[inst].K.g * fabs(h));
+        }
+    }
+    __syncthreads();
+    if (live) {
+        const Consts K = T[inst].K;
+        const SliceInfo S = M.sl[slice];
+        const int64_t e1 = S.ebase + S.ecount;
+        int err = 0;
+        for (int64_t e = S.ebase + lane; e < e1; e += 64) {
+            const int off = M.elist[e];
+            const int r = off & 63;
+            const int64_t slot = S.sbase + off;
+            const Node Ir = U[slice * 64 + r];
+            const Node J = U[col_at(M, slot, slice * 64 + r)];
+            lw[128 + off] = dij_from_c(ld_entry(M.c + slot), Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
+                                       sqrt(K.g * fabs(J.h)), K, err);
+        }
+        raise_flag(flags_out, inst, err);
+    }
+    __syncthreads();
+    fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, LAM_LDS>(M, T, U, ulo_h, ulo_q, dmu, res, nullptr, lw + 128, alpha,
+                                                       roundoff, flags_out, remap);
 }
 
 // Antidiffusive flux f_ij of compute_fct_matrix (update.f90:732-745) for an off-diagonal entry,
@@ -1433,7 +1492,17 @@ void launch_fct_lo(Pool& P, const Node* U, const double* d0) {
     const int ro = P.mode.roundoff;
     auto go = [&](auto src, auto has_d0) {
         constexpr bool S = decltype(src)::value, D = decltype(has_d0)::value;
-        if constexpr (D) {
+        if (!D && L.kind == 1) {
+            // entry-parallel lambda: (2 + wmax)*64 doubles of LDS per wave, within 64 KiB per workgroup
+            int block = L.block;
+            while (block > 128 && (size_t)(2 + M.wmax) * 64 * sizeof(double) * (block / 64) > 65536) block /= 2;
+            SWE_BS_SWITCH(block, {
+                const size_t lds = (size_t)(2 + M.wmax) * 64 * sizeof(double) * (BS / 64);
+                hipLaunchKernelGGL((k_fct_lo_lds<BS, ALPHA, WRITE_LO, S>), dim3(blocks_of((int64_t)M.nact * 64, BS)),
+                                   dim3(BS), lds, P.stream, M, (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu,
+                                   P.res, (const double*)P.alpha, ro, P.flags, L.remap);
+            })
+        } else if constexpr (D) {
             SWE_BS_SWITCH(L.block, hipLaunchKernelGGL((k_fct_lo1<BS, ALPHA, WRITE_LO, S>),
                                                       dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream,
                                                       M, (const InstDev*)P.tab, U, P.ulo_h, P.ulo_q, P.dmu, P.res, d0,
@@ -1460,6 +1529,8 @@ void launch_fct_lo(Pool& P, const Node* U, const double* d0) {
 
 const char* const kKernelNames[KID_COUNT] = {"k_compute_dt", "k_alpha", "k_stage_lo", "k_fct_lo",
                                              "k_fct_rminus", "k_fct_apply", "k_ev_update"};
+
+int kinds_of(int kid) { return (kid == KID_FCT_LO) ? 2 : 1; }
 
 bool block_supported(int kid, int block) {
     if (kid == KID_STAGE_LO || kid == KID_EV) return block == 256;

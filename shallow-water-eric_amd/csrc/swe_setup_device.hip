@@ -588,7 +588,7 @@ hipError_t read_u64(const unsigned long long* d, unsigned long long* h, int n, h
 // The base order of the layout.  *base stays null for the caller's numbering; otherwise it lists the
 // nodes along a Hilbert curve of their coordinates, the device counterpart of the host
 // layout's reverse Cuthill-McKee order, under the same SWE_REORDER policy: 1 when the caller's
-// numbering scatters neighbours (mean |i-j| above 4 sqrt(np)) and the curve brings them closer, 2
+// numbering scatters neighbours (mean |i-j| above 2 sqrt(np)) and the curve brings them closer, 2
 // always.  Only the processing order changes; results do not depend on it.
 hipError_t locality_order(const swe_assembly* a, hipStream_t s, int** base) {
     *base = nullptr;
@@ -613,7 +613,7 @@ hipError_t locality_order(const swe_assembly* a, hipStream_t s, int** base) {
     if (e == hipSuccess) e = read_u64(acc, h, 1, s);
     const double mean0 = (double)h[0] / offdiag;
     bool take = false;
-    if (e == hipSuccess && (mode == 2 || mean0 > 4.0 * std::sqrt((double)np))) {
+    if (e == hipSuccess && (mode == 2 || mean0 > 2.0 * std::sqrt((double)np))) {
         hipLaunchKernelGGL(k_bbox, dim3(bl < 1024u ? bl : 1024u), dim3(256), 0, s, np, (const double*)a->rr, acc + 2);
         e = hipGetLastError();
         if (e == hipSuccess) e = read_u64(acc + 2, h + 2, 4, s);

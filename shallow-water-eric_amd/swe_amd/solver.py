@@ -90,6 +90,8 @@ SIGNATURES = {
     "swe_tune": (_I, [_P, _I]),
     "swe_get_schedule": (_I, [_P, _P, _P]),
     "swe_set_schedule": (_I, [_P, _P, _P]),
+    "swe_get_schedule_kinds": (_I, [_P, _P]),
+    "swe_set_schedule_kinds": (_I, [_P, _P]),
     "swe_use_graph": (_I, [_P, _I]),
     "swe_fct_generic_csr": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "swe_stream_bandwidth": (_I, [C.c_ulonglong, _I, C.POINTER(_D), C.POINTER(_D)]),
@@ -437,16 +439,23 @@ class Solver:
 
     @property
     def schedule(self):
+        """{kernel: (workgroup size, XCD-aware order, schedule kind)}."""
         b = np.zeros(NUM_KERNELS, np.int32)
         r = np.zeros(NUM_KERNELS, np.int32)
+        kd = np.zeros(NUM_KERNELS, np.int32)
         _check(load_library().swe_get_schedule(self.h, _ptr(b), _ptr(r)))
-        return {k: (int(b[i]), int(r[i])) for i, k in enumerate(KERNELS)}
+        _check(load_library().swe_get_schedule_kinds(self.h, _ptr(kd)))
+        return {k: (int(b[i]), int(r[i]), int(kd[i])) for i, k in enumerate(KERNELS)}
 
     @schedule.setter
     def schedule(self, sched):
+        """Takes (workgroup size, XCD-aware order) or (workgroup size, XCD-aware order, kind) per
+        kernel; a missing kind is 0."""
         b = np.array([sched[k][0] for k in KERNELS], np.int32)
         r = np.array([sched[k][1] for k in KERNELS], np.int32)
+        kd = np.array([sched[k][2] if len(sched[k]) > 2 else 0 for k in KERNELS], np.int32)
         _check(load_library().swe_set_schedule(self.h, _ptr(b), _ptr(r)))
+        _check(load_library().swe_set_schedule_kinds(self.h, _ptr(kd)))
 
     def use_graph(self, on=True):
         _check(load_library().swe_use_graph(self.h, int(bool(on))))

@@ -158,12 +158,14 @@ def test_schedules_do_not_change_results():
         assert same_bits(s.get_state(), p.u0)
         s.run(20, time=0.0)
         assert same_bits(s.get_state(), ref)
-        for blocks, remap in ((128, 0), (512, 1), (256, 0)):
-            # the first-order and entropy-viscosity kernels launch with 256 threads only
-            s.schedule = {k: (256 if k in ("k_stage_lo", "k_ev_update") else blocks, remap) for k in sched}
+        for blocks, remap, kind in ((128, 0, 0), (512, 1, 0), (256, 0, 0), (128, 1, 1), (512, 0, 1), (256, 1, 1)):
+            # the first-order and entropy-viscosity kernels launch with 256 threads only; k_fct_lo
+            # has two schedule kinds
+            s.schedule = {k: (256 if k in ("k_stage_lo", "k_ev_update") else blocks, remap,
+                              kind if k == "k_fct_lo" else 0) for k in sched}
             s.set_state(p.u0)
             s.run(20, time=0.0)
-            assert same_bits(s.get_state(), ref), (blocks, remap)
+            assert same_bits(s.get_state(), ref), (blocks, remap, kind)
 
 
 def test_failed_tuning_trial_restores_the_state(monkeypatch):

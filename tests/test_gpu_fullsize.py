@@ -1,5 +1,6 @@
-"""The workloads at their full sizes: the 1,006,209-node dam break (FCT + alpha + Manning, walls) and
-the 16,785,409-node bumps mesh (FCT + alpha, sinusoidal inflow, dry islands).
+"""The workloads at their full sizes: the 1,006,209-node dam break (FCT + alpha + Manning, walls), the
+16,785,409-node bumps problem on the cell-split jittered lattice and the same problem on the
+16.7M-node Poisson-disk Delaunay mesh (FCT + alpha, sinusoidal inflow, dry islands).
 
 The oracle is too slow for whole runs at these sizes, so: the first steps against the oracle on the
 full mesh (bitwise for the bumps case; 1e-11 relative with Manning friction, whose pow differs from
@@ -96,4 +97,36 @@ def test_bumps_16m_200_steps_properties(bumps):
           flush=True)
     assert np.all(np.isfinite(u)) and u[:, 0].min() >= 0.0 and np.all(dts > 0)
     u2, t2, dts2 = _run(p, 200, device_setup=True)
+    assert same_bits(u, u2) and same_bits(dts, dts2) and t == t2
+
+
+@pytest.fixture(scope="module")
+def bumps_unstructured():
+    from swe_amd import problems
+    p = problems.bumps_unstructured()
+    print("bumps unstructured", p.np, p.nnz, flush=True)
+    return p
+
+
+def test_bumps_unstructured_16m_first_step_against_oracle(oracle, bumps_unstructured):
+    """The 16.7M-node Poisson-disk Delaunay mesh (the headline workload): one step on the whole mesh,
+    every bit and the dt against the oracle, for the device set-up (Hilbert locality order)."""
+    p = bumps_unstructured
+    ref, rdts, rt, err = _oracle_run(oracle, p, 1)
+    assert err == 0
+    u, t, dts = _run(p, 1, device_setup=True)
+    print("bumps unstructured 16.7M, 1 step: bitwise fraction %.6f" % np.mean(u == ref), flush=True)
+    assert same_bits(dts, rdts) and t == rt
+    assert same_bits(u, ref)
+
+
+def test_bumps_unstructured_16m_200_steps_properties(bumps_unstructured):
+    """The whole 200-step run: depth >= 0 and finite, and the same bits for the host layout (reverse
+    Cuthill-McKee order) and the device layout (Hilbert order)."""
+    p = bumps_unstructured
+    u, t, dts = _run(p, 200, device_setup=True)
+    print("bumps unstructured 16.7M, 200 steps: t = %.6g, min h %.3g, dry nodes %d"
+          % (t, u[:, 0].min(), int(np.sum(u[:, 0] == 0))), flush=True)
+    assert np.all(np.isfinite(u)) and u[:, 0].min() >= 0.0 and np.all(dts > 0)
+    u2, t2, dts2 = _run(p, 200)
     assert same_bits(u, u2) and same_bits(dts, dts2) and t == t2

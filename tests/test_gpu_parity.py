@@ -21,7 +21,12 @@ RUNS = {
 }
 
 
-def solver_from(z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, bc=0):
+def set_lo_kind(s, kind):
+    """Schedule kind of k_fct_lo: 0 lambda inline per row, 1 lambda entry-parallel in LDS."""
+    s.schedule = {k: (b, r, kind if k == "k_fct_lo" else 0) for k, (b, r, _) in s.schedule.items()}
+
+
+def solver_from(z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, bc=0, kind=0):
     from swe_amd.solver import Solver
     g, htiny, hmax, cfl = z["scalars"]
     s = Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
@@ -32,6 +37,7 @@ def solver_from(z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, bc
         s.set_bc(2, amp, period, level)
     else:
         s.set_bc(bc)
+    set_lo_kind(s, kind)
     return s
 
 
@@ -62,14 +68,15 @@ def test_lambda_bitwise_on_random_states(oracle):
             assert s.compute_dt(u) == orc.compute_dt(u)[0]
 
 
-def test_euler_every_mode_bitwise():
+@pytest.mark.parametrize("kind", [0, 1])
+def test_euler_every_mode_bitwise(kind):
     from swe_amd.solver import SWE_ENEG_H, SweError
     z = golden("par41_modes.npz")
     for k in range(2):
         for m, mode in enumerate(z["modes"]):
             visc, alpha, vt = str(mode).split("|")
-            print("euler state", k, "mode", mode, flush=True)
-            with solver_from(z, visc, bool(int(alpha)), 5, viscous_type=1 if vt == "type1" else 2) as s:
+            print("euler state", k, "mode", mode, "kind", kind, flush=True)
+            with solver_from(z, visc, bool(int(alpha)), 5, viscous_type=1 if vt == "type1" else 2, kind=kind) as s:
                 if z["status"][k, m] == 1:
                     with pytest.raises(SweError) as e:
                         s.euler(z["states"][k], float(z["dts"][k]))
@@ -109,15 +116,16 @@ def test_first_order_workload_2000_steps_bitwise():
         assert t == float(z["time"])
 
 
+@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("name", sorted(RUNS))
-def test_runs_against_reference(name):
+def test_runs_against_reference(name, kind):
     z = golden(name)
     visc, alpha, tt, mannings, bc = RUNS[name]
-    with solver_from(z, visc, alpha, tt, mannings, bc=bc) as s:
+    with solver_from(z, visc, alpha, tt, mannings, bc=bc, kind=kind) as s:
         s.set_state(z["u0"])
         t, done = float(z["time0"]), 0
         for snap, step in zip(z["snaps"], z["snap_steps"]):
-            print(name, "to step", step, flush=True)
+            print(name, "kind", kind, "to step", step, flush=True)
             t, d = s.run(int(step) - done, time=t)
             u = s.get_state()
             if mannings > 0:
