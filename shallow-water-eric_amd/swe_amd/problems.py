@@ -193,7 +193,8 @@ def family_member(n_nodes, seed, kind="bumps", L=1000.0, tfinal=None, amp=0.05, 
     "lattice" gives the cell-split jittered lattice instead) with its own 64 Gaussian bumps, scaled down
     where needed so that max z <= zmax: the bumps stay submerged (the reference scheme stops with
     'Min h<0' within a few hundred steps on many coarse meshes with emerging islands, so the long
-    batched and served runs keep the islands out).  Kinds:
+    served runs keep the islands out).  zmax = None keeps the bumps as drawn: where they sum above the
+    still-water level they emerge as dry islands (h = 0 there from the start).  Kinds:
       'bumps'   still water of level 1 with the sinusoidal west inflow, walls elsewhere;
       'dam'     dam break over the bumps: level 1 on x < L/2, level 0.8 beyond, q = 0, walls;
       'drydam'  the dry-bed dam break of the 1M-node workload on this mesh: flat bed, h = 1 on
@@ -207,7 +208,7 @@ def family_member(n_nodes, seed, kind="bumps", L=1000.0, tfinal=None, amp=0.05, 
         if kind == "drydam":
             return np.zeros(rr.shape[0])
         z = bump_field(rr, L, 64, seed)
-        return z * (zmax / z.max()) if z.max() > zmax else z
+        return z * (zmax / z.max()) if zmax is not None and z.max() > zmax else z
 
     walls = dict(ux_sides=(2, 4), uy_sides=(1, 3), bc=(BC_WALLS, 0.0, 1.0, 0.0))
     if kind == "bumps":
@@ -248,14 +249,19 @@ def family_member(n_nodes, seed, kind="bumps", L=1000.0, tfinal=None, amp=0.05, 
     return _finish("%s_%d" % (kind, raw.np), raw, bath, u0, params, nsteps=1000, tfinal=tfinal, **bc)
 
 
-def batch_family_specs(count=256, nmin=5000, nmax=500000, seed=20261202, dam_fraction=0.5):
+def batch_family_specs(count=256, nmin=5000, nmax=500000, seed=20261202, dam_fraction=0.5, island_fraction=0.25):
     """Sizes (log-uniform in [nmin, nmax]), seeds and kinds of the batched workload's instances:
-    dam breaks ('dam') with probability dam_fraction, still water ('basin') otherwise."""
+    dam breaks ('dam') with probability dam_fraction, still water ('basin') otherwise; with
+    probability island_fraction an instance keeps its bumps as drawn (zmax None: dry islands where
+    they emerge), the others have them scaled under the still-water level."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), count))
     kinds = np.where(rng.uniform(size=count) < dam_fraction, "dam", "basin")
     seeds = rng.integers(1, 2 ** 31 - 1, count)
-    return [dict(n_nodes=int(n), seed=int(s), kind=str(k)) for n, s, k in zip(sizes, seeds, kinds)]
+    # drawn after the others so that sizes, kinds and seeds stay those of the same seed without islands
+    islands = rng.uniform(size=count) < island_fraction
+    return [dict(n_nodes=int(n), seed=int(s), kind=str(k), zmax=None if isl else 0.7)
+            for n, s, k, isl in zip(sizes, seeds, kinds, islands)]
 
 
 def serve_request_specs(count, rate, seed=20261202, median=100000, sigma=1.0, nmax=2000000, nmin=2000,
@@ -276,7 +282,7 @@ def serve_request_specs(count, rate, seed=20261202, median=100000, sigma=1.0, nm
 
 def build_family(specs, workers=1):
     """Problems of a list of specs (family_member keyword sets), in parallel processes if workers > 1."""
-    keys = ("n_nodes", "seed", "kind", "L", "tfinal", "period", "mesh")
+    keys = ("n_nodes", "seed", "kind", "L", "tfinal", "period", "mesh", "zmax")
     args = [{k: s[k] for k in keys if k in s} for s in specs]
     if workers <= 1 or len(args) < 2:
         return [family_member(**a) for a in args]
