@@ -15,7 +15,10 @@ engine: the plain per-instance loop the server is measured against.
 The engine interface (add / step / finished / collect) is what the loop needs from a batch:
 ``step(n)`` queues n steps and returns a ticket, ``finished(ticket)`` lists the instances that
 ticket's table copy shows done or failed, ``collect(k)`` returns (state, time, steps, error) and
-frees the instance.  :class:`BatchEngine` binds it to :class:`swe_amd.batch.Batch`.
+frees the instance.  :class:`BatchEngine` binds it to :class:`swe_amd.batch.Batch`.  A batch runs
+one scheme (viscosity type, viscous type, alpha limiter, roundoff fix, SGN model): a stream that
+mixes schemes is served by :class:`SchemeEngine`, one batch per scheme on its own stream, all
+advanced by the same loop so that their steps overlap on the device.
 """
 from __future__ import annotations
 
@@ -71,6 +74,57 @@ class BatchEngine:
         st, u, t, steps, err = self.batch.result(k)
         self.batch.remove(k)
         return u, t, steps, err
+
+    def close(self):
+        self.batch.close()
+
+
+def scheme_of(params):
+    """What the instances of one batch share (swe_batch_add refuses the others): viscosity type,
+    viscous type, alpha limiter (never with galerkin), roundoff fix, and the SGN model (type_test 13)."""
+    visc = params.get("viscosity", "fct")
+    return (visc, int(params.get("viscous_type", 2)), bool(params.get("alpha_limit", False)) and visc != "galerkin",
+            bool(params.get("roundoff", True)), int(params.get("type_test", 4)) == 13)
+
+
+class SchemeEngine:
+    """The engine interface over one engine (batch) per scheme.  ``make_engine(scheme)`` creates the
+    engine of a scheme the first time a request of that scheme arrives (a :class:`BatchEngine` over
+    a batch whose capacity is that scheme's share of the memory budget); instance ids are
+    (scheme, id) pairs and a ticket holds one ticket per engine that has instances."""
+
+    def __init__(self, make_engine):
+        self.make_engine = make_engine
+        self.engines = {}
+        self.live = collections.Counter()
+
+    def add(self, req: Request):
+        sch = scheme_of(req.problem.params)
+        eng = self.engines.get(sch)
+        if eng is None:
+            eng = self.engines[sch] = self.make_engine(sch)
+        k = eng.add(req)
+        if k is None:
+            return None
+        self.live[sch] += 1
+        return (sch, k)
+
+    def step(self, n):
+        return {sch: eng.step(n) for sch, eng in self.engines.items() if self.live[sch]}
+
+    def finished(self, ticket):
+        return [(sch, k) for sch, t in ticket.items() for k in self.engines[sch].finished(t)]
+
+    def collect(self, key):
+        sch, k = key
+        self.live[sch] -= 1
+        return self.engines[sch].collect(k)
+
+    def close(self):
+        for eng in self.engines.values():
+            if hasattr(eng, "close"):
+                eng.close()
+        self.engines.clear()
 
 
 class Server:

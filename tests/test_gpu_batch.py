@@ -250,6 +250,40 @@ def test_served_results_equal_alone_and_in_the_mix():
         assert t == r.time and same_bits(s.get_state(), r.state)
 
 
+def test_served_stream_of_several_schemes():
+    """A stream mixing FCT + alpha, first-order viscous and first-order type1 requests (a batch
+    runs one scheme): served side by side, one batch per scheme, every result equals the request
+    run alone on a fresh solver (state, time, steps)."""
+    from swe_amd import problems
+    from swe_amd.batch import Batch
+    from swe_amd.serve import BatchEngine, Request, SchemeEngine, Server, scheme_of
+    specs = problems.serve_request_specs(9, rate=300.0, median=5000, sigma=0.5, nmax=15000, nmin=3000,
+                                         dry_fraction=0.0, period=2.0)
+    ps = problems.build_family(specs)
+    variants = [dict(viscosity="fct"), dict(viscosity="viscous", viscous_type=2),
+                dict(viscosity="viscous", viscous_type=1, alpha_limit=False)]
+    for q, p in enumerate(ps):
+        p.params.update(variants[q % 3])
+    assert len({scheme_of(p.params) for p in ps}) == 3
+    reqs = [Request(rid=q, problem=p, tfinal=s["tfinal"], arrival=s["arrival"])
+            for q, (p, s) in enumerate(zip(ps, specs))]
+    rows = sum((p.np + 63) // 64 * 64 for p in ps)
+    ent = int(sum(p.nnz for p in ps) * 1.4) + 4096
+    eng = SchemeEngine(lambda sch: BatchEngine(Batch(rows, ent, 4096, 16)))
+    try:
+        done = Server(eng, chunk=8).run(reqs)
+        assert len(eng.engines) == 3
+    finally:
+        eng.close()
+    for r in done:
+        print("request", r.rid, scheme_of(r.problem.params), r.steps, r.error, flush=True)
+        assert r.error == 0 and r.time >= r.tfinal and r.steps > 0
+        with r.problem.solver() as s:
+            s.set_state(r.problem.u0)
+            t, _ = s.run(r.steps, time=0.0)
+            assert t == r.time and same_bits(s.get_state(), r.state), r.rid
+
+
 def test_tickets_report_finished_instances_once():
     """Pipelined reads: a ticket's table copy reports an instance once it is done, the result read
     while later steps run equals the state a full synchronisation reads, and an instance admitted

@@ -5,7 +5,7 @@ import math
 
 import pytest
 
-from swe_amd.serve import Request, Server, run_plain, summarize
+from swe_amd.serve import Request, SchemeEngine, Server, run_plain, scheme_of, summarize
 
 
 class FakeClock:
@@ -21,9 +21,10 @@ class FakeClock:
 
 
 class FakeProblem:
-    def __init__(self, n, dt):
+    def __init__(self, n, dt, params=None):
         self.np = n
         self.dt = dt
+        self.params = params or {}
 
 
 class FakeEngine:
@@ -131,3 +132,31 @@ def test_idle_server_waits_for_arrivals():
     reqs = [Request(rid=q, problem=FakeProblem(100, 0.5), tfinal=1.0, arrival=10.0 * q) for q in range(3)]
     out = Server(FakeEngine(1000, clock), clock=clock, sleep=clock.sleep).run(reqs)
     assert [math.floor(r.admitted / 10.0) for r in out] == [0, 1, 2]
+
+
+def test_several_schemes_are_served_side_by_side():
+    """One engine per scheme behind SchemeEngine: each request lands in its scheme's engine, within
+    that engine's capacity, and takes the steps its final time requires, as when served alone."""
+    clock = FakeClock()
+    schemes = [dict(viscosity="fct", alpha_limit=True), dict(viscosity="viscous", alpha_limit=True),
+               dict(viscosity="viscous", viscous_type=1), dict(viscosity="galerkin", alpha_limit=True)]
+    reqs = stream(n=48, seed=5)
+    for q, r in enumerate(reqs):
+        r.problem.params = schemes[q % 4]
+    made = {}
+
+    def make(sch):
+        made[sch] = FakeEngine(500, clock)
+        return made[sch]
+    eng = SchemeEngine(make)
+    out = Server(eng, chunk=4, clock=clock, sleep=clock.sleep).run(reqs)
+    assert len(made) == 4 and scheme_of(schemes[3]) == ("galerkin", 2, False, True, False)
+    for sch, e in made.items():
+        assert e.max_rows <= 500 and not e.inst
+        assert e.order == sorted(e.order) and {scheme_of(reqs[q].problem.params) for q in e.order} == {sch}
+    for r in out:
+        t, n = 0.0, 0
+        while t < r.tfinal:
+            t = t + r.problem.dt
+            n += 1
+        assert r.steps == n and r.time == t and r.finished >= r.admitted >= r.arrival
