@@ -602,6 +602,8 @@ def main():
     ap.add_argument("--size", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules")
+    ap.add_argument("--step-form", type=int, default=None,
+                    help="force the step form after tuning (0 kernel launches / graph, 1 persistent)")
     ap.add_argument("--lo-kind", type=int, default=None,
                     help="force the schedule kind of k_fct_lo after tuning (0 lambda inline, 1 lambda in LDS)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -669,6 +671,8 @@ def main():
         s.tune(3)           # launch schedules (workgroup size, XCD order, kind) timed on this mesh; state restored
     if args.lo_kind is not None:
         s.schedule = {n: (b, r, args.lo_kind if n == "k_fct_lo" else kd) for n, (b, r, kd) in s.schedule.items()}
+    if args.step_form is not None:
+        s.step_form = args.step_form
     s.run(args.warmup)
     s.sync()
     s.enable_kernel_timing(dominant)

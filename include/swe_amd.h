@@ -255,6 +255,18 @@ int swe_set_schedule_kinds(swe_handle *h, const int *kinds);
 /* Queue steps as one captured HIP graph per step (1, the default) or as individual launches (0).
  * Timed kernels (swe_enable_kernel_timing) always launch individually. */
 int swe_use_graph(swe_handle *h, int on);
+/* How swe_run (and swe_batch_step*) queue steps.  SWE_STEP_LAUNCHES (the default): the passes of a
+ * step as a sequence of kernel launches, replayed as one captured graph per step when graphs are
+ * on.  SWE_STEP_PERSISTENT: the fct scheme's steps in cooperative launches of one kernel that runs
+ * every pass, with grid-wide barriers where the launch sequence has kernel boundaries (for meshes too
+ * small to fill the GPU, where a launch costs more than its pass); other schemes keep the launch
+ * sequence.  Results are the same bits either way.  swe_tune and swe_batch_tune time both forms
+ * on the resident meshes and keep the faster.  A barrier wait is bounded: should one give up,
+ * the instances that were stepping fail with SWE_EHIP.  SWE_EINVAL for an unknown form or a device
+ * without cooperative launches.  No reference counterpart (a launch choice). */
+typedef enum swe_step_form { SWE_STEP_LAUNCHES = 0, SWE_STEP_PERSISTENT = 1 } swe_step_form;
+int swe_set_step_form(swe_handle *h, int form);
+int swe_get_step_form(swe_handle *h, int *form);
 
 /* -------------------------------------------------------------------------------------------
  * Batches: many meshes of different sizes resident in one device arena and advanced together,
@@ -335,6 +347,8 @@ int swe_batch_free(swe_batch *b, long long *rows, long long *entries, int *insta
 
 int swe_batch_tune(swe_batch *b, int steps);
 int swe_batch_use_graph(swe_batch *b, int on);
+int swe_batch_set_step_form(swe_batch *b, int form);   /* as swe_set_step_form */
+int swe_batch_get_step_form(swe_batch *b, int *form);
 void *swe_batch_stream(swe_batch *b);
 int swe_batch_enable_kernel_timing(swe_batch *b, const char *kernel);
 int swe_batch_kernel_timing(swe_batch *b, const char *kernel, int *launches, double *ms);

@@ -57,6 +57,7 @@ int check_flags(Pool* P, int k) {
         if (!rc) rc = launch_poke(*P, &P->tab[k].state, &st, sizeof st);
         if (rc) return rc;
         SWE_HIPCHK(hipStreamSynchronize(P->stream));
+        if (f & FLAG_STEP) return fail(SWE_EHIP, "persistent step: a grid barrier gave up waiting");
         if (f & FLAG_LAMBDA) return fail(SWE_ELAMBDA, "BUG in compute_lambda_vacc (Delta < 0)");
         if (f & FLAG_NEG_H) return fail(SWE_ENEG_H, "Min h<0, STOP");
     }
@@ -619,6 +620,36 @@ int swe_set_schedule_kinds(swe_handle* h, const int* kinds) {
     return set_schedule_kinds(h->P, kinds);
 }
 
+static int set_step_form(Pool* P, int form) {
+    if (form != SWE_STEP_LAUNCHES && form != SWE_STEP_PERSISTENT) return fail(SWE_EINVAL, "no step form %d", form);
+    if (form == SWE_STEP_PERSISTENT && !P->coop_ok)
+        return fail(SWE_EINVAL, "the device does not take cooperative launches");
+    P->step_form = form;
+    return SWE_OK;
+}
+
+int swe_set_step_form(swe_handle* h, int form) {
+    if (!h) return fail(SWE_EINVAL, "null handle");
+    return set_step_form(h->P, form);
+}
+
+int swe_get_step_form(swe_handle* h, int* form) {
+    if (!h || !form) return fail(SWE_EINVAL, "null argument");
+    *form = h->P->step_form;
+    return SWE_OK;
+}
+
+int swe_batch_set_step_form(swe_batch* b, int form) {
+    if (!b) return fail(SWE_EINVAL, "null batch");
+    return set_step_form(b->P, form);
+}
+
+int swe_batch_get_step_form(swe_batch* b, int* form) {
+    if (!b || !form) return fail(SWE_EINVAL, "null argument");
+    *form = b->P->step_form;
+    return SWE_OK;
+}
+
 int swe_use_graph(swe_handle* h, int on) {
     if (!h) return fail(SWE_EINVAL, "null handle");
     h->P->use_graph = on != 0;
@@ -741,7 +772,7 @@ int swe_batch_result(swe_batch* b, int id, double* un, int* state, double* time,
     if (state) *state = st;
     if (time) *time = D.time;
     if (steps) *steps = D.steps;
-    if (error) *error = (f & FLAG_LAMBDA) ? SWE_ELAMBDA : (f & FLAG_NEG_H) ? SWE_ENEG_H : SWE_OK;
+    if (error) *error = (f & FLAG_STEP) ? SWE_EHIP : (f & FLAG_LAMBDA) ? SWE_ELAMBDA : (f & FLAG_NEG_H) ? SWE_ENEG_H : SWE_OK;
     return SWE_OK;
 }
 
@@ -764,7 +795,7 @@ int swe_batch_status(swe_batch* b, int id, int* state, double* time, long long* 
     if (state) *state = st;
     if (time) *time = D.time;
     if (steps) *steps = D.steps;
-    if (error) *error = (f & FLAG_LAMBDA) ? SWE_ELAMBDA : (f & FLAG_NEG_H) ? SWE_ENEG_H : SWE_OK;
+    if (error) *error = (f & FLAG_STEP) ? SWE_EHIP : (f & FLAG_LAMBDA) ? SWE_ELAMBDA : (f & FLAG_NEG_H) ? SWE_ENEG_H : SWE_OK;
     return SWE_OK;
 }
 

@@ -453,7 +453,8 @@ void free_pool(Pool* P) {
     void* bufs[] = {P->sl,  P->len,     P->dpos,  P->perm, P->m,     P->z,     P->col,   P->dcol,  P->c,
                     P->elist,   P->S[0],  P->S[1], P->S[2],  P->scratch, P->ulo_h, P->ulo_q, P->d0,    P->dmu,
                     P->ework,   P->alpha, P->res,  P->Rm,    P->nwork, P->stage3, P->tab, P->vmax,
-                    P->flags, P->bc,    P->bcv,  P->X[0], P->X[1], P->X[2], P->X[3], P->stage5, P->xscratch};
+                    P->flags, P->bc,    P->bcv,  P->X[0], P->X[1], P->X[2], P->X[3], P->stage5, P->xscratch,
+                    P->bar};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (P->act) (void)hipFree(P->act);
@@ -560,6 +561,7 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     P->bcv_stride = std::max<int64_t>(caps.bc, 1);
     ALLOC(P->bcv, 3 * (size_t)P->bcv_stride);
     ALLOC(P->act, (size_t)slices);
+    ALLOC(P->bar, 2);
 #undef ALLOC
     P->listed.assign((size_t)caps.instances, 0);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
@@ -592,6 +594,11 @@ int pool_create(const PoolCaps& caps, Pool** out) {
     if (e == hipSuccess) e = hipMemset(P->len, 0, sizeof(uint8_t) * R);
     if (e == hipSuccess) e = hipMemset(P->vmax, 0, sizeof(unsigned long long) * (size_t)caps.instances);
     if (e == hipSuccess) e = hipMemset(P->flags, 0, sizeof(int) * (size_t)caps.instances);
+    if (e == hipSuccess) e = hipMemset(P->bar, 0, 2 * sizeof(unsigned));
+    int coop = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, P->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&P->num_cu, hipDeviceAttributeMultiprocessorCount, P->device);
+    P->coop_ok = coop != 0;
     if (e == hipSuccess) e = hipMemset(P->bc, 0xff, sizeof(BcEntry) * (size_t)std::max<int64_t>(caps.bc, 1));
     if (e == hipSuccess) e = hipEventRecord(P->released, P->stream);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -965,7 +972,9 @@ int ensure_graph(Pool* P) {
 int pool_queue_steps(Pool* P, int nsteps) {
     if (nsteps <= 0 || P->nact == 0) return SWE_OK;
     int rc;
-    if (P->use_graph && !P->timing) {
+    if (P->step_form == 1 && !P->timing && persistent_applies(*P)) {
+        if ((rc = launch_step_persistent(*P, nsteps))) return rc;
+    } else if (P->use_graph && !P->timing) {
         if ((rc = ensure_graph(P))) return rc;
         for (int it = 0; it < nsteps; ++it) SWE_HIPCHK(hipGraphLaunch(P->gexec, P->stream));
     } else {
@@ -1152,6 +1161,50 @@ int pool_tune(Pool* P, int steps, int trials) {
     P->timing = timing0;
     P->timed_name = name0;
     P->tk.used = 0;
+    // the step form, on whole steps: the launch sequence (captured graph when enabled) or the
+    // persistent step; the first run of each form is untimed (graph capture, code load)
+    if (!rc && persistent_applies(*P)) {
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        e = hipEventCreate(&ev[0]);
+        if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+        if (e != hipSuccess) rc = fail(SWE_EHIP, "tuning events: %s", hipGetErrorString(e));
+        const int form0 = P->step_form;
+        int best_form = form0;
+        double best_ms = std::numeric_limits<double>::infinity();
+        for (int form = 0; form <= 1 && !rc; ++form) {
+            P->step_form = form;
+            double ms_min = std::numeric_limits<double>::infinity();
+            for (int t = 0; t <= trials && !rc; ++t) {
+                dirty = true;
+                if ((e = hipEventRecord(ev[0], P->stream)) != hipSuccess) {
+                    rc = fail(SWE_EHIP, "tuning run: %s", hipGetErrorString(e));
+                    break;
+                }
+                if ((rc = pool_queue_steps(P, steps))) break;
+                if (++trial_no == fail_at) {
+                    rc = fail(SWE_EHIP, "tuning run: failure injected at trial %d (SWE_TUNE_FAIL_AT)", trial_no);
+                    break;
+                }
+                if ((e = hipEventRecord(ev[1], P->stream)) == hipSuccess) e = hipStreamSynchronize(P->stream);
+                if (e != hipSuccess) {
+                    rc = fail(SWE_EHIP, "tuning run: %s", hipGetErrorString(e));
+                    break;
+                }
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (t > 0) ms_min = std::min(ms_min, (double)ms);
+                if ((e = restore()) != hipSuccess) rc = fail(SWE_EHIP, "tuning restore: %s", hipGetErrorString(e));
+                else dirty = false;
+            }
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best_form = form;
+            }
+        }
+        P->step_form = rc ? form0 : best_form;
+        for (hipEvent_t x : ev)
+            if (x) (void)hipEventDestroy(x);
+    }
     if (dirty) {
         // a trial failed part-way: wait for what it queued, then restore the snapshot; a failed
         // restore is reported on its own (the tuning error is kept in the message)

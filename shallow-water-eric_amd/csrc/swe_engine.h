@@ -171,6 +171,10 @@ struct Pool {
     long long next_ticket = 0;
     // schedules, graph, timing
     Tuning tune;
+    int step_form = 0;                // swe_step_form: 0 launch sequence (graph), 1 persistent (fct)
+    bool coop_ok = false;             // the device takes cooperative launches
+    int num_cu = 1;
+    unsigned* bar = nullptr;          // the persistent step's barrier word and abort word
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
     int64_t graph_key[6] = {-1, -1, -1, -1, -1, -1};
@@ -259,6 +263,10 @@ void launch_compute_dt(Pool& P, const Node* U, double* d0, int all);
 void launch_finalize_dt(Pool& P, int advance);
 int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage);
 void launch_bdy(Pool& P, Node* u, int stage);
+// The persistent step form: whether it covers the pool's scheme (fct, three components) on this
+// device, and nsteps steps of every listed instance in cooperative launches.
+bool persistent_applies(const Pool& P);
+int launch_step_persistent(Pool& P, int nsteps);
 void launch_dij_full(Pool& P, const Node* U, double* dij);
 void launch_pack(Pool& P, const InstHost& I, const double* un3, Node* dst);
 void launch_unpack(Pool& P, const InstHost& I, const Node* src, double* un3, hipStream_t stream = nullptr);

@@ -278,14 +278,13 @@ __device__ __forceinline__ double alpha_of(const Mesh& M, const Node* __restrict
 // phase 2 sums each row in entry order.  all = 1 (host-array calls) covers every admitted
 // instance; otherwise only the instances that take the step.
 // =============================================================================================
+// The dt pass on one slice per wave (slice -1: a wave past the end of the list, which still takes
+// part in the workgroup's barriers); every wave of the workgroup calls it together.
 template <int BS>
-__global__ void SWE_LB_DT(BS)
-k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flags, const Node* __restrict__ U,
-             double* __restrict__ d0, unsigned long long* __restrict__ vmax, int* __restrict__ flags_out, int all,
-             int remap) {
-    extern __shared__ double smem[];
-    double* lw = wave_lds(smem, M.wmax);
-    const int slice = wave_slice<BS>(M, remap);
+__device__ __forceinline__ void dt_slice(const Mesh& M, const InstDev* __restrict__ T, const int* __restrict__ flags,
+                                         const Node* __restrict__ U, double* __restrict__ d0,
+                                         unsigned long long* __restrict__ vmax, int* __restrict__ flags_out, int all,
+                                         int slice, double* lw) {
     const int lane = lane_of();
     bool wave_live = false;
     int inst = -1;
@@ -361,14 +360,21 @@ k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flag
     }
 }
 
+template <int BS>
+__global__ void SWE_LB_DT(BS)
+k_compute_dt(Mesh M, const InstDev* __restrict__ T, const int* __restrict__ flags, const Node* __restrict__ U,
+             double* __restrict__ d0, unsigned long long* __restrict__ vmax, int* __restrict__ flags_out, int all,
+             int remap) {
+    extern __shared__ double smem[];
+    dt_slice<BS>(M, T, flags, U, d0, vmax, flags_out, all, wave_slice<BS>(M, remap), wave_lds(smem, M.wmax));
+}
+
 // Per-instance step bookkeeping after the dt pass (SOURCES/main.f90:61-91):
 // an instance takes the step while time < tfinal (and its step limit and error flags allow);
 // dt = CFL*1/MAXVAL(ABS(d_ii)/m_i); the boundary values of the three stages are evaluated at
 // t+dt, (t+dt)+dt/2 and (t+dt/2)+dt; time = to + dt.  advance = 0 only computes dt.
-__global__ void k_finalize_dt(InstDev* __restrict__ T, unsigned long long* __restrict__ vmax,
-                              const int* __restrict__ flags, int ninst, int advance) {
-    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= ninst) return;
+__device__ __forceinline__ void finalize_one(InstDev* __restrict__ T, unsigned long long* __restrict__ vmax,
+                                             const int* __restrict__ flags, int k, int advance) {
     InstDev& D = T[k];
     if (D.state == INST_FREE) return;
     const double mx = __longlong_as_double((long long)vmax[k]);
@@ -407,6 +413,12 @@ __global__ void k_finalize_dt(InstDev* __restrict__ T, unsigned long long* __res
     D.go = 1;
 }
 
+__global__ void k_finalize_dt(InstDev* __restrict__ T, unsigned long long* __restrict__ vmax,
+                              const int* __restrict__ flags, int ninst, int advance) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < ninst) finalize_one(T, vmax, flags, k, advance);
+}
+
 // compute_dij with the full dij%aa (diagonal = -row sum), for swe_compute_dij.
 template <int BS>
 __global__ void __launch_bounds__(BS)
@@ -443,10 +455,8 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
 // =============================================================================================
 // alpha_limit, first loop: alpha_i = (|sum(h_j-h_i)| / sum|h_j-h_i|)**8, 1 if dry.
 // =============================================================================================
-template <int BS>
-__global__ void __launch_bounds__(BS)
-k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ alpha, int remap) {
-    const int slice = wave_slice<BS>(M, remap);
+__device__ __forceinline__ void alpha_slice(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict__ U,
+                                            double* __restrict__ alpha, int slice) {
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
@@ -454,6 +464,12 @@ k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doubl
     const int len = M.len[i];
     if (len == 0) return;
     alpha[i] = alpha_of(M, U, i, U[i].h, T[inst].K.htiny);
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_alpha(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, double* __restrict__ alpha, int remap) {
+    alpha_slice(M, T, U, alpha, wave_slice<BS>(M, remap));
 }
 
 // =============================================================================================
@@ -747,8 +763,7 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
             double2* __restrict__ ulo_q,
             double2* __restrict__ dmu, double* __restrict__ res, const double* __restrict__ d0,
             const double* __restrict__ lam, const double* __restrict__ alpha, int roundoff,
-            int* __restrict__ flags_out, int remap) {
-    const int slice = wave_slice<BS>(M, remap);
+            int* __restrict__ flags_out, int slice) {
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
@@ -926,7 +941,7 @@ k_fct_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, doub
          double2* __restrict__ ulo_q, double2* __restrict__ dmu, double* __restrict__ res,
          const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out, int remap) {
     fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, LAM_INLINE>(M, T, U, ulo_h, ulo_q, dmu, res, nullptr, nullptr, alpha,
-                                                          roundoff, flags_out, remap);
+                                                          roundoff, flags_out, wave_slice<BS>(M, remap));
 }
 
 template <int BS, bool ALPHA, bool WRITE_LO, bool HAS_SRC>
@@ -936,7 +951,7 @@ k_fct_lo1(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, dou
           const double* __restrict__ d0, const double* __restrict__ alpha, int roundoff, int* __restrict__ flags_out,
           int remap) {
     fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, LAM_D0>(M, T, U, ulo_h, ulo_q, dmu, res, d0, nullptr, alpha, roundoff,
-                                                      flags_out, remap);
+                                                      flags_out, wave_slice<BS>(M, remap));
 }
 
 // Stages 2 and 3 with lambda entry-parallel (schedule kind 1): the wave first evaluates
@@ -986,7 +1001,7 @@ k_fct_lo_lds(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
     }
     __syncthreads();
     fct_lo_body<BS, ALPHA, WRITE_LO, HAS_SRC, LAM_LDS>(M, T, U, ulo_h, ulo_q, dmu, res, nullptr, lw + 128, alpha,
-                                                       roundoff, flags_out, remap);
+                                                       roundoff, flags_out, slice);
 }
 
 // Antidiffusive flux f_ij of compute_fct_matrix (update.f90:732-745) for an off-diagonal entry,
@@ -1006,12 +1021,9 @@ __device__ __forceinline__ double fct_flux(const Node& I, const Node& J, double 
 // =============================================================================================
 // FCT stage, pass 2: P-, Q-, R- of FCT_positivity (fct.f90:15-30).
 // =============================================================================================
-template <int BS>
-__global__ void __launch_bounds__(BS)
-k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const double* __restrict__ ulo_h,
-             const double2* __restrict__ dmu, const double* __restrict__ res,
-             double* __restrict__ Rm, int remap) {
-    const int slice = wave_slice<BS>(M, remap);
+__device__ __forceinline__ void rminus_slice(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict__ U,
+                                             const double* __restrict__ ulo_h, const double2* __restrict__ dmu,
+                                             const double* __restrict__ res, double* __restrict__ Rm, int slice) {
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
@@ -1042,16 +1054,23 @@ k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
     Rm[i] = R;
 }
 
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_fct_rminus(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const double* __restrict__ ulo_h,
+             const double2* __restrict__ dmu, const double* __restrict__ res,
+             double* __restrict__ Rm, int remap) {
+    rminus_slice(M, T, U, ulo_h, dmu, res, Rm, wave_slice<BS>(M, remap));
+}
+
 // =============================================================================================
 // FCT stage, pass 3: l_ij, limited viscosities, apply_viscosity, update, RK combination.
 // =============================================================================================
-template <int BS>
-__global__ void __launch_bounds__(BS)
-k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
-            const double* __restrict__ ulo_h, const double2* __restrict__ ulo_q, Node* __restrict__ out,
-            const double2* __restrict__ dmu, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
-            int* __restrict__ flags_out, int remap) {
-    const int slice = wave_slice<BS>(M, remap);
+__device__ __forceinline__ void apply_slice(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict__ U,
+                                            const Node* __restrict__ Uo, const double* __restrict__ ulo_h,
+                                            const double2* __restrict__ ulo_q, Node* __restrict__ out,
+                                            const double2* __restrict__ dmu, const double* __restrict__ res,
+                                            const double* __restrict__ Rm, int stage, int* __restrict__ flags_out,
+                                            int slice) {
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
@@ -1109,6 +1128,15 @@ k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
     o.z = I.z;
     out[i] = o;
     raise_flag(flags_out, inst, err);
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS)
+k_fct_apply(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const Node* __restrict__ Uo,
+            const double* __restrict__ ulo_h, const double2* __restrict__ ulo_q, Node* __restrict__ out,
+            const double2* __restrict__ dmu, const double* __restrict__ res, const double* __restrict__ Rm, int stage,
+            int* __restrict__ flags_out, int remap) {
+    apply_slice(M, T, U, Uo, ulo_h, ulo_q, out, dmu, res, Rm, stage, flags_out, wave_slice<BS>(M, remap));
 }
 
 // =============================================================================================
@@ -1215,10 +1243,9 @@ k_ev_update(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
 // =============================================================================================
 // bdy: Dirichlet overwrites after a stage, for every instance that takes the step.
 // =============================================================================================
-__global__ void k_bdy(Node* __restrict__ u, const BcEntry* __restrict__ E, int n, const double* __restrict__ vals,
-                      int64_t stride, const InstDev* __restrict__ T, int stage) {
-    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (q >= n) return;
+__device__ __forceinline__ void bdy_one(Node* __restrict__ u, const BcEntry* __restrict__ E, int q,
+                                        const double* __restrict__ vals, int64_t stride,
+                                        const InstDev* __restrict__ T, int stage) {
     const BcEntry e = E[q];
     if (e.inst < 0) return;
     const InstDev& D = T[e.inst];
@@ -1237,6 +1264,147 @@ __global__ void k_bdy(Node* __restrict__ u, const BcEntry* __restrict__ E, int n
     } else {
         x.qy = v;
     }
+}
+
+__global__ void k_bdy(Node* __restrict__ u, const BcEntry* __restrict__ E, int n, const double* __restrict__ vals,
+                      int64_t stride, const InstDev* __restrict__ T, int stage) {
+    const int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q < n) bdy_one(u, E, q, vals, stride, T, stage);
+}
+
+// =============================================================================================
+// Persistent FCT step: every pass of `nsteps` steps in one cooperative launch, the passes
+// separated by grid-wide barriers instead of kernel boundaries (the same passes, the same per-slice
+// code and the same order as the launch sequence of one step, so the same bits).  Meant for
+// meshes too small to fill the GPU, where a dependent kernel launch costs more than the pass.
+// The barrier wait is bounded: past kBarrierSpins polls a workgroup raises the abort word and every
+// workgroup leaves, so the launch always ends; k_step_check then fails every stepping instance with
+// FLAG_STEP.  (The cooperative launch places every workgroup at once, so the bound is a guard.)
+// =============================================================================================
+struct StepArgs {
+    Mesh M;
+    InstDev* T;
+    int* flags;
+    Node *S0, *S1, *S2;
+    double* d0;
+    double* alpha;
+    double* ulo_h;
+    double2* ulo_q;
+    double2* dmu;
+    double* res;
+    double* Rm;
+    unsigned long long* vmax;
+    const BcEntry* bc;
+    const double* bcv;
+    int64_t bcv_stride;
+    int nbc;
+    int ninst;
+    int roundoff;
+    unsigned* bar;      // bar[0]: barrier arrivals since the launch; bar[1]: abort word
+};
+
+constexpr long kBarrierSpins = 1L << 21;
+
+// All threads of every workgroup call it together.  Each thread first waits until its own memory
+// operations have completed (they are in the L2 of its XCD); then one thread per workgroup
+// releases at device scope (the L2 written back, so the other XCDs see it), arrives, waits for
+// the other workgroups and acquires at device scope (the CU's L1 and the L2 invalidated), before
+// the workgroup goes on.
+__device__ __forceinline__ bool grid_barrier(const StepArgs& A, unsigned& epoch) {
+    __shared__ int ok;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    epoch += 1;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);   // the write-back is complete before the arrival counts
+        const unsigned target = epoch * gridDim.x;
+        // relaxed: the fences before and after order the workgroup's data
+        __hip_atomic_fetch_add(A.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int good = 1;
+        long spins = 0;
+        while (__hip_atomic_load(A.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(A.bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                good = 0;
+                break;
+            }
+            if (++spins > kBarrierSpins) {
+                __hip_atomic_fetch_or(A.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+#ifndef SWE_WAVES_STEP
+#define SWE_WAVES_STEP 2
+#endif
+template <bool ALPHA, bool HAS_SRC>
+__global__ void __launch_bounds__(256, SWE_WAVES_STEP)
+k_step_fct(StepArgs A, int nsteps) {
+    constexpr int BS = 256, WPB = BS / 64;
+    extern __shared__ double smem[];
+    double* lw = wave_lds(smem, A.M.wmax);
+    const Mesh& M = A.M;
+    const int wave = (int)(threadIdx.x >> 6);
+    const int nw = (int)gridDim.x * WPB;
+    const int gw = (int)blockIdx.x * WPB + wave;
+    const int gt = (int)(blockIdx.x * BS + threadIdx.x), nt = (int)gridDim.x * BS;
+    unsigned epoch = 0;
+    for (int step = 0; step < nsteps; ++step) {
+        // COMPUTE_DT: whole workgroups take rounds of WPB slices (the pass synchronises a workgroup)
+        for (int base = (int)blockIdx.x * WPB; base < M.nact; base += nw) {
+            const int w = base + wave;
+            const int slice = (w < M.nact) ? __builtin_amdgcn_readfirstlane(M.act[w]) : -1;
+            dt_slice<BS>(M, A.T, A.flags, A.S0, A.d0, A.vmax, A.flags, 0, slice, lw);
+        }
+        if (!grid_barrier(A, epoch)) return;
+        for (int k = gt; k < A.ninst; k += nt) finalize_one(A.T, A.vmax, A.flags, k, 1);
+        if (!grid_barrier(A, epoch)) return;
+        for (int stage = 0; stage < 3; ++stage) {
+            const Node* U = (stage == 0) ? A.S0 : (stage == 1) ? A.S1 : A.S2;
+            Node* out = (stage == 0) ? A.S1 : (stage == 1) ? A.S2 : A.S0;
+            if (ALPHA) {
+                for (int w = gw; w < M.nact; w += nw)
+                    alpha_slice(M, A.T, U, A.alpha, __builtin_amdgcn_readfirstlane(M.act[w]));
+                if (!grid_barrier(A, epoch)) return;
+            }
+            for (int w = gw; w < M.nact; w += nw) {
+                const int slice = __builtin_amdgcn_readfirstlane(M.act[w]);
+                if (stage == 0)
+                    fct_lo_body<BS, ALPHA, true, HAS_SRC, LAM_D0>(M, A.T, U, A.ulo_h, A.ulo_q, A.dmu, A.res, A.d0,
+                                                                  nullptr, A.alpha, A.roundoff, A.flags, slice);
+                else
+                    fct_lo_body<BS, ALPHA, true, HAS_SRC, LAM_INLINE>(M, A.T, U, A.ulo_h, A.ulo_q, A.dmu, A.res,
+                                                                      nullptr, nullptr, A.alpha, A.roundoff, A.flags,
+                                                                      slice);
+            }
+            if (!grid_barrier(A, epoch)) return;
+            for (int w = gw; w < M.nact; w += nw)
+                rminus_slice(M, A.T, U, A.ulo_h, A.dmu, A.res, A.Rm, __builtin_amdgcn_readfirstlane(M.act[w]));
+            if (!grid_barrier(A, epoch)) return;
+            for (int w = gw; w < M.nact; w += nw)
+                apply_slice(M, A.T, U, A.S0, A.ulo_h, A.ulo_q, out, A.dmu, A.res, A.Rm, stage, A.flags,
+                            __builtin_amdgcn_readfirstlane(M.act[w]));
+            if (!grid_barrier(A, epoch)) return;
+            for (int q = gt; q < A.nbc; q += nt) bdy_one(out, A.bc, q, A.bcv, A.bcv_stride, A.T, stage);
+            if (!grid_barrier(A, epoch)) return;
+        }
+    }
+}
+
+// After a persistent launch: if a barrier gave up, every instance that took the step fails.
+__global__ void k_step_check(const unsigned* __restrict__ bar, const InstDev* __restrict__ T, int* __restrict__ flags,
+                             int ninst) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < ninst && bar[1] != 0u && T[k].go) atomicOr(flags + k, FLAG_STEP);
 }
 
 // =============================================================================================
@@ -1673,6 +1841,43 @@ void launch_bdy(Pool& P, Node* u, int stage) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_bdy, dim3(blocks_of(n, 256)), dim3(256), 0, P.stream, u, (const BcEntry*)P.bc, (int)n,
                        (const double*)P.bcv, P.bcv_stride, (const InstDev*)P.tab, stage);
+}
+
+bool persistent_applies(const Pool& P) {
+    return P.mode_set && P.mode.viscosity_type == SWE_FCT && !P.mode.sgn && P.coop_ok;
+}
+
+int launch_step_persistent(Pool& P, int nsteps) {
+    const Mesh M = mesh_view(P);
+    if (M.nact == 0 || nsteps <= 0) return SWE_OK;
+    if (!persistent_applies(P)) return fail(SWE_EBADOPT, "the persistent step covers the fct scheme only");
+    StepArgs A{M,       P.tab,    P.flags,  P.S[0],  P.S[1],        P.S[2],    P.d0,
+               P.alpha, P.ulo_h,  P.ulo_q,  P.dmu,   P.res,         P.Rm,      P.vmax,
+               P.bc,    P.bcv,    P.bcv_stride,      (int)P.nbc(),  P.ninst(), P.mode.roundoff, P.bar};
+    const size_t lds = (size_t)(2 + M.wmax) * 64 * sizeof(double) * 4;
+    const bool alpha = P.mode.alpha != 0;
+    const void* fn = alpha ? (P.any_src ? (const void*)k_step_fct<true, true> : (const void*)k_step_fct<true, false>)
+                           : (P.any_src ? (const void*)k_step_fct<false, true> : (const void*)k_step_fct<false, false>);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds);
+    if (e != hipSuccess || per_cu < 1)
+        return fail(SWE_EHIP, "persistent step: no workgroup fits a CU (%s)", hipGetErrorString(e));
+    // every workgroup resident at once; no more than the slices need
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * P.num_cu, ((int64_t)M.nact + 3) / 4));
+    // the arrival count stays below 2^31 within a launch: at most 17 barriers per step
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(256, ((int64_t)1 << 30) / (17 * (int64_t)blocks)));
+    for (int done = 0; done < nsteps; done += chunk) {
+        int n = std::min(chunk, nsteps - done);
+        hipLaunchKernelGGL(k_poke, dim3(1), dim3(kPokeWords), 0, P.stream, (uint32_t*)P.bar, PokeWords{}, 2);
+        void* args[] = {(void*)&A, (void*)&n};
+        e = hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(256), args, (unsigned)lds, P.stream);
+        if (e != hipSuccess) return fail(SWE_EHIP, "persistent step launch: %s", hipGetErrorString(e));
+        hipLaunchKernelGGL(k_step_check, dim3(blocks_of(P.ninst(), 64)), dim3(64), 0, P.stream, (const unsigned*)P.bar,
+                           (const InstDev*)P.tab, P.flags, P.ninst());
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail(SWE_EHIP, "persistent step: %s", hipGetErrorString(e));
+    return SWE_OK;
 }
 
 void launch_dij_full(Pool& P, const Node* U, double* dij) {

@@ -46,7 +46,8 @@ struct Consts {
 };
 
 enum SrcKind : int { SRC_NONE = 0, SRC_MANNING = 1, SRC_LINEAR = 2, SRC_CORIOLIS = 3, SRC_SGN = 4 };
-enum : int { FLAG_NEG_H = 1, FLAG_LAMBDA = 2 };
+// FLAG_STEP: a grid barrier of the persistent step waited past its bound (the step did not finish)
+enum : int { FLAG_NEG_H = 1, FLAG_LAMBDA = 2, FLAG_STEP = 4 };
 
 // One 64-row slice (one wave).
 struct alignas(32) SliceInfo {

@@ -214,6 +214,17 @@ class Batch:
         _check(load_library().swe_batch_use_graph(self.h, int(bool(on))))
 
     @property
+    def step_form(self):
+        """As :attr:`swe_amd.solver.Solver.step_form`."""
+        f = C.c_int()
+        _check(load_library().swe_batch_get_step_form(self.h, C.byref(f)))
+        return f.value
+
+    @step_form.setter
+    def step_form(self, form):
+        _check(load_library().swe_batch_set_step_form(self.h, int(form)))
+
+    @property
     def stream(self):
         return load_library().swe_batch_stream(self.h)
 

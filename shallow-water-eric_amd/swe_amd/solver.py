@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libswe_amd.so")
 SWE_OK, SWE_ENEG_H, SWE_ELAMBDA, SWE_EBADOPT, SWE_EHIP, SWE_EINVAL, SWE_ENOMEM, SWE_EFULL = range(8)
 INSTANCE_RUNNING, INSTANCE_DONE, INSTANCE_FAILED = 1, 2, 3
 NUM_KERNELS = 7
+STEP_LAUNCHES, STEP_PERSISTENT = 0, 1
 KERNELS = ("k_compute_dt", "k_alpha", "k_stage_lo", "k_fct_lo", "k_fct_rminus", "k_fct_apply", "k_ev_update")
 GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
 BC_NONE, BC_WALLS, BC_INFLOW, BC_VALUES, BC_STAGES = 0, 1, 2, 3, 4
@@ -93,6 +94,8 @@ SIGNATURES = {
     "swe_get_schedule_kinds": (_I, [_P, _P]),
     "swe_set_schedule_kinds": (_I, [_P, _P]),
     "swe_use_graph": (_I, [_P, _I]),
+    "swe_set_step_form": (_I, [_P, _I]),
+    "swe_get_step_form": (_I, [_P, _P]),
     "swe_fct_generic_csr": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "swe_stream_bandwidth": (_I, [C.c_ulonglong, _I, C.POINTER(_D), C.POINTER(_D)]),
     "swe_step_begin": (_I, [_P, C.POINTER(_D)]),
@@ -117,6 +120,8 @@ SIGNATURES = {
     "swe_batch_free": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(_I)]),
     "swe_batch_tune": (_I, [_P, _I]),
     "swe_batch_use_graph": (_I, [_P, _I]),
+    "swe_batch_set_step_form": (_I, [_P, _I]),
+    "swe_batch_get_step_form": (_I, [_P, _P]),
     "swe_batch_stream": (_P, [_P]),
     "swe_batch_enable_kernel_timing": (_I, [_P, C.c_char_p]),
     "swe_batch_kernel_timing": (_I, [_P, C.c_char_p, C.POINTER(_I), C.POINTER(_D)]),
@@ -459,6 +464,18 @@ class Solver:
 
     def use_graph(self, on=True):
         _check(load_library().swe_use_graph(self.h, int(bool(on))))
+
+    @property
+    def step_form(self):
+        """STEP_LAUNCHES (kernel launches, captured graph per step) or STEP_PERSISTENT (fct: one
+        cooperative kernel per run of steps, grid barriers between the passes)."""
+        f = C.c_int()
+        _check(load_library().swe_get_step_form(self.h, C.byref(f)))
+        return f.value
+
+    @step_form.setter
+    def step_form(self, form):
+        _check(load_library().swe_set_step_form(self.h, int(form)))
 
     def error_evaluator(self, jj, rr):
         """compute_errors (main.f90:198-263) on the device state; jj (me, 3) 0-based, rr (np, 2)."""
