@@ -28,13 +28,18 @@ MESH_NAMES = {
     "bumps_lattice": "cell-split jittered lattice (structured topology)",
     "dam_break": "cell-split jittered lattice 2241 x 449",
     "paraboloid": "81 x 81 grid",
+    "sgn": "the SGN canal [0,10]x[0,1], cell-split jittered lattice 6401 x 641",
 }
 
 
-def kernel_bytes_per_node(kernel, k, alpha=True):
+def kernel_bytes_per_node(kernel, k, alpha=True, nc=3):
     """Algorithmic HBM bytes per node of one launch (one graph sweep, each array read once).
-    G = 4 + 20k: row pointer, column index and c_ij (16 B) per entry."""
+    G = 4 + 20k: row pointer, column index and c_ij (16 B) per entry.  nc = 5 (the SGN model): the
+    stage pass also reads and writes h*eta, h*w (16 B each way) and reads them at U^n in stages 2
+    and 3 (2/3 of 16 B on average)."""
     G = 4 + 20 * k
+    if nc == 5 and kernel == "k_stage_lo":
+        return kernel_bytes_per_node(kernel, k, alpha) + 16 + 16 + 16 * 2.0 / 3.0
     return {
         "k_fct_lo": G + 80,        # U,z,m,alpha in (48); U^L, res out (32)
         "k_fct_rminus": G + 72,    # U,z (32), res (8), U^L.h... (m, hL, res in; R out)
@@ -45,13 +50,15 @@ def kernel_bytes_per_node(kernel, k, alpha=True):
     }[kernel]
 
 
-def step_bytes_per_node_stage(k, fct=True, alpha=True):
+def step_bytes_per_node_stage(k, fct=True, alpha=True, nc=3):
     """Compulsory HBM bytes per node-stage of the whole step, k = stored entries per row: the sum of
     the per-launch figures of kernel_bytes_per_node over one stage (FCT: three graph sweeps plus
-    the alpha pass and a third of the dt pass), i.e. 324 + 70.67k with FCT and alpha."""
+    the alpha pass and a third of the dt pass), i.e. 324 + 70.67k with FCT and alpha; nc = 5 adds
+    the SGN model's two components as in kernel_bytes_per_node."""
     if fct:
         return 324 + 70.67 * k
-    return (124 + 30.67 * k) if alpha else (96 + 26.67 * k)
+    extra = (16 + 16 + 16 * 2.0 / 3.0) if nc == 5 else 0.0
+    return ((124 + 30.67 * k) if alpha else (96 + 26.67 * k)) + extra
 
 
 def measured_traffic(kernel, workload, nodes):
@@ -73,6 +80,24 @@ def measured_traffic(kernel, workload, nodes):
     return None
 
 
+def measured_valu(kernel, workload, nodes):
+    """(VALU busy fraction, active-lane fraction) of `kernel` over a step's launches from the committed
+    rocprofv3 VALU summary for this workload and mesh size (profiles/), or None."""
+    pdir = os.path.join(REPO, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".txt"):
+            continue
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                parts = line.split()
+                if (len(parts) == 6 and parts[0] == "valu_per_launch" and parts[1] == kernel
+                        and parts[2] == workload and int(parts[3]) == nodes):
+                    return float(parts[4]), float(parts[5])
+    return None
+
+
 def build_problem(name, size=None, host_assembly=True):
     from swe_amd import problems
     if name == "bumps":
@@ -84,6 +109,9 @@ def build_problem(name, size=None, host_assembly=True):
         return problems.dam_break(nx=nx, ny=(nx - 1) // 5 + 1)
     if name == "paraboloid":
         return problems.paraboloid(n=size or 81)
+    if name == "sgn":
+        nx = size or 6401
+        return problems.sgn_canal(nx=nx, ny=(nx - 1) // 10 + 1)
     raise SystemExit("unknown workload %s" % name)
 
 
@@ -119,15 +147,16 @@ def reference_baseline(name, p, seconds, driver=None, all_cores=True):
     import shutil
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as O
-    test = {"bumps": 4, "dam_break": 10, "paraboloid": 5}[name]
+    test = {"bumps": 4, "dam_break": 10, "paraboloid": 5, "sgn": 13}[name]
     cores = host_cores()
     pin = cores[0] if shutil.which("taskset") else None
     d = O.tmpdir()
     try:
         init = O.ref_case(d, p, test)
-        el0 = O.ref_time_loop(d, init, 1, driver=driver, pin=pin, out="one")[3]
+        nc = 5 if test == 13 else 3
+        el0 = O.ref_time_loop(d, init, 1, driver=driver, pin=pin, out="one", nc=nc)[3]
         n = max(1, int(seconds / max(el0, 1e-6)))
-        el = O.ref_time_loop(d, init, n, driver=driver, pin=pin, out="pinned")[3]
+        el = O.ref_time_loop(d, init, n, driver=driver, pin=pin, out="pinned", nc=nc)[3]
         res = {"value": p.np * 3 * n / el, "unit": "node-stage updates/s", "cores": 1, "kind": "reference",
                "cpu_model": cpu_model(), "pinned_core": pin,
                "sample": "%s, %d nodes, %d steps (%.1f s of time loop), the reference Fortran (amdflang -O2 build of "
@@ -164,7 +193,7 @@ def cpu_baseline(name, seconds=12.0):
     reference build, the C restatement alone."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as O
-    size = {"bumps": 262_144, "dam_break": 561, "paraboloid": 81}[name]
+    size = {"bumps": 262_144, "dam_break": 561, "paraboloid": 81, "sgn": 1601}[name]
     p = build_problem(name, size)
     port = port_baseline(name, p, seconds)
     if not (O.ref_available() and os.path.exists(O.REF_DRIVER_HOIST)):
@@ -173,7 +202,7 @@ def cpu_baseline(name, seconds=12.0):
         ref = reference_baseline(name, p, seconds, driver=O.REF_DRIVER_HOIST)
         # the unmodified reference, whose rate falls with the mesh size (O(np^2) rescale loop), on a
         # mesh a quarter the size, a few seconds
-        small = build_problem(name, {"bumps": 65_536, "dam_break": 281, "paraboloid": 81}[name])
+        small = build_problem(name, {"bumps": 65_536, "dam_break": 281, "paraboloid": 81, "sgn": 801}[name])
         plain = reference_baseline(name, small, seconds / 3, all_cores=False)
     except (RuntimeError, OSError, subprocess.SubprocessError) as e:
         print("# reference baseline failed (%s); reporting the C restatement" % str(e)[:200],
@@ -195,7 +224,7 @@ def port_baseline(name, p, seconds):
     orc = O.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=pr["gravity"], htiny=pr["htiny"], cfl=pr["cfl"],
                    max_water_h=pr["max_water_h"], viscosity_type=visc[pr["viscosity"]],
                    viscous_type=pr["viscous_type"], alpha_limit=pr["alpha_limit"], type_test=pr["type_test"],
-                   mannings=pr["mannings"])
+                   mannings=pr["mannings"], lambda_sgn=pr.get("lambda_sgn", 0.0))
     kind, amp, period, level = p.bc
     kw = dict(bc_kind=kind, amp=amp, period=period, level=level, nodes_h=hd, nodes_ux=uxd, nodes_uy=uyd)
     u = p.u0
@@ -587,8 +616,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="bumps",
-                    choices=["bumps", "bumps_lattice", "dam_break", "paraboloid", "batch", "serve", "setup",
-                             "errors"])
+                    choices=["bumps", "bumps_lattice", "dam_break", "paraboloid", "sgn", "batch", "serve",
+                             "setup", "errors"])
     ap.add_argument("--instances", type=int, default=256, help="batch: meshes in the batch")
     ap.add_argument("--requests", type=int, default=64, help="serve: requests in the stream")
     ap.add_argument("--rates", default="20", help="serve: arrival rates (per s), comma separated")
@@ -661,6 +690,7 @@ def main():
     nnz = s.nnz
     k = nnz / p.np
     fct = p.params["viscosity"] == "fct"
+    nc = 5 if p.params.get("type_test") == 13 else 3
     alpha = p.params["alpha_limit"]
     dominant = "k_fct_lo" if fct else "k_stage_lo"
     print("# workload %s: %d nodes, %d entries (k=%.3f), mesh and data %.1f s, solver %.1f s"
@@ -699,11 +729,11 @@ def main():
             s.enable_kernel_timing(None)
         if n_l:
             ms_l = t_ms / n_l
-            gbs = kernel_bytes_per_node(name, k, alpha) * p.np / (ms_l / 1e3) / 1e9
+            gbs = kernel_bytes_per_node(name, k, alpha, nc) * p.np / (ms_l / 1e3) / 1e9
             per_kernel[name] = {"launch_ms": ms_l, "per_step": n_l / (args.steps if name == dominant else 3),
                                 "alg_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
     rate = p.np * 3 * args.steps / el
-    bpn = kernel_bytes_per_node(dominant, k, alpha)
+    bpn = kernel_bytes_per_node(dominant, k, alpha, nc)
     avg_s = kms / 1e3 / max(nlaunch, 1)
     achieved = bpn * p.np / avg_s / 1e9
     ceiling = hbm_ceiling()
@@ -728,12 +758,13 @@ def main():
                    "hmin_end": hmin, "schedule": schedule, "setup": "device" if device_setup else "host"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
+                     "valu_busy_and_lanes": measured_valu(dominant, p.name, p.np),
                      "launch_ms": avg_s * 1e3, "bytes_per_node": bpn, "achievable_GBs": achievable,
                      "frac_of_achievable": achieved / achievable},
         "hbm_ceiling": ceiling,
-        "step_effective_GBs": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9,
-        "step_frac": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9 / HBM_PEAK_GBS,
-        "step_frac_of_achievable": rate * step_bytes_per_node_stage(k, fct, alpha) / 1e9 / achievable,
+        "step_effective_GBs": rate * step_bytes_per_node_stage(k, fct, alpha, nc) / 1e9,
+        "step_frac": rate * step_bytes_per_node_stage(k, fct, alpha, nc) / 1e9 / HBM_PEAK_GBS,
+        "step_frac_of_achievable": rate * step_bytes_per_node_stage(k, fct, alpha, nc) / 1e9 / achievable,
         "kernels": per_kernel,
     }
     s.close()

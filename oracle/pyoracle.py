@@ -299,7 +299,8 @@ def ref_case(d, problem, test, mesh_file="mesh.FEM"):
     data_file(os.path.join(d, "data"), mesh_dir=".", mesh_file=mesh_file, test=test, cfl=pr["cfl"],
               viscosity=pr["viscosity"], viscous_type="type%d" % pr["viscous_type"], alpha=pr["alpha_limit"],
               h_dir=problem.h_sides, ux_dir=problem.ux_sides, uy_dir=problem.uy_sides,
-              mannings=pr["mannings"] if test in (9, 10) else None)
+              mannings=pr["mannings"] if test in (9, 10) else None,
+              lambda_sgn=pr.get("lambda_sgn") if test == 13 else None)
     if problem.reference_init:
         return "none"
     kind, amp, period, level = problem.bc
@@ -308,13 +309,13 @@ def ref_case(d, problem, test, mesh_file="mesh.FEM"):
     return "init.bin"
 
 
-def ref_time_loop(d, init, nsteps, driver=None, pin=None, out="run", timeout=3600):
-    """nsteps of the driver's SSP-RK3 loop on the case in d: (final (np,3), dts, end time, seconds of
+def ref_time_loop(d, init, nsteps, driver=None, pin=None, out="run", timeout=3600, nc=3):
+    """nsteps of the driver's SSP-RK3 loop on the case in d: (final (np,nc), dts, end time, seconds of
     the time loop by the driver's own clock, set-up excluded)."""
     od = os.path.join(d, out)
     os.makedirs(od, exist_ok=True)
     run_ref(["run", "data", init, nsteps, out], cwd=d, timeout=timeout, driver=driver, pin=pin)
-    final = np.fromfile(os.path.join(od, "final.f8"), dtype="<f8").reshape(-1, 3)
+    final = np.fromfile(os.path.join(od, "final.f8"), dtype="<f8").reshape(-1, nc)
     return (final, np.fromfile(os.path.join(od, "dt.f8")), float(np.fromfile(os.path.join(od, "time.f8"))[0]),
             float(np.fromfile(os.path.join(od, "loop_seconds.f8"))[0]))
 

@@ -132,6 +132,41 @@ def dam_break(nx=2241, ny=449, jitter=0.2, seed=20261202, cfl=0.25, mannings=0.0
 
 
 # ----------------------------------------------------------------------------------------------
+def sgn_solitary_wave(rr, g=9.81):
+    """The SGN model's initial state of test 13: the Favrie-Gavrilyuk solitary wave of
+    sol_anal(1..5, rr, t=0) (SOURCES/condlim.f90:751-827): h1 = 0.10, h2 = 0.11, crest at x0 = 2.75,
+    celerity D = sqrt(g h2); components h, h*u, h*v, h*eta = h^2, h*w."""
+    h1, h2, x0 = 10.0 / 100.0, 11.0 / 100.0, 2.75
+    D = math.sqrt(g * h2)
+    zz = math.sqrt((3.0 * (h2 - h1)) / (h2 * h1 ** 2.0))
+    arg = 1.0 / 2.0 * zz * (rr[:, 0] - x0)
+    sech2 = (1.0 / np.cosh(arg)) ** 2.0
+    h = np.maximum(h1 + (h2 - h1) * sech2, 0.0)
+    u = np.empty((rr.shape[0], 5))
+    u[:, 0] = h
+    u[:, 1] = h * D - h1 * D
+    u[:, 2] = h * 0.0
+    u[:, 3] = h * h
+    u[:, 4] = -D * h1 * ((h1 - h2) * zz * sech2 * np.tanh(arg))
+    return u
+
+
+def sgn_canal(nx=6401, ny=641, jitter=0.2, seed=20261202, viscosity="viscous", alpha=True, cfl=0.08,
+              lambda_sgn=9.0) -> Problem:
+    """The SGN model (test 13) at scale: the reference's canal [0,10]x[0,1]
+    (MESHES/SGN_CANAL/rectangle.*.FEM, data lambdaSGN = 9, CFL 0.08, max_water_h = 0.11) on an
+    nx-by-ny jittered lattice (6401x641 = 4,103,041 nodes), the solitary wave of sgn_solitary_wave,
+    flat bottom, no Dirichlet nodes (the wave stays far from the ends over a bench run), first-order
+    viscous scheme with the alpha limiter."""
+    g, max_water_h = 9.81, 0.11
+    params = dict(gravity=g, htiny=1.0e-4 * max_water_h, cfl=cfl, max_water_h=max_water_h, viscosity=viscosity,
+                  viscous_type=2, alpha_limit=alpha, type_test=13, mannings=0.0, lambda_sgn=lambda_sgn)
+    return _finish("sgn_canal", M.structured_mesh(nx, ny, 0.0, 10.0, 0.0, 1.0, jitter=jitter, seed=seed),
+                   lambda rr: np.zeros(rr.shape[0]), lambda rr, z: sgn_solitary_wave(rr, g), params, nsteps=200,
+                   reference_init=True)
+
+
+# ----------------------------------------------------------------------------------------------
 def bump_field(rr, L, nbumps=64, seed=20261202):
     """z = sum_k A_k exp(-|x-c_k|^2/(2 w_k^2)), A ~ U[0,0.8], w ~ U[0.02L, 0.1L], c ~ U[0,L]^2."""
     rng = np.random.default_rng(seed)

@@ -156,3 +156,32 @@ def test_reference_driver_runs_sgn_on_the_gpu(tmp_path):
     ref, gpu = outs
     assert len([l for l in ref if l.startswith("time")]) > 40
     assert gpu == ref
+
+
+def test_sgn_canal_4m_nodes_against_oracle_and_properties(oracle):
+    """The SGN model at scale: the solitary wave in the reference's canal on a 6401x641 lattice
+    (4,103,041 nodes, viscous type2 + alpha): the first two steps bitwise against the oracle on the
+    whole mesh, then 100 more steps through size-independent properties (every value finite, depth
+    >= 0, h*eta > 0 where h > 0)."""
+    from swe_amd import problems
+    p = problems.sgn_canal()
+    print("SGN canal", p.np, p.nnz, flush=True)
+    pr = p.params
+    orc = oracle.Oracle(p.ia, p.ja, p.cij, p.lumped, p.bath, gravity=pr["gravity"], htiny=pr["htiny"], cfl=pr["cfl"],
+                        max_water_h=pr["max_water_h"], viscosity_type=oracle.VISCOUS, viscous_type=2,
+                        alpha_limit=True, type_test=13, lambda_sgn=pr["lambda_sgn"])
+    try:
+        ref, rdts, rt, err = orc.run(p.u0, 2)
+    finally:
+        orc.close()
+    assert err == 0
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t, dts = s.run(2, time=0.0)
+        u = s.get_state()
+        assert u.shape == (p.np, 5) and same_bits(dts, rdts) and t == rt and same_bits(u, ref)
+        t, dts = s.run(100)
+        u = s.get_state()
+        print("SGN canal after 102 steps: t = %.6g, min h %.6g" % (t, u[:, 0].min()), flush=True)
+        assert np.all(np.isfinite(u)) and u[:, 0].min() >= 0.0 and np.all(dts > 0)
+        assert np.all(u[u[:, 0] > 0, 3] > 0)
