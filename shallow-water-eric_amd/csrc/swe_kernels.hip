@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <type_traits>
@@ -1301,6 +1302,7 @@ struct StepArgs {
     int ninst;
     int roundoff;
     unsigned* bar;      // bar[0]: barrier arrivals since the launch; bar[1]: abort word
+    long spin_limit;    // polls a barrier waits at most (kBarrierSpins; SWE_BARRIER_SPINS overrides)
 };
 
 constexpr long kBarrierSpins = 1L << 21;
@@ -1328,7 +1330,7 @@ __device__ __forceinline__ bool grid_barrier(const StepArgs& A, unsigned& epoch)
                 good = 0;
                 break;
             }
-            if (++spins > kBarrierSpins) {
+            if (++spins > A.spin_limit) {
                 __hip_atomic_fetch_or(A.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 good = 0;
                 break;
@@ -1400,11 +1402,12 @@ k_step_fct(StepArgs A, int nsteps) {
     }
 }
 
-// After a persistent launch: if a barrier gave up, every instance that took the step fails.
+// After a persistent launch: if a barrier gave up, every running instance fails (its step may have
+// stopped anywhere).
 __global__ void k_step_check(const unsigned* __restrict__ bar, const InstDev* __restrict__ T, int* __restrict__ flags,
                              int ninst) {
     const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k < ninst && bar[1] != 0u && T[k].go) atomicOr(flags + k, FLAG_STEP);
+    if (k < ninst && bar[1] != 0u && T[k].state == INST_RUNNING) atomicOr(flags + k, FLAG_STEP);
 }
 
 // =============================================================================================
@@ -1851,9 +1854,13 @@ int launch_step_persistent(Pool& P, int nsteps) {
     const Mesh M = mesh_view(P);
     if (M.nact == 0 || nsteps <= 0) return SWE_OK;
     if (!persistent_applies(P)) return fail(SWE_EBADOPT, "the persistent step covers the fct scheme only");
+    // SWE_BARRIER_SPINS (environment) lowers the barrier's wait bound, to exercise the path where a
+    // barrier gives up (every workgroup leaves, the stepping instances fail)
+    const char* sp = std::getenv("SWE_BARRIER_SPINS");
+    const long spins = sp ? std::max(0L, std::atol(sp)) : kBarrierSpins;
     StepArgs A{M,       P.tab,    P.flags,  P.S[0],  P.S[1],        P.S[2],    P.d0,
                P.alpha, P.ulo_h,  P.ulo_q,  P.dmu,   P.res,         P.Rm,      P.vmax,
-               P.bc,    P.bcv,    P.bcv_stride,      (int)P.nbc(),  P.ninst(), P.mode.roundoff, P.bar};
+               P.bc,    P.bcv,    P.bcv_stride,      (int)P.nbc(),  P.ninst(), P.mode.roundoff, P.bar, spins};
     const size_t lds = (size_t)(2 + M.wmax) * 64 * sizeof(double) * 4;
     const bool alpha = P.mode.alpha != 0;
     const void* fn = alpha ? (P.any_src ? (const void*)k_step_fct<true, true> : (const void*)k_step_fct<true, false>)

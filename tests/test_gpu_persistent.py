@@ -116,3 +116,29 @@ def test_step_form_refusals():
             s.step_form = 7
         assert e.value.status == SWE_EINVAL
         assert s.step_form == STEP_LAUNCHES
+
+
+def test_barrier_that_gives_up_fails_the_step(monkeypatch):
+    """A barrier whose wait bound is forced to zero from the host (SWE_BARRIER_SPINS=0): the workgroups
+    leave the persistent kernel at once, and the run reports SWE_EHIP instead of a state; a new solver
+    with the normal bound then runs bitwise as before."""
+    from swe_amd import problems
+    from swe_amd.solver import STEP_PERSISTENT, SWE_EHIP, SweError
+    p = problems.bumps(n_side=65, L=1000.0, seed=11)
+    with p.solver() as s:
+        s.set_state(p.u0)
+        t_ref, _ = s.run(5, time=0.0)
+        ref = s.get_state()
+    monkeypatch.setenv("SWE_BARRIER_SPINS", "0")
+    with p.solver() as s:
+        s.step_form = STEP_PERSISTENT
+        s.set_state(p.u0)
+        with pytest.raises(SweError) as e:
+            s.run(5, time=0.0)
+        assert e.value.status == SWE_EHIP
+    monkeypatch.delenv("SWE_BARRIER_SPINS")
+    with p.solver() as s:
+        s.step_form = STEP_PERSISTENT
+        s.set_state(p.u0)
+        t, _ = s.run(5, time=0.0)
+        assert t == t_ref and same_bits(s.get_state(), ref)
