@@ -53,6 +53,9 @@ constexpr double kPi = 3.141592653589793;   // ACOS(-1.d0)
 #ifndef SWE_UNROLL_LO
 #define SWE_UNROLL_LO 1
 #endif
+#ifndef SWE_UNROLL_LO1
+#define SWE_UNROLL_LO1 1
+#endif
 #ifndef SWE_UNROLL_RM
 #define SWE_UNROLL_RM 1
 #endif
@@ -786,7 +789,8 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
     double emax = 0.0, emin = 0.0;
     bool split = false;   // the entropy residual's momentum sums have left the low-order ones
     const int64_t pb = row_base(M, i);
-    SWE_UNROLL(SWE_UNROLL_LO)
+    constexpr int kUnroll = (LAM == LAM_D0) ? SWE_UNROLL_LO1 : SWE_UNROLL_LO;
+#pragma unroll kUnroll
     for (int q = 0; q < len; ++q) {
         const int64_t p = pb + 64 * (int64_t)q;
         const int j = col_at(M, p, i);
