@@ -755,7 +755,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": p.name, "mesh": MESH_NAMES.get(args.workload, ""), "nodes": p.np, "entries": nnz,
                    "viscosity": p.params["viscosity"], "alpha_limit": bool(alpha), "steps": args.steps,
-                   "hmin_end": hmin, "schedule": schedule, "setup": "device" if device_setup else "host"},
+                   "hmin_end": hmin, "schedule": schedule, "step_form": s.step_form,
+                   "setup": "device" if device_setup else "host"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, p.name, p.np),
                      "valu_busy_and_lanes": measured_valu(dominant, p.name, p.np),
