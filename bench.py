@@ -280,14 +280,16 @@ def _batch_caps(ps, spare_rows=0):
 
 
 def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=5000, nmax=500000, dropin_steps=3,
-                dropin_count=None):
+                dropin_count=None, device_setup=True):
     """Batched form: `count` meshes of the 16.8M-node workload's family, sizes log-uniform in
     [nmin, nmax], dam-break or still-water initial data, all resident in one batch and advanced
     together (one captured graph per step, autotuned schedules); against two plain per-instance
     loops over the same meshes one at a time: the device-resident loop (state kept on the device,
     one captured graph per step) and the drop-in loop (the reference's host time loop calling
     compute_dij and euler per stage, `dropin_steps` steps per mesh, on the first `dropin_count`
-    meshes, a random sample of the family, when given)."""
+    meshes, a random sample of the family, when given).  device_setup: the batch and the
+    device-resident loop admit each mesh from its element list (assembly and layout on the GPU, with
+    the Hilbert locality order) rather than from the host assembly and layout (reverse Cuthill-McKee)."""
     from swe_amd import problems
     from swe_amd.batch import Batch
     t0 = time.perf_counter()
@@ -300,7 +302,7 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
     total = warmup + steps
     b = Batch(rows, entries, bnd, count)
     t0 = time.perf_counter()
-    ids = [b.add_problem(p, max_steps=total) for p in ps]
+    ids = [b.add_problem(p, max_steps=total, device_setup=device_setup) for p in ps]
     t_admit = time.perf_counter() - t0
     if tune:
         b.tune(2)
@@ -320,7 +322,7 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
     failed = sum(1 for st in stat if st[3] != 0)
     b.close()
     b = Batch(rows, entries, bnd, count)
-    ids = [b.add_problem(p, max_steps=total) for p in ps]
+    ids = [b.add_problem(p, max_steps=total, device_setup=device_setup) for p in ps]
     if tune:
         b.tune(2)
     b.step(warmup)
@@ -337,7 +339,7 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
     el_plain, work_plain = 0.0, 0
     chunk = 10
     for p in ps:
-        k = one.add_problem(p, max_steps=total)
+        k = one.add_problem(p, max_steps=total, device_setup=device_setup)
         one.step(warmup)
         one.sync()
         n0 = one.status(k)[2]
@@ -379,7 +381,7 @@ def bench_batch(count, steps, warmup, workers, tune=True, seed=20261202, nmin=50
             "dropin_node_stage_per_s": rate_d, "dropin_steps": dropin_steps,
             "dropin_meshes": len(ps[:dropin_count]), "speedup_vs_dropin": rate_b / rate_d,
             "failed_instances": failed,
-            "admission_s": t_admit,
+            "admission_s": t_admit, "setup": "device" if device_setup else "host",
             "roofline": {"bound": "hbm", "kernel": "k_fct_lo", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "launch_ms": launch_s * 1e3,
                          "bytes_per_node": bpn}}
@@ -438,7 +440,8 @@ def replay_fifo(arrivals, durations):
     return fin
 
 
-def bench_serve_sweep(count, rates, workers, chunk=16, budget_gb=200.0, seed=20261202, median=100000, nmax=2000000):
+def bench_serve_sweep(count, rates, workers, chunk=16, budget_gb=200.0, seed=20261202, median=100000, nmax=2000000,
+                      device_setup=True):
     """The served form over several arrival rates: the same `count` requests (sizes lognormal with
     the given median, capped at nmax, final times of 1-20 wave periods, 20% dry-bed dam breaks; the
     seed fixes them, the rate only scales the arrival gaps) served by continuous batching at each
@@ -464,7 +467,7 @@ def bench_serve_sweep(count, rates, workers, chunk=16, budget_gb=200.0, seed=202
     big = max(ps, key=lambda p: p.nnz)
     r1, e1, b1 = _batch_caps([big])
     one = Batch(r1, e1, max(b1, max(sum(x.size for x in p.dirichlet()) for p in ps) + 64), 1)
-    plain = run_plain(BatchEngine(one), reqs(base), chunk=chunk)
+    plain = run_plain(BatchEngine(one, device_setup=device_setup), reqs(base), chunk=chunk)
     one.close()
     plain_by_id = {r.rid: r for r in plain}
     dur = [plain_by_id[q].finished - max(plain_by_id[q].admitted, 0.0) for q in range(count)]
@@ -474,11 +477,12 @@ def bench_serve_sweep(count, rates, workers, chunk=16, budget_gb=200.0, seed=202
     out = {"requests": count, "budget_gb": budget_gb, "chunk_steps": chunk, "nodes": int(sum(p.np for p in ps)),
            "max_nodes": int(max(p.np for p in ps)),
            "dry_bed_fraction": sum(1 for x in base if x["kind"] == "drydam") / float(count),
+           "setup": "device" if device_setup else "host",
            "plain_solve_s": float(sum(dur)), "rates": []}
     for rate in rates:
         sp = specs[rate]
         b = Batch.for_budget(min(budget, 1.05 * need + (1 << 30)), max_instances=max(64, count))
-        served = Server(BatchEngine(b, device_setup=False), chunk=chunk).run(reqs(sp))
+        served = Server(BatchEngine(b, device_setup=device_setup), chunk=chunk).run(reqs(sp))
         b.close()
         same = all(a.steps == plain_by_id[a.rid].steps and a.time == plain_by_id[a.rid].time and
                    np.array_equal(a.state, plain_by_id[a.rid].state, equal_nan=True) for a in served)
