@@ -657,7 +657,7 @@ def main():
         res = {"metric": "node-stage updates/s (FP64, SSP-RK3, FCT + alpha), batched", "unit": "node-stage updates/s",
                "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
                "data": "synthetic"}
-        r = bench_batch(args.instances, args.steps, args.warmup, args.workers)
+        r = bench_batch(args.instances, args.steps, args.warmup, args.workers, tune=not args.no_tune)
         res.update({"value": r["node_stage_per_s"], "steps": args.steps, "warmup": args.warmup,
                     "ms_per_step": r["ms_per_step"], "roofline": r.pop("roofline"),
                     "config": {"workload": "batch", **r}})
