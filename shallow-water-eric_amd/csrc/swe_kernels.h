@@ -19,6 +19,26 @@ namespace swe {
 __device__ __forceinline__ double fmaxF(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double fminF(double a, double b) { return (a < b) ? a : b; }
 
+// SQRT of an argument that is a finite double of at least 2^-767: the instructions of the correctly
+// rounded sqrt hipcc emits for gfx950 (reciprocal square root, then Newton corrections in FMA),
+// without its scaling of tiny arguments and its handling of 0 and infinity, which such arguments
+// do not need -- the same bits for every such argument (checked exhaustively on random arguments).
+// Used where the argument is >= 1 (lambda's wave speeds) or a squared |c_ij| of an off-diagonal
+// entry (zero only on a degenerate mesh, on which the reference's d_ij is NaN).
+__device__ __forceinline__ double sqrt_normal(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return g;
+}
+
 __device__ __forceinline__ double ovh_of(double h, double htiny) {
     const double m = fmaxF(h, htiny);
     return 2.0 * h / (h * h + m * m);
@@ -77,10 +97,11 @@ __device__ __forceinline__ double lambda_vacc(double hl, double qxl, double qyl,
             ht = sqrmin * sqrmax * (1.0 + K.sqrt_2_g * (vl - vr) / (sqrmin + sqrmax));
         }
     }
-    const double lbdl = vl - sql * sqrt((1.0 + fmaxF((ht - hl) * ovhl / 2.0, 0.0)) *
-                                        (1.0 + fmaxF((ht - hl) * ovhl, 0.0)));
-    const double lbdr = vr + sqr * sqrt((1.0 + fmaxF((ht - hr) * ovhr / 2.0, 0.0)) *
-                                        (1.0 + fmaxF((ht - hr) * ovhr, 0.0)));
+    // the arguments are products of two factors >= 1
+    const double lbdl = vl - sql * sqrt_normal((1.0 + fmaxF((ht - hl) * ovhl / 2.0, 0.0)) *
+                                               (1.0 + fmaxF((ht - hl) * ovhl, 0.0)));
+    const double lbdr = vr + sqr * sqrt_normal((1.0 + fmaxF((ht - hr) * ovhr / 2.0, 0.0)) *
+                                               (1.0 + fmaxF((ht - hr) * ovhr, 0.0)));
     return fmaxF(fabs(lbdl), fabs(lbdr));
 }
 

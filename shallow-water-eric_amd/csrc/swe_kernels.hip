@@ -215,7 +215,7 @@ __device__ __forceinline__ void add_sources(int src, const Node& I, double mi, d
 // instead of storing 24 bytes of normal and norm per entry (measured faster in every pass).
 __device__ __forceinline__ double dij_from_c(const double2 c, const Node& I, const Node& J, double ovh_i,
                                              double ovh_j, double sql, double sqr, const Consts& K, int& err) {
-    const double norm = sqrt(0.0 + c.x * c.x + c.y * c.y);
+    const double norm = sqrt_normal(0.0 + c.x * c.x + c.y * c.y);
     return dij_of(make_double2(c.x / norm, c.y / norm), norm, I, J, ovh_i, ovh_j, sql, sqr, K, err);
 }
 
