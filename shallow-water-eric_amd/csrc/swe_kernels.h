@@ -39,6 +39,15 @@ __device__ __forceinline__ double sqrt_normal(double x) {
     return g;
 }
 
+// SQRT, the same bits as sqrt() for every argument: sqrt_normal, and sqrt() itself for the lanes
+// whose argument is not a finite double >= 2^-767 (zero on dry nodes, negative, infinite, NaN), a
+// branch only the waves that have such a lane take.
+__device__ __forceinline__ double sqrt_rn(double x) {
+    double r = sqrt_normal(x);
+    if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) r = sqrt(x);
+    return r;
+}
+
 __device__ __forceinline__ double ovh_of(double h, double htiny) {
     const double m = fmaxF(h, htiny);
     return 2.0 * h / (h * h + m * m);
@@ -53,8 +62,8 @@ __device__ __forceinline__ double phi_f(double h, double ul, double hl, double u
     const bool gl = h > hl, gr = h > hr;
     const double ovh = ovh_of(h, K.htiny);
     const double gh = K.g * h;
-    const double Sl = sqrt(gl ? K.g_half * (ovh + ovhl) : gh);
-    const double Sr = sqrt(gr ? K.g_half * (ovh + ovhr) : gh);
+    const double Sl = sqrt_rn(gl ? K.g_half * (ovh + ovhl) : gh);
+    const double Sr = sqrt_rn(gr ? K.g_half * (ovh + ovhr) : gh);
     const double nan = __builtin_nan("");
     const double sgl = (hl >= 0.0) ? sql : nan;
     const double sgr = (hr >= 0.0) ? sqr : nan;
@@ -82,8 +91,8 @@ __device__ __forceinline__ double lambda_vacc(double hl, double qxl, double qyl,
         ht = fminF(K.x0 * hmin, t * t / K.g16);
     } else {
         const double fh2 = phi_f(K.x0 * hmax, vl, hl, vr, hr, ovhl, ovhr, sql, sqr, K);
-        const double sqrmin = sqrt(hmin);
-        const double sqrmax = sqrt(hmax);
+        const double sqrmin = sqrt_rn(hmin);
+        const double sqrmax = sqrt_rn(hmax);
         if (0.0 <= fh2) {
             const double cc = -hmin * K.a_c2 - sqrmin * sqrmax + sqrmin * (vr - vl) / K.two_sqrt_g;
             const double Delta = hmin - 4.0 * K.a_c2 * cc;
@@ -91,7 +100,7 @@ __device__ __forceinline__ double lambda_vacc(double hl, double qxl, double qyl,
                 err |= FLAG_LAMBDA;
                 return 0.0;
             }
-            const double t = (-sqrmin + sqrt(Delta)) / (2.0 * K.a_c2);
+            const double t = (-sqrmin + sqrt_rn(Delta)) / (2.0 * K.a_c2);
             ht = fminF(K.x0 * hmax, t * t);
         } else {
             ht = sqrmin * sqrmax * (1.0 + K.sqrt_2_g * (vl - vr) / (sqrmin + sqrmax));

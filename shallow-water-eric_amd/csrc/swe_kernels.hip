@@ -194,7 +194,7 @@ __device__ __forceinline__ void add_sources(int src, const Node& I, double mi, d
     if constexpr (!HAS_SRC) return;
     if (src == SRC_MANNING) {
         const double hs = pow43(fmaxF(I.h, K.htiny));
-        const double vel = K.fric * sqrt(v1 * v1 + v2 * v2);
+        const double vel = K.fric * sqrt_rn(v1 * v1 + v2 * v2);
         const double hloc = 1.0 * vel * dt;
         const double den = hs + fmaxF(hs, hloc);
         r2 = r2 - mi * I.qx * vel * 2.0 / den;
@@ -305,7 +305,7 @@ __device__ __forceinline__ void dt_slice(const Mesh& M, const InstDev* __restric
     if (valid) {
         I = U[i];
         lw[lane] = ovh_of(I.h, K.htiny);
-        lw[64 + lane] = sqrt(K.g * fabs(I.h));
+        lw[64 + lane] = sqrt_rn(K.g * fabs(I.h));
         lw[128 + 64 * M.dpos[i] + lane] = 0.0;
     }
     __syncthreads();
@@ -319,7 +319,7 @@ __device__ __forceinline__ void dt_slice(const Mesh& M, const InstDev* __restric
             const Node Ir = U[slice * 64 + r];
             const Node J = U[col_at(M, slot, slice * 64 + r)];
             const double d = dij_from_c(ld_entry(M.c + slot), Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
-                                        sqrt(K.g * fabs(J.h)), K, err);
+                                        sqrt_rn(K.g * fabs(J.h)), K, err);
             lw[128 + off] = d;
             if (d0) st_entry1(d0 + slot, d);
         }
@@ -439,7 +439,7 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
     int err = 0;
     const Node I = U[i];
     const double ovh_i = ovh_of(I.h, K.htiny);
-    const double sql = sqrt(K.g * fabs(I.h));
+    const double sql = sqrt_rn(K.g * fabs(I.h));
     double s = 0.0;
     int64_t pd = -1;
     const int64_t pb = row_base(M, i);
@@ -448,7 +448,7 @@ k_dij_full(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, do
         const int j = col_at(M, p, i);
         if (j == i) { pd = p; dij[p] = 0.0; continue; }
         const Node J = U[j];
-        const double d = dij_from_c(M.c[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt(K.g * fabs(J.h)), K, err);
+        const double d = dij_from_c(M.c[p], I, J, ovh_i, ovh_of(J.h, K.htiny), sql, sqrt_rn(K.g * fabs(J.h)), K, err);
         dij[p] = d;
         s = s + d;
     }
@@ -495,7 +495,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
     const double dt = T[inst].dt;
     int err = 0;
     const Node I = U[i];
-    const double sql = sqrt(K.g * fabs(I.h));
+    const double sql = sqrt_rn(K.g * fabs(I.h));
     const double mi = M.m[i];
     const double ovh_i = ovh_of(I.h, K.htiny);
     const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
@@ -519,7 +519,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
         const bool off = (j != i);
         if (SMB != SMB_GALERKIN && off)
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
-                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt_rn(K.g * fabs(J.h)), K, err),
                                    ai, alpha, j, d, mu);
         if constexpr (SMB == SMB_1) {
             // smb_1 (update.f90:378-409)
@@ -616,7 +616,7 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // pTilde (update.f90:515-518) at node n: paper_constant = lambdaSGN*g/(3*SQRT(lumped)),
 // x = (h*eta)/h**2; the literal integer powers are products from the left, as flang expands them.
 __device__ __forceinline__ double sgn_ptilde(double h, double he, double m, const Consts& K) {
-    const double pc = K.lambda_sgn * K.g / (3.0 * sqrt(m));
+    const double pc = K.lambda_sgn * K.g / (3.0 * sqrt_rn(m));
     const double x = he / (h * h);
     return pc * (h * h * h) * (2.0 + 4.0 * (x * x * x) - 6.0 * (x * x * x * x));
 }
@@ -639,7 +639,7 @@ k_stage_sgn(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
     int err = 0;
     const Node I = U[i];
     const NodeX IX = UX[i];
-    const double sql = sqrt(K.g * fabs(I.h));
+    const double sql = sqrt_rn(K.g * fabs(I.h));
     const double mi = M.m[i];
     const double ovh_i = ovh_of(I.h, K.htiny);
     const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
@@ -669,7 +669,7 @@ k_stage_sgn(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         const bool off = (j != i);
         if (SMB != SMB_GALERKIN && off)
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j,
-                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err),
+                                   d0 ? d0[p] : dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt_rn(K.g * fabs(J.h)), K, err),
                                    ai, alpha, j, d, mu);
         if constexpr (SMB == SMB_1) {
             for (int k = 0; k < 5; ++k) {
@@ -720,7 +720,7 @@ k_stage_sgn(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, c
         r[2] = r[2] - pt * c.y;
     }
     {
-        const double pc = K.lambda_sgn * K.g / (3.0 * sqrt(mi));
+        const double pc = K.lambda_sgn * K.g / (3.0 * sqrt_rn(mi));
         const double x = IX.he / (I.h * I.h);
         const double psi = 12.0 * (x - 1.0);
         const double t = IX.he / I.h;
@@ -778,7 +778,7 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
     const double dt = T[inst].dt;
     int err = 0;
     const Node I = U[i];
-    const double sql = sqrt(K.g * fabs(I.h));
+    const double sql = sqrt_rn(K.g * fabs(I.h));
     const double mi = M.m[i];
     const double ovh_i = ovh_of(I.h, K.htiny);
     const double v1i = I.qx * ovh_i, v2i = I.qy * ovh_i;
@@ -814,7 +814,7 @@ fct_lo_body(const Mesh& M, const InstDev* __restrict__ T, const Node* __restrict
             double dlam;
             if constexpr (LAM == LAM_D0) dlam = ld_entry1(d0 + p);
             else if constexpr (LAM == LAM_LDS) dlam = lam[64 * q + lane_of()];
-            else dlam = dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt(K.g * fabs(J.h)), K, err);
+            else dlam = dij_from_c(c, I, J, ovh_i, ovh_j, sql, sqrt_rn(K.g * fabs(J.h)), K, err);
             first_order_dmu<ALPHA>(c, v1i, v2i, v1j, v2j, dlam, ai, alpha, j, d, mu);
             st_entry(dmu + p, make_double2(d, mu));
         }
@@ -984,7 +984,7 @@ k_fct_lo_lds(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
         if (M.len[i] > 0) {
             const double h = U[i].h;
             lw[lane] = ovh_of(h, T[inst].K.htiny);
-            lw[64 + lane] = sqrt(T[inst].K.g * fabs(h));
+            lw[64 + lane] = sqrt_rn(T[inst].K.g * fabs(h));
         }
     }
     __syncthreads();
@@ -1000,7 +1000,7 @@ k_fct_lo_lds(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, 
             const Node Ir = U[slice * 64 + r];
             const Node J = U[col_at(M, slot, slice * 64 + r)];
             lw[128 + off] = dij_from_c(ld_entry(M.c + slot), Ir, J, lw[r], ovh_of(J.h, K.htiny), lw[64 + r],
-                                       sqrt(K.g * fabs(J.h)), K, err);
+                                       sqrt_rn(K.g * fabs(J.h)), K, err);
         }
         raise_flag(flags_out, inst, err);
     }
