@@ -22,6 +22,7 @@ REF_DRIVER_HOIST = os.path.join(HERE, "_ref", "ref_driver_hoist")
 
 GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
 VISC_NAMES = {GALERKIN: "galerkin", VISCOUS: "viscous", ENTROPY_VISC: "entropy_visc", FCT: "fct"}
+ORC_OK, ENEG_H, ELAMBDA, EBADOPT = 0, 1, 2, 3   # orc_run status (swe_oracle.h)
 
 
 class OrcParams(C.Structure):

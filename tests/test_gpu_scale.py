@@ -133,3 +133,53 @@ def test_c5_served_stream_equals_requests_alone(oracle):
         ref, t, err = oracle_run(oracle, p, r.steps)
         assert err == 0 and t == r.time
         assert agrees(r.state, ref, p.params["mannings"] > 0), (r.rid, p.name)
+
+
+def test_c4_island_members_stop_where_the_reference_stops(oracle):
+    """The C4 members whose bumps emerge as dry islands (a quarter of the family), 1000 steps in one
+    batch: the instances that the reference would stop with 'Min h<0' (check_hmin) fail at a step
+    of their own while the others go on; each failed instance reports the same step, time and error
+    when run alone, and for the smallest of them the oracle runs without error up to the step before
+    (the same bits) and reports the negative depth at that step."""
+    from swe_amd import problems
+    from swe_amd.batch import INSTANCE_DONE, INSTANCE_FAILED, Batch
+    from swe_amd.solver import SWE_ENEG_H
+    specs = [s for s in problems.batch_family_specs(256, 5000, 500000) if s["zmax"] is None]
+    ps = problems.build_family(specs, workers=16)
+    n = 1000
+    print("C4 islands: %d meshes, %d nodes" % (len(ps), sum(p.np for p in ps)), flush=True)
+    rows = sum((p.np + 63) // 64 * 64 for p in ps)
+    entries = int(sum(p.nnz for p in ps) * 1.35)
+    bnd = sum(sum(x.size for x in p.dirichlet()) for p in ps) + 64
+    with Batch(rows, entries, bnd, len(ps)) as b:
+        ids = [b.add_problem(p, max_steps=n, device_setup=True) for p in ps]
+        b.step(n + 1)
+        stat = {k: b.status(k) for k in ids}
+    failed = [(p, stat[k]) for k, p in zip(ids, ps) if stat[k][0] == INSTANCE_FAILED]
+    print("C4 islands: %d failed at steps %s" % (len(failed), sorted(st[2] for _, st in failed)), flush=True)
+    assert all(stat[k][0] in (INSTANCE_DONE, INSTANCE_FAILED) for k in ids)
+    assert all(stat[k][2] == n for k in ids if stat[k][0] == INSTANCE_DONE)
+    assert failed and all(st[3] == SWE_ENEG_H and st[2] <= n for _, st in failed)
+
+    def alone(p, steps):
+        with Batch((p.np + 63) // 64 * 64, int(p.nnz * 1.4) + 4096, sum(x.size for x in p.dirichlet()) + 64, 1) as b1:
+            k = b1.add_problem(p, max_steps=steps, device_setup=True)
+            b1.step(steps + 1)
+            return b1.status(k), b1.get_state(k)
+    for p, st in failed:
+        assert alone(p, n)[0] == st, p.name
+    p, st = min(failed, key=lambda x: x[0].np)
+    before, u = alone(p, st[2] - 1)
+    assert before[0] == INSTANCE_DONE and before[2] == st[2] - 1
+    print("C4 islands: oracle on %s (%d nodes), %d steps" % (p.name, p.np, st[2]), flush=True)
+    hd, uxd, uyd = p.dirichlet()
+    kind, amp, period, level = p.bc
+    orc = oracle_of(oracle, p)
+    try:
+        kw = dict(bc_kind=kind, amp=amp, period=period, level=level, nodes_h=hd, nodes_ux=uxd, nodes_uy=uyd)
+        ref, _, t, err = orc.run(p.u0, st[2] - 1, t0=0.0, **kw)
+        assert err == 0 and t == before[1] and same_bits(u, ref)
+        _, _, _, err = orc.run(ref, 1, t0=t, **kw)
+        assert err == oracle.ENEG_H
+    finally:
+        orc.close()
