@@ -131,9 +131,10 @@ int swe_create_from_assembly(const swe_assembly *a, const double *bath, int n_h_
 /* compute_errors (SOURCES/main.f90:198-263) on the device state of a solver: sol_anal
  * (SOURCES/condlim.f90:311-831) at the Gauss points of r_gauss (main.f90:303-322) and the sums of
  * ns_anal_l1 / ns_anal_0 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1679-1716, 1513-1553) for the analytic
- * tests 3 to 7, 11 and 12 (the solver's type_test; others give SWE_EBADOPT; for the steady bump flows
- * 3, 6 and 7 the device's acos, cos and pow may differ from the C library's in the last bit of a
- * summand).  swe_errors_create takes
+ * tests 3 to 7, 11, 12 and 13 (the solver's type_test; others give SWE_EBADOPT; test 13 takes h, qx,
+ * qy of its five components; for the steady bump flows 3, 6 and 7 the device's acos, cos and pow, and
+ * for test 13 its cosh, may differ from the C library's in the last bit of a summand).
+ * swe_errors_create takes
  * the element list jj(3,me) (1-based, the solver's node numbering) and rr(2,np); the handle must
  * outlive the evaluator.  swe_compute_errors writes out[12]: for k = h, qx, qy in turn the L1 error
  * of u(k,:) - exact, its L1 norm (0 - exact), the L2 error and the L2 norm.  Each sum is accumulated

@@ -1352,6 +1352,7 @@ int orc_compute_errors(int np, int me, const int *jj, const double *rr, const do
     int ok;
     orc_sol_anal(type_test, 1, 0.0, 0.0, t, gravity, htiny, &ok);
     if (!ok) return ORC_EBADOPT;
+    const int nc = (type_test == 13) ? 5 : 3;   /* un(syst_size,np): h, qx, qy are its first rows */
     /* the 3-point rule of element_2d (gauss_points_2d_p1.f90:347-393) and its geometry (106-132) */
     const double one = 1, two = 2, three = 3, six = 6;
     const double xx[3] = {one / six, two / three, one / six};
@@ -1389,7 +1390,7 @@ int orc_compute_errors(int np, int me, const int *jj, const double *rr, const do
                 const double fa = orc_sol_anal(type_test, k + 1, rg[0], rg[1], t, gravity, htiny, &ok);
                 double fl = 0.0, f0 = 0.0;
                 for (int n = 0; n < 3; n++) {
-                    fl = fl + u[3 * nd[n] + k] * ww[n][l];
+                    fl = fl + u[nc * nd[n] + k] * ww[n][l];
                     f0 = f0 + 0.0 * ww[n][l];
                 }
                 const double d = fl - fa, d0 = f0 - fa;

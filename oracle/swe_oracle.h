@@ -81,7 +81,8 @@ double orc_sin_rn(double x);
  * the only times the reference defines them); *ok = 0 for other cases. */
 double orc_sol_anal(int type_test, int k, double x, double y, double t, double gravity, double htiny, int *ok);
 
-/* The quadrature norms of compute_errors (SOURCES/main.f90:198-263): r_gauss (303-322),
+/* The quadrature norms of compute_errors (SOURCES/main.f90:198-263, tests 3-7, 11, 12 and 13; u holds
+ * 5 components per node for test 13, of which h, qx, qy enter): r_gauss (303-322),
  * sol_anal at the Gauss points, ns_anal_l1 and ns_anal_0 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:
  * 1679-1716, 1513-1553) of u(k,:) - exact and of 0 - exact, sequential sums in element and Gauss
  * point order.  jj is 0-based jj[3*m+n]; out[4*(k-1) + {0,1,2,3}] = L1 error, L1 norm, L2 error,

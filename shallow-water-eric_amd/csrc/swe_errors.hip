@@ -1,5 +1,5 @@
 // swe_errors.hip -- compute_errors (SOURCES/main.f90:198-263) on the device state: sol_anal
-// (SOURCES/condlim.f90:311-831, analytic tests 3 to 7, 11 and 12) at the Gauss points of r_gauss
+// (SOURCES/condlim.f90:311-831, analytic tests 3 to 7, 11, 12 and 13) at the Gauss points of r_gauss
 // (main.f90:303-322), and the quadrature sums of ns_anal_l1 / ns_anal_0
 // (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1679-1716, 1513-1553) of u(k,:) - exact and 0 - exact.
 //
@@ -36,6 +36,8 @@ struct Anal {
     double cT1 = 0.0, cT2 = 0.0, T3 = 0.0, delta = 0.0, eta = 0.0, w = 0.0;       // test 12 (701-750)
     // the steady flows over the bump, tests 3 (346-384), 6 (457-521) and 7 (522-605)
     double q0 = 0.0, b0 = 0.0, d = 0.0, ber = 0.0, fpio3 = 0.0, xshock = 0.0, bath_shock = 0.0, h_post = 0.0;
+    // test 13, the SGN model's solitary wave (751-827): h1, h2, crest x0, celerity D, z and D*t
+    double h1 = 0.0, h2 = 0.0, zz = 0.0, Dw = 0.0, Dt = 0.0;
 };
 
 // The bump of tests 2, 3, 6, 7: MAX(0, 0.2*(x-8)**3*(12-x)**3/64) (condlim.f90:355).
@@ -142,16 +144,35 @@ int make_anal(int test, double t, double g, double htiny, Anal& A) {
         A.eta = eta;
         return SWE_OK;
     }
-    return fail(SWE_EBADOPT, "compute_errors: no analytic solution for type_test %d (tests 3 to 7, 11 and 12 have one)",
-                test);
+    if (test == 13) {   // the case sets gravity to 9.81; h, qx, qy are defined at every t
+        const double gg = 9.81;
+        A.h1 = 10.0 / 100.0;
+        A.h2 = 11.0 / 100.0;
+        A.x0 = 2.75;
+        A.Dw = std::sqrt(gg * A.h2);
+        A.zz = std::sqrt((3.0 * (A.h2 - A.h1)) / (A.h2 * (A.h1 * A.h1)));
+        A.Dt = A.Dw * t;
+        return SWE_OK;
+    }
+    return fail(SWE_EBADOPT,
+                "compute_errors: no analytic solution for type_test %d (tests 3 to 7, 11, 12 and 13 have one)", test);
 }
 
 __device__ __forceinline__ double fmaxF(double a, double b) { return (a > b) ? a : b; }
 
 // sol_anal(1..3) at one point, given the t-dependent factors.  The steady bump flows take the roots
-// of a cubic per point with the device's acos, cos and pow, which may differ from the host C
-// library's in the last bit.
+// of a cubic per point with the device's acos, cos and pow, and the SGN wave the device's cosh, which
+// may differ from the host C library's in the last bit.
 __device__ __forceinline__ void sol_anal3(const Anal& A, double x, double y, double v[3]) {
+    if (A.test == 13) {
+        // h1 + (h2-h1)*(1/COSH(z/2*(x-x0-D*t)))**2, h*D - h1*D, h*0 (condlim.f90:759-798)
+        const double sech = 1.0 / cosh(1.0 / 2.0 * A.zz * (x - A.x0 - A.Dt));
+        const double vv = fmaxF(A.h1 + (A.h2 - A.h1) * (sech * sech), 0.0);
+        v[0] = vv;
+        v[1] = vv * A.Dw - A.h1 * A.Dw;
+        v[2] = vv * 0.0;
+        return;
+    }
     if (A.test == 3 || A.test == 6 || A.test == 7) {
         v[2] = 0.0;
         const double bathi = bump_of(x);

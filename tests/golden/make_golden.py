@@ -386,6 +386,16 @@ def gen_errors():
         assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
         out["%s|final|time" % name] = errs(case, z["final"], float(z["time"]))
         case.close()
+    # the SGN model (test 13: h, qx, qy of its five components) on the canal of sgn_run.npz, at the
+    # initial state and after the 200 steps
+    z = np.load(os.path.join(HERE, "sgn_run.npz"))
+    case = RefCase(fem_path=os.path.join(REF, "MESHES", "SGN_CANAL", "rectangle.3.FEM"), test=13, viscosity="viscous",
+                   alpha=True, cfl=0.08, h_sides=(1, 3), lambda_sgn=9.0)
+    S = case.setup()
+    assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
+    out["sgn_run|states0|0.0"] = errs(case, z["states"][0], 0.0)
+    out["sgn_run|final|time"] = errs(case, z["final"], float(z["time"]))
+    case.close()
     save("errors.npz", **out)
     # the steady flows over a bump (tests 3, 6, 7) on a strip of [0,25]x[0,1], at their initial states,
     # at t = 0 and at t = 1 (the analytic steady state)

@@ -13,7 +13,11 @@ from conftest import golden
 pytestmark = pytest.mark.gpu
 
 CASES = {"par41_modes": ("viscous", False, 5), "cav025_run": ("fct", True, 4),
-         "cdisk_coriolis": ("viscous", True, 12), "par_friction": ("viscous", True, 11)}
+         "cdisk_coriolis": ("viscous", True, 12), "par_friction": ("viscous", True, 11),
+         "sgn_run": ("viscous", True, 13)}
+# sol_anal through a device libm function (cosh for the SGN wave of test 13): a summand may differ
+# from the host's in its last bits, so the sums are compared to the reference's within 1e-12
+DEVICE_LIBM = {13}
 
 
 def _solver(z, viscosity, alpha, test):
@@ -56,7 +60,12 @@ def test_device_errors_match_the_reference(oracle, key):
     g, htiny = z["scalars"][0], z["scalars"][1]
     out, terms = oracle.compute_errors(z["jj"], z["rr"], u, int(z["type_test"]), t, g, htiny, terms=True)
     assert np.array_equal(dev, again)
-    _check(dev, out, terms, golden("errors.npz")[key])
+    ref = golden("errors.npz")[key]
+    if int(z["type_test"]) in DEVICE_LIBM:
+        scale = np.maximum(np.abs(ref), 1e-300)
+        assert np.all(np.abs(dev - ref) <= 1e-12 * scale) and np.all(np.abs(dev - out) <= 1e-12 * scale)
+    else:
+        _check(dev, out, terms, ref)
 
 
 def test_device_errors_after_a_run_at_scale(oracle):
