@@ -130,3 +130,28 @@ def test_bumps_unstructured_16m_200_steps_properties(bumps_unstructured):
     assert np.all(np.isfinite(u)) and u[:, 0].min() >= 0.0 and np.all(dts > 0)
     u2, t2, dts2 = _run(p, 200)
     assert same_bits(u, u2) and same_bits(dts, dts2) and t == t2
+
+
+def test_closed_basin_16m_keeps_its_volume(bumps_unstructured):
+    """The 16.7M-node Poisson-disk mesh closed by walls on all four sides, the bumps scaled to stay
+    below 0.7 and a Gaussian hump of 0.05 on the free surface of level 1 (wet everywhere): 200 fct steps
+    keep sum_i m_i h_i to 1e-14 relative (the reference's fct step conserves the volume of a closed wet
+    domain; its first-order step does not, see test_mass_balance.py), with depth >= 0 and finite."""
+    import dataclasses
+    import math
+    from swe_amd import problems
+    q = bumps_unstructured
+    z = q.bath * (0.7 / q.bath.max())
+    rr = q.mesh.rr
+    eta = 1.0 + 0.05 * np.exp(-((rr[:, 0] - 400.0) ** 2 + (rr[:, 1] - 600.0) ** 2) / (2.0 * 50.0 ** 2))
+    u0 = np.zeros((q.np, 3))
+    u0[:, 0] = eta - z
+    params = dict(q.params, alpha_limit=False)
+    p = dataclasses.replace(q, name="closed_basin", bath=z, u0=u0, params=params, h_sides=(), ux_sides=(2, 4),
+                            uy_sides=(1, 3), bc=(problems.BC_WALLS, 0.0, 1.0, 0.0))
+    v0 = math.fsum(p.lumped * u0[:, 0])
+    u, t, dts = _run(p, 200)
+    v = math.fsum(p.lumped * u[:, 0])
+    print("closed basin 16.7M, 200 fct steps: t = %.6g, relative volume change %.3g" % (t, (v - v0) / v0), flush=True)
+    assert np.all(np.isfinite(u)) and u[:, 0].min() >= 0.0 and np.all(dts > 0)
+    assert abs(v - v0) <= 1e-14 * v0
