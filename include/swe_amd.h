@@ -144,6 +144,12 @@ int swe_create_from_assembly(const swe_assembly *a, const double *bath, int n_h_
 typedef struct swe_errors swe_errors;
 int swe_errors_create(swe_handle *h, int me, const int *jj, const double *rr, swe_errors **out);
 int swe_compute_errors(swe_errors *e, double time, double *out);
+/* The diagnostic the time loop writes to unit 10 after every step of tests 1, 2 and 10
+ * (SOURCES/main.f90:94-99, replacing its three host calls of ns_l1, SFEMaNS/FEMSUB_OBJET/fem_tn.f90:
+ * 2119-2153, on the device state): out[4] = {q1, q2, q3, 0.5*(q1+q2)/(gravity*q3)}, the L1 norms
+ * (3-point P1 quadrature of |u_h|) of qx, qy and h*sqrt(h).  Any type_test; the sums are accumulated
+ * as swe_compute_errors' are (exactly rounded, not sequential). */
+int swe_ns_l1_ratio(swe_errors *e, double *out);
 int swe_errors_destroy(swe_errors *e);
 int swe_destroy(swe_handle *h);
 int swe_set_params(swe_handle *h, const swe_params *prm);

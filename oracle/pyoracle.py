@@ -69,6 +69,8 @@ def lib():
         L.orc_compute_errors.restype = C.c_int
         L.orc_compute_errors.argtypes = [C.c_int, C.c_int, P, P, P, C.c_int, C.c_double, C.c_double, C.c_double,
                                          P, P]
+        L.orc_ns_l1_ratio.restype = None
+        L.orc_ns_l1_ratio.argtypes = [C.c_int, C.c_int, P, P, P, C.c_int, C.c_double, P, P]
         L.orc_sol_anal.restype = C.c_double
         L.orc_sol_anal.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.POINTER(C.c_int)]
@@ -132,6 +134,21 @@ def compute_errors(jj0, rr, u, type_test, t, gravity, htiny, terms=False):
         raise ValueError("compute_errors: no analytic solution for test %d" % type_test)
     out = out.reshape(3, 4)
     return (out, tq.reshape(3, 4, 3 * me)) if terms else out
+
+
+def ns_l1_ratio(jj0, rr, u, gravity, terms=False):
+    """The time loop's unit-10 diagnostic (main.f90:94-99): [q1, q2, q3, 0.5*(q1+q2)/(g*q3)] with the
+    ns_l1 norms of qx, qy and h*sqrt(h); u (np, nc) with h, qx, qy first; with terms=True also every
+    summand, (3, 3*me)."""
+    jj0 = np.ascontiguousarray(jj0, dtype=np.int32)
+    rr = np.ascontiguousarray(rr, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    me = jj0.shape[0]
+    out = np.zeros(4)
+    tq = np.zeros(3 * 3 * me) if terms else None
+    lib().orc_ns_l1_ratio(rr.shape[0], me, _p(jj0), _p(rr), _p(u), int(u.shape[1]), float(gravity), _p(out),
+                          _p(tq) if terms else None)
+    return (out, tq.reshape(3, 3 * me)) if terms else out
 
 
 def sol_anal(type_test, k, x, y, t, gravity, htiny):

@@ -28,6 +28,10 @@
 !       Gauss points, ns_anal_l1 and ns_anal_0 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1513-1553,
 !       1679-1716) of U(k,:) - exact and of 0 - exact; writes 12 reals (k-major: L1 error, L1 norm,
 !       L2 error, L2 norm).
+!   ref_driver flow <data> <init|none> <state> <out>
+!       the diagnostic the time loop writes to unit 10 for tests 1, 2 and 10 (SOURCES/main.f90:94-99):
+!       ns_l1 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:2119-2153) of U(2,:), U(3,:) and U(1,:)*SQRT(U(1,:)),
+!       and 0.5*(q1+q2)/(gravity*q3); writes those 4 reals.
 !
 ! <init> is an optional custom initial-data file for problems that have no init case in the
 ! reference (see custom_init below); 'none' calls the reference init (SOURCES/condlim.f90:92).
@@ -72,6 +76,8 @@ PROGRAM ref_driver
      CALL do_run
   CASE ('errors')
      CALL do_errors
+  CASE ('flow')
+     CALL do_flow
   CASE DEFAULT
      WRITE(*,*) 'ref_driver: unknown command ', TRIM(cmd)
      STOP 2
@@ -372,6 +378,23 @@ CONTAINS
     END DO
     CALL write_real(arg(6), out)
   END SUBROUTINE do_errors
+
+  !---------------------------------------------------------------------------------------------
+  ! The unit-10 diagnostic of the time loop (SOURCES/main.f90:94-99), the same calls and expression.
+  SUBROUTINE do_flow
+    REAL(KIND=8) :: q1, q2, q3
+    REAL(KIND=8), DIMENSION(4) :: out
+    CALL prepare(arg(2), arg(3))
+    CALL read_state(arg(4), un)
+    CALL ns_l1(mesh, un(2,:), q1)
+    CALL ns_l1(mesh, un(3,:), q2)
+    CALL ns_l1(mesh, un(1,:)*SQRT(un(1,:)), q3)
+    out(1) = q1
+    out(2) = q2
+    out(3) = q3
+    out(4) = 0.5d0*(q1+q2)/(inputs%gravity*q3)
+    CALL write_real(arg(5), out)
+  END SUBROUTINE do_flow
 
   !---------------------------------------------------------------------------------------------
   ! Fixed-step form of the SSP-RK3 loop of SOURCES/main.f90:61-92.

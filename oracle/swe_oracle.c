@@ -1410,3 +1410,54 @@ int orc_compute_errors(int np, int me, const int *jj, const double *rr, const do
     }
     return ORC_OK;
 }
+
+/* The unit-10 diagnostic of the time loop (main.f90:94-99): ns_l1 (fem_tn.f90:2119-2153) of qx, qy and
+ * the nodal h*SQRT(h), with the element, Gauss point and node loops of ns_l1 and the 3-point rule of
+ * element_2d (gauss_points_2d_p1.f90:347-393, 106-132), then 0.5d0*(q1+q2)/(gravity*q3). */
+void orc_ns_l1_ratio(int np, int me, const int *jj, const double *rr, const double *u, int nc, double gravity,
+                     double *out, double *terms)
+{
+    const double one = 1, two = 2, three = 3, six = 6;
+    const double xx[3] = {one / six, two / three, one / six};
+    const double yy[3] = {one / six, one / six, two / three};
+    const double dd[2][3] = {{-one, one, 0.0}, {-one, 0.0, one}};
+    const double pp = one / six;
+    double ww[3][3];
+    for (int l = 0; l < 3; l++) {
+        ww[0][l] = one - xx[l] - yy[l];
+        ww[1][l] = xx[l];
+        ww[2][l] = yy[l];
+    }
+    double *h32 = (double *)xmalloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+    for (int i = 0; i < np; i++) h32[i] = u[(size_t)nc * i] * sqrt(u[(size_t)nc * i]);
+    const long nq = 3L * me;
+    double q[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < me; m++) {
+        const int *nd = jj + 3 * m;
+        for (int l = 0; l < 3; l++) {
+            double dr[2][2];
+            for (int k = 0; k < 2; k++)
+                for (int k1 = 0; k1 < 2; k1++) {
+                    double s = 0.0;
+                    for (int n = 0; n < 3; n++) s = s + rr[2 * nd[n] + k] * dd[k1][n];
+                    dr[k][k1] = s;
+                }
+            const double rj = (dr[0][0] * dr[1][1] - dr[0][1] * dr[1][0]) * pp;
+            for (int f = 0; f < 3; f++) {
+                double fl = 0.0;
+                for (int n = 0; n < 3; n++) {
+                    const double v = (f == 2) ? h32[nd[n]] : u[(size_t)nc * nd[n] + 1 + f];
+                    fl = fl + v * ww[n][l];
+                }
+                const double tq = fabs(fl) * rj;
+                q[f] = q[f] + tq;
+                if (terms) terms[f * nq + 3L * m + l] = tq;
+            }
+        }
+    }
+    free(h32);
+    out[0] = q[0];
+    out[1] = q[1];
+    out[2] = q[2];
+    out[3] = 0.5 * (q[0] + q[1]) / (gravity * q[2]);
+}

@@ -91,6 +91,14 @@ double orc_sol_anal(int type_test, int k, double x, double y, double t, double g
 int orc_compute_errors(int np, int me, const int *jj, const double *rr, const double *u, int type_test,
                        double t, double gravity, double htiny, double *out, double *terms);
 
+/* The diagnostic the time loop writes to unit 10 for tests 1, 2 and 10 (SOURCES/main.f90:94-99):
+ * ns_l1 (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:2119-2153), the sum over elements and Gauss points of
+ * |sum_n f(jj(n,m)) ww(n,l)| rj(l,m), of f = qx, qy and h*sqrt(h), sequential in element and Gauss
+ * point order; out = {q1, q2, q3, 0.5*(q1+q2)/(gravity*q3)}.  u holds nc components per node (h, qx,
+ * qy first).  terms (may be NULL) receives every summand: terms[f*3*me + 3*m + l]. */
+void orc_ns_l1_ratio(int np, int me, const int *jj, const double *rr, const double *u, int nc, double gravity,
+                     double *out, double *terms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -292,6 +292,18 @@ int swe_compute_errors(swe_errors* e, double time, double* out) {
     return errors_eval(P, e->E, P->S[0], I.prm.type_test, time, I.prm.gravity, I.prm.htiny, out);
 }
 
+int swe_ns_l1_ratio(swe_errors* e, double* out) {
+    if (!e || !out) return fail(SWE_EINVAL, "null argument");
+    if (swe_handle* h = e->h) {
+        if (!h->state_set) return fail(SWE_EINVAL, "no state has been set");
+        if (h->step_open) return fail(SWE_EINVAL, "a step is open (swe_step_begin without swe_step_finish)");
+    }
+    Pool* P = e->P;
+    const InstHost& I = P->inst[(size_t)e->k];
+    if (!I.used || I.since != e->since) return fail(SWE_EINVAL, "instance %d is no longer the evaluator's mesh", e->k);
+    return errors_flow(P, e->E, P->S[0], I.prm.gravity, out);
+}
+
 int swe_errors_destroy(swe_errors* e) {
     if (!e) return SWE_OK;
     errors_destroy(e->E);

@@ -251,6 +251,7 @@ struct ErrEval;
 int errors_create(Pool* P, int k, int me, const int* jj1, const double* rr, ErrEval** out);
 int errors_eval(Pool* P, const ErrEval* E, const Node* U, int type_test, double t, double g, double htiny,
                 double out[12]);
+int errors_flow(Pool* P, const ErrEval* E, const Node* U, double g, double out[4]);
 void errors_destroy(ErrEval* E);
 
 int pool_add_assembled(Pool* P, const swe_assembly* a, const DeviceLayout& DL, const double* bath,

@@ -1,7 +1,9 @@
 // swe_errors.hip -- compute_errors (SOURCES/main.f90:198-263) on the device state: sol_anal
 // (SOURCES/condlim.f90:311-831, analytic tests 3 to 7, 11, 12 and 13) at the Gauss points of r_gauss
 // (main.f90:303-322), and the quadrature sums of ns_anal_l1 / ns_anal_0
-// (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1679-1716, 1513-1553) of u(k,:) - exact and 0 - exact.
+// (SFEMaNS/FEMSUB_OBJET/fem_tn.f90:1679-1716, 1513-1553) of u(k,:) - exact and 0 - exact; and the
+// time loop's unit-10 diagnostic of tests 1, 2, 10 (main.f90:94-99), ns_l1 (fem_tn.f90:2119-2153) of qx,
+// qy and h*sqrt(h) with the ratio 0.5*(q1+q2)/(g*q3).
 //
 // Every summand is computed with the reference's expressions (the file is compiled without FMA
 // contraction; the factors of sol_anal that depend on t only are evaluated once on the host with the
@@ -375,6 +377,77 @@ __global__ void __launch_bounds__(64) k_errors_final(const DD* __restrict__ part
     out[q] = ((q & 3) >= 2) ? sqrt(v) : v;
 }
 
+constexpr int kFlowSums = 3;   // ns_l1 of qx, qy and h*sqrt(h)
+
+// The time loop's unit-10 diagnostic (main.f90:94-99): ns_l1 (fem_tn.f90:2119-2153) of qx, qy and the
+// nodal h*SQRT(h), the sum over elements and Gauss points of |sum_n f(jj(n,m)) ww(n,l)| rj(l,m).  The
+// same partition and reductions as k_errors, in double-double.
+__global__ void __launch_bounds__(kErrBlock) k_flow(ErrArgs a, DD* __restrict__ part) {
+    const swe_p1::P1Rule R;
+    DD acc[kFlowSums];
+#pragma unroll
+    for (int q = 0; q < kFlowSums; ++q) acc[q] = DD{0.0, 0.0};
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < a.me; m += nthr) {
+        int nd[3];
+        double f[3][3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) nd[n] = a.jj[3 * m + n];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const Node u = a.U[a.row[nd[n]]];
+            f[0][n] = u.qx;
+            f[1][n] = u.qy;
+            f[2][n] = u.h * sqrt(u.h);
+        }
+        double dr[2][2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int k1 = 0; k1 < 2; ++k1) {
+                double s = 0.0;
+#pragma unroll
+                for (int n = 0; n < 3; ++n) s = s + a.rr[2 * (size_t)nd[n] + k] * R.dd[k1][n];
+                dr[k][k1] = s;
+            }
+        const double rj = (dr[0][0] * dr[1][1] - dr[0][1] * dr[1][0]) * R.pp;
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int k = 0; k < kFlowSums; ++k) {
+                double fl = 0.0;
+#pragma unroll
+                for (int n = 0; n < 3; ++n) fl = fl + f[k][n] * R.ww[n][l];
+                acc[k] = dd_add1(acc[k], fabs(fl) * rj);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < kFlowSums; ++q)
+        for (int off = 32; off > 0; off >>= 1) {
+            const DD o{__shfl_xor(acc[q].hi, off, 64), __shfl_xor(acc[q].lo, off, 64)};
+            acc[q] = dd_add(acc[q], o);
+        }
+    __shared__ DD wsum[kErrBlock / 64][kFlowSums];
+    const int wave = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < kFlowSums; ++q) wsum[wave][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < kFlowSums) {
+        DD s = wsum[0][threadIdx.x];
+        for (int w = 1; w < kErrBlock / 64; ++w) s = dd_add(s, wsum[w][threadIdx.x]);
+        part[(size_t)blockIdx.x * kFlowSums + threadIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_flow_final(const DD* __restrict__ part, int nblk, double* __restrict__ out) {
+    const int q = (int)threadIdx.x;
+    if (q >= kFlowSums) return;
+    DD s{0.0, 0.0};
+    for (int b = 0; b < nblk; ++b) s = dd_add(s, part[(size_t)b * kFlowSums + q]);
+    out[q] = s.hi + s.lo;
+}
+
 __global__ void k_node_rows(const int* __restrict__ perm, int64_t r0, int np, int* __restrict__ row) {
     const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (r < np) row[perm[r0 + r]] = (int)(r0 + r);
@@ -448,6 +521,18 @@ int errors_eval(Pool* P, const ErrEval* E, const Node* U, int type_test, double 
     SWE_HIPCHK(hipGetLastError());
     SWE_HIPCHK(hipMemcpyAsync(out, E->out, sizeof(double) * kSums, hipMemcpyDeviceToHost, P->stream));
     SWE_HIPCHK(hipStreamSynchronize(P->stream));
+    return SWE_OK;
+}
+
+int errors_flow(Pool* P, const ErrEval* E, const Node* U, double g, double out[4]) {
+    const int nblk = (int)std::min<int64_t>(kErrGrid, std::max<int64_t>(1, (E->me + kErrBlock - 1) / kErrBlock));
+    const ErrArgs a{E->jj, E->rr, E->row, U, E->me};
+    hipLaunchKernelGGL(k_flow, dim3((unsigned)nblk), dim3(kErrBlock), 0, P->stream, a, E->part);
+    hipLaunchKernelGGL(k_flow_final, dim3(1), dim3(64), 0, P->stream, (const DD*)E->part, nblk, E->out);
+    SWE_HIPCHK(hipGetLastError());
+    SWE_HIPCHK(hipMemcpyAsync(out, E->out, sizeof(double) * kFlowSums, hipMemcpyDeviceToHost, P->stream));
+    SWE_HIPCHK(hipStreamSynchronize(P->stream));
+    out[3] = 0.5 * (out[0] + out[1]) / (g * out[2]);   // main.f90:98
     return SWE_OK;
 }
 

@@ -109,6 +109,14 @@ MODULE swe_amd_c
        INTEGER(c_int) :: rc
      END FUNCTION swe_compute_errors
 
+     ! the unit-10 diagnostic of tests 1, 2, 10 (main.f90:94-99): q1, q2, q3 and 0.5*(q1+q2)/(g*q3)
+     FUNCTION swe_ns_l1_ratio(errors, out) BIND(C, NAME='swe_ns_l1_ratio') RESULT(rc)
+       IMPORT :: c_int, c_double, c_ptr
+       TYPE(c_ptr), VALUE :: errors
+       REAL(c_double), INTENT(OUT) :: out(4)
+       INTEGER(c_int) :: rc
+     END FUNCTION swe_ns_l1_ratio
+
      FUNCTION swe_errors_destroy(errors) BIND(C, NAME='swe_errors_destroy') RESULT(rc)
        IMPORT :: c_int, c_ptr
        TYPE(c_ptr), VALUE :: errors

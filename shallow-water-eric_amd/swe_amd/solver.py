@@ -67,6 +67,7 @@ SIGNATURES = {
     "swe_destroy": (_I, [_P]),
     "swe_errors_create": (_I, [_P, _I, _P, _P, C.POINTER(_P)]),
     "swe_compute_errors": (_I, [_P, C.c_double, _P]),
+    "swe_ns_l1_ratio": (_I, [_P, _P]),
     "swe_errors_destroy": (_I, [_P]),
     "swe_batch_errors_create": (_I, [_P, _I, _I, _P, _P, C.POINTER(_P)]),
     "swe_set_params": (_I, [_P, C.POINTER(Params)]),
@@ -514,6 +515,13 @@ class ErrorEvaluator:
             t = self.owner.status(self.k)[1]
         _check(load_library().swe_compute_errors(self.e, t, _ptr(out)))
         return out.reshape(3, 4)
+
+    def ns_l1_ratio(self):
+        """The time loop's unit-10 diagnostic of tests 1, 2, 10 (main.f90:94-99): [q1, q2, q3, ratio],
+        the L1 norms of qx, qy and h*sqrt(h) and 0.5*(q1+q2)/(g*q3)."""
+        out = np.zeros(4)
+        _check(load_library().swe_ns_l1_ratio(self.e, _ptr(out)))
+        return out
 
     def relative(self, time=None):
         """What compute_errors prints: the relative L1 and L2 errors on h (main.f90:214,221), and

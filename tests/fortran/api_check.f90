@@ -2,7 +2,8 @@
 ! api_check.f90 -- the device set-up and compute_errors entry points called from Fortran through
 ! the interface blocks of shallow-water-eric_amd/fortran/swe_amd_c.f90, the way a reference driver
 ! calls them: construct_matrices on the device (swe_assemble_device + swe_create_from_assembly),
-! the time loop (swe_run), then compute_errors (swe_errors_create + swe_compute_errors); then the same
+! the time loop (swe_run), then compute_errors (swe_errors_create + swe_compute_errors) and the time
+! loop's unit-10 diagnostic (swe_ns_l1_ratio); then the same
 ! mesh admitted twice into a batch (swe_batch_add_assembled), stepped, read back, and its errors.
 !
 ! Reads in.bin and writes out.bin in the current directory (raw little-endian streams):
@@ -11,7 +12,8 @@
 !        viscous_type, alpha_limit, roundoff_fix; int32 jj(3,me) (1-based); real8 rr(2,np),
 !        bath(np), un(3,np); int32 h_dir(nh), ux_dir(nux), uy_dir(nuy) (1-based)
 !   out: real8 un(3,np), time, dts(nsteps), errors(12), then for the two batch instances
-!        un(3,np) each, time each, the steps each (as real8) and the first instance's errors(12)
+!        un(3,np) each, time each, the steps each (as real8) and the first instance's errors(12);
+!        last the solver's ns_l1 diagnostic (4)
 !==============================================================================================
 PROGRAM api_check
   USE iso_c_binding
@@ -19,7 +21,7 @@ PROGRAM api_check
   IMPLICIT NONE
   INTEGER(c_int) :: np, me, nh, nux, nuy, nsteps, bc_kind, rc
   INTEGER(c_int) :: ints(5)
-  REAL(c_double) :: bcr(3), reals(5), t, e12(12)
+  REAL(c_double) :: bcr(3), reals(5), t, e12(12), f4(4)
   INTEGER(c_int), ALLOCATABLE :: jj(:,:), hd(:), uxd(:), uyd(:)
   REAL(c_double), ALLOCATABLE :: rr(:,:), bath(:), un(:,:), dts(:)
   TYPE(swe_params) :: prm
@@ -64,6 +66,7 @@ PROGRAM api_check
   CALL check(swe_get_state(h, un), 'swe_get_state')
   CALL check(swe_errors_create(h, me, jj, rr, errs), 'swe_errors_create')
   CALL check(swe_compute_errors(errs, t, e12), 'swe_compute_errors')
+  CALL check(swe_ns_l1_ratio(errs, f4), 'swe_ns_l1_ratio')
   CALL check(swe_errors_destroy(errs), 'swe_errors_destroy')
   CALL check(swe_destroy(h), 'swe_destroy')
   WRITE(*,*) ' Relative L1 error on h (Gaussian)', e12(1)/e12(2)
@@ -96,6 +99,7 @@ PROGRAM api_check
   OPEN(11, FILE='out.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE')
   WRITE(11) un, t, dts(1:nsteps), e12
   WRITE(11) ub, tb, REAL(nst, c_double), e12b
+  WRITE(11) f4
   CLOSE(11)
 
 CONTAINS

@@ -21,6 +21,8 @@ links the reference modules) and the outputs it wrote.  Cases:
   errors.npz          the quadrature norms of compute_errors (SOURCES/main.f90:198-263) on states of the
                       fixtures above (tests 4, 5, 11, 12), by the reference's sol_anal, ns_anal_l1, ns_anal_0
   errors_bump.npz     the same norms for the steady bump flows (tests 3, 6, 7) on a [0,25]x[0,1] strip
+  flow.npz            the time loop's unit-10 diagnostic (SOURCES/main.f90:94-99, ns_l1 of qx, qy, h*sqrt(h))
+                      on the dam-break states (test 10) and the SGN canal's final state
 """
 from __future__ import annotations
 
@@ -413,13 +415,47 @@ def gen_errors():
     save("errors_bump.npz", **bump)
 
 
+def gen_flow():
+    """The time loop's unit-10 diagnostic for tests 1, 2 and 10 (main.f90:94-99: ns_l1 of qx, qy and
+    h*sqrt(h), and 0.5*(q1+q2)/(g*q3)), 4 reals per key, on the states of the dam-break fixture (test
+    10, the case that writes it) and on the final state of the SGN canal (five components per node)."""
+    out = {}
+
+    def flow(case, u):
+        np.ascontiguousarray(u, dtype="<f8").tofile(os.path.join(case.d, "state.f8"))
+        O.run_ref(["flow", "data", case.init, "state.f8", "flow.f8"], cwd=case.d)
+        return np.fromfile(os.path.join(case.d, "flow.f8"))
+
+    z = np.load(os.path.join(HERE, "dambreak_run.npz"))
+    p = P.dam_break(101, 21, seed=7)
+    raw = M.structured_mesh(101, 21, 0.0, 100.0, 0.0, 20.0, jitter=0.2, seed=7)
+    case = RefCase(raw, test=10, viscosity="fct", alpha=True, cfl=p.params["cfl"], h_sides=p.h_sides,
+                   ux_sides=p.ux_sides, uy_sides=p.uy_sides, mannings=0.02,
+                   init=(p.bath, p.u0, p.params["gravity"], p.params["max_water_h"]), bc=p.bc)
+    S = case.setup()
+    assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
+    out["dambreak_run|u0"] = flow(case, z["u0"])
+    for k, st in enumerate(z["snap_steps"]):
+        out["dambreak_run|snap%d" % int(st)] = flow(case, z["snaps"][k])   # the last snapshot is the final state
+    case.close()
+    z = np.load(os.path.join(HERE, "sgn_run.npz"))
+    case = RefCase(fem_path=os.path.join(REF, "MESHES", "SGN_CANAL", "rectangle.3.FEM"), test=13, viscosity="viscous",
+                   alpha=True, cfl=0.08, h_sides=(1, 3), lambda_sgn=9.0)
+    S = case.setup()
+    assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
+    out["sgn_run|final"] = flow(case, z["final"])
+    case.close()
+    save("flow.npz", **out)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built: run `make -C oracle ref` first")
     only = set(sys.argv[1:])
     gens = [("lambda", gen_lambda), ("fct", gen_fct), ("par41", gen_par41_modes), ("par81", gen_par81_run),
             ("dambreak", gen_dambreak), ("dambreak9", gen_dambreak9), ("sgn", gen_sgn), ("bumps", gen_bumps), ("cav025", gen_cav025),
-            ("cdisk", gen_cdisk_coriolis), ("friction", gen_par_friction), ("errors", gen_errors)]
+            ("cdisk", gen_cdisk_coriolis), ("friction", gen_par_friction), ("errors", gen_errors),
+            ("flow", gen_flow)]
     for name, fn in gens:
         if not only or name in only:
             fn()

@@ -79,3 +79,51 @@ def test_oracle_bump_flow_errors_are_the_references(oracle, test):
     for t in (0, 1):
         out = oracle.compute_errors(z["jj"], z["rr"], z["u0_%d" % test], test, float(t), g, htiny)
         assert np.array_equal(out, z["errors_%d_t%d" % (test, t)]), t
+
+
+def _flow_case(key):
+    name, st = key.split("|")
+    z = golden(name + ".npz")
+    if st.startswith("snap"):
+        u = z["snaps"][list(z["snap_steps"]).index(int(st[4:]))]
+    else:
+        u = z[st]
+    return z, u
+
+
+@pytest.mark.parametrize("key", sorted(golden("flow.npz")))
+def test_oracle_flow_ratio_is_the_references(oracle, key):
+    """The time loop's unit-10 diagnostic of tests 1, 2, 10 (main.f90:94-99): the ns_l1 norms of qx, qy
+    and h*sqrt(h) and their ratio, bit for bit the reference's (tests/golden/flow.npz)."""
+    z, u = _flow_case(key)
+    out = oracle.ns_l1_ratio(z["jj"], z["rr"], u, float(z["scalars"][0]))
+    assert np.array_equal(out, golden("flow.npz")[key])
+
+
+@pytest.mark.parametrize("key", sorted(golden("flow.npz")))
+def test_oracle_flow_sums_within_the_summation_bound(oracle, key):
+    z, u = _flow_case(key)
+    out, terms = oracle.ns_l1_ratio(z["jj"], z["rr"], u, float(z["scalars"][0]), terms=True)
+    n = terms.shape[-1]
+    gam = (n - 1) * 2.0 ** -53 / (1 - (n - 1) * 2.0 ** -53)
+    for f in range(3):
+        assert np.all(terms[f] >= 0.0)
+        assert abs(out[f] - math.fsum(terms[f])) <= gam * terms[f].sum()
+
+
+def test_flow_ratio_of_uniform_fields(oracle):
+    """Closed form: for uniform h = H, qx = a, qy = b on a jittered [0,100]x[0,20] mesh the three norms
+    are |a|, |b| and H^1.5 times the area 2000 (P1 interpolation of a constant is exact), and the
+    ratio is 0.5 (|a| + |b|) / (g H^1.5).  A dropped absolute value (b < 0) or a swapped component
+    fails it."""
+    from swe_amd import mesh as M
+    m = M.structured_mesh(41, 9, 0.0, 100.0, 0.0, 20.0, jitter=0.2, seed=3)
+    H, a, b, g = 0.64, 0.3, -0.05, 9.81
+    u = np.empty((m.np, 3))
+    u[:, 0], u[:, 1], u[:, 2] = H, a, b
+    out = oracle.ns_l1_ratio(m.jj, m.rr, u, g)
+    area = 2000.0
+    assert out[0] == pytest.approx(abs(a) * area, rel=1e-13)
+    assert out[1] == pytest.approx(abs(b) * area, rel=1e-13)
+    assert out[2] == pytest.approx(H ** 1.5 * area, rel=1e-13)
+    assert out[3] == pytest.approx(0.5 * (abs(a) + abs(b)) / (g * H ** 1.5), rel=1e-13)

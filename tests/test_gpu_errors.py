@@ -2,7 +2,8 @@
 reference's own routines on these states (test_errors_oracle.py).  The device sums are double-double
 over a fixed partition, i.e. the exactly rounded sums: they must equal math.fsum of the oracle's
 summands to 2 ulp, and the reference's sequential sums within that sum's rounding bound (far below
-1e-11 relative on these meshes)."""
+1e-11 relative on these meshes).  The same holds for swe_ns_l1_ratio, the time loop's unit-10
+diagnostic of tests 1, 2 and 10 (main.f90:94-99), against the reference's ns_l1 (tests/golden/flow.npz)."""
 import math
 
 import numpy as np
@@ -167,3 +168,68 @@ def test_device_bump_flow_errors(oracle, test):
             assert np.all(np.abs(dev - ref) <= 1e-12 * np.maximum(np.abs(ref), 1e-300)), (dev, ref)
     ev.close()
     s.close()
+
+
+FLOW_CASES = {"dambreak_run": ("fct", True, 10), "sgn_run": ("viscous", True, 13)}
+
+
+def _flow_check(dev, out, terms, ref=None):
+    """dev: the device's [q1, q2, q3, ratio]; exactly rounded sums (math.fsum of the oracle's summands)
+    to 4 ulp, the sequential sums within their rounding bound, the ratio to 1e-12."""
+    n = terms.shape[-1]
+    gam = (n - 1) * 2.0 ** -53 / (1 - (n - 1) * 2.0 ** -53)
+    for f in range(3):
+        exact = math.fsum(terms[f])
+        assert abs(dev[f] - exact) <= 4 * 2.0 ** -53 * abs(exact), (f, dev[f], exact)
+        assert abs(dev[f] - out[f]) <= gam * terms[f].sum() + 4 * 2.0 ** -53 * abs(exact)
+        if ref is not None:
+            assert abs(dev[f] - ref[f]) <= 1e-12 * max(abs(ref[f]), 1e-300)
+    assert abs(dev[3] - out[3]) <= 1e-12 * abs(out[3])
+    if ref is not None:
+        assert abs(dev[3] - ref[3]) <= 1e-12 * abs(ref[3])
+
+
+@pytest.mark.parametrize("key", sorted(golden("flow.npz")))
+def test_device_flow_ratio_matches_the_reference(oracle, key):
+    """swe_ns_l1_ratio, the time loop's unit-10 diagnostic (main.f90:94-99), on the dam-break states
+    (test 10) and the SGN canal's final state, against the reference's own ns_l1 (tests/golden/flow.npz)."""
+    name, st = key.split("|")
+    z = golden(name + ".npz")
+    u = z["snaps"][list(z["snap_steps"]).index(int(st[4:]))] if st.startswith("snap") else z[st]
+    print(key, flush=True)
+    with _solver(z, *FLOW_CASES[name]) as s:
+        s.set_state(u)
+        ev = s.error_evaluator(z["jj"], z["rr"])
+        dev = ev.ns_l1_ratio()
+        again = ev.ns_l1_ratio()
+        ev.close()
+    assert np.array_equal(dev, again)
+    out, terms = oracle.ns_l1_ratio(z["jj"], z["rr"], u, float(z["scalars"][0]), terms=True)
+    _flow_check(dev, out, terms, golden("flow.npz")[key])
+
+
+def test_device_flow_ratio_every_step_of_a_dam_break(oracle):
+    """The loop of main.f90:71-99 for test 10 with the state on the device: one swe_run step, then the
+    diagnostic, 40 times on a 301x61 dam break (FCT + alpha + Manning, walls), for a solver laid out on
+    the host and one assembled on the device (other internal numbering): the same bits from both, and
+    each value against the oracle's ns_l1 of the state the step left."""
+    from swe_amd import problems as PR
+    p = PR.dam_break(301, 61, seed=5)
+    g = p.params["gravity"]
+    series = []
+    for dev_setup in (False, True):
+        with p.solver(device_setup=dev_setup) as s:
+            s.set_state(p.u0)
+            ev = s.error_evaluator(p.mesh.jj, p.mesh.rr)
+            vals, t = [], 0.0
+            for step in range(40):
+                t, _ = s.run(1, time=t)
+                vals.append(ev.ns_l1_ratio())
+                if not dev_setup and step % 10 == 9:
+                    out, terms = oracle.ns_l1_ratio(p.mesh.jj, p.mesh.rr, s.get_state(), g, terms=True)
+                    _flow_check(vals[-1], out, terms)
+            ev.close()
+        series.append(np.array(vals))
+    print("dam break 301x61: ratio after 1 and 40 steps %.6g %.6g" % (series[0][0, 3], series[0][-1, 3]), flush=True)
+    assert np.array_equal(series[0], series[1])
+    assert np.all(np.diff(series[0][:, 3]) > 0)     # the dam-break front accelerates from rest

@@ -1,7 +1,7 @@
 """The Fortran interface blocks of the device set-up and of compute_errors, called from a Fortran
 program (tests/fortran/api_check.f90, built by `make -C tests/fortran`): its final state, time, every dt
-and the 12 error sums equal, bit for bit, those of the same calls made from Python (which other tests
-pin against the oracle)."""
+and the 12 error sums (and the 4 values of the unit-10 diagnostic) equal, bit for bit, those of the
+same calls made from Python (which other tests pin against the oracle)."""
 import os
 import subprocess
 
@@ -52,6 +52,7 @@ def test_fortran_calls_equal_python_calls(tmp_path, case):
         # with test 4's dam-break solution, which is just as good a check of the calls
         ev = s.error_evaluator(p.mesh.jj, p.mesh.rr)
         errs = ev.sums(t).reshape(-1)
+        flow = ev.ns_l1_ratio()
         ev.close()
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     out = np.fromfile(str(tmp_path / "out.bin"), "<f8")
@@ -64,8 +65,9 @@ def test_fortran_calls_equal_python_calls(tmp_path, case):
     # the batch part: two instances of the same mesh, each the solver's bits
     rest = out[m:]
     ub = rest[:6 * npn].reshape(2, npn, 3)
-    tb, nst, eb = rest[6 * npn:6 * npn + 2], rest[6 * npn + 2:6 * npn + 4], rest[6 * npn + 4:]
-    assert eb.size == 12
+    tb, nst, eb = rest[6 * npn:6 * npn + 2], rest[6 * npn + 2:6 * npn + 4], rest[6 * npn + 4:6 * npn + 16]
+    assert eb.size == 12 and rest.size == 6 * npn + 20
+    assert np.array_equal(rest[6 * npn + 16:], flow)
     assert np.array_equal(ub[0], u) and np.array_equal(ub[1], u)
     assert np.all(tb == t) and np.all(nst == n)
     assert np.array_equal(eb, errs)
