@@ -611,7 +611,8 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // update.f90:366-423, 554-682 with syst_size = 5), with the model's flux (condlim.f90:52-75: the
 // second velocity is v*0, so only x-fluxes remain; lambda and mu keep the full discharge), then
 // mSGN_RHS (update.f90:488-551) as a second pass over the row, the update of every component and
-// the RK combination.  The meshes of this model are small: one kernel, no launch tuning.
+// the RK combination.  One kernel per stage, 256-thread workgroups; its slice order (dispatch or
+// XCD-aware) is the tuned one of the first-order stage, whose place it takes in the step.
 // =============================================================================================
 // pTilde (update.f90:515-518) at node n: paper_constant = lambdaSGN*g/(3*SQRT(lumped)),
 // x = (h*eta)/h**2; the literal integer powers are products from the left, as flang expands them.
@@ -626,8 +627,8 @@ __global__ void __launch_bounds__(BS)
 k_stage_sgn(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, const NodeX* __restrict__ UX,
             const Node* __restrict__ Uo, const NodeX* __restrict__ UoX, Node* __restrict__ out,
             NodeX* __restrict__ outX, const double* __restrict__ d0, const double* __restrict__ alpha, int stage,
-            int* __restrict__ flags_out) {
-    const int slice = wave_slice<BS>(M, 0);
+            int* __restrict__ flags_out, int remap) {
+    const int slice = wave_slice<BS>(M, remap);
     if (slice < 0) return;
     const int inst = slice_inst(M, slice);
     if (inst < 0 || !T[inst].go) return;
@@ -1769,7 +1770,8 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         timed_begin(P, KID_STAGE_LO);
 #define SWE_SGN_LAUNCH(S, A)                                                                                       \
     hipLaunchKernelGGL((k_stage_sgn<BS, S, A>), g, dim3(BS), 0, P.stream, M, T, U, UX, Uo, UoX, out, outX,          \
-                       (S == SMB_GALERKIN) ? nullptr : d0, (const double*)P.alpha, stage, P.flags)
+                       (S == SMB_GALERKIN) ? nullptr : d0, (const double*)P.alpha, stage, P.flags,                  \
+                       P.tune.k[KID_STAGE_LO].remap)
         if (smb == SMB_GALERKIN) SWE_SGN_LAUNCH(SMB_GALERKIN, false);
         else if (smb == SMB_1) {
             if (alpha) SWE_SGN_LAUNCH(SMB_1, true);
