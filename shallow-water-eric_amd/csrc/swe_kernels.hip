@@ -611,8 +611,8 @@ k_stage_lo(Mesh M, const InstDev* __restrict__ T, const Node* __restrict__ U, co
 // update.f90:366-423, 554-682 with syst_size = 5), with the model's flux (condlim.f90:52-75: the
 // second velocity is v*0, so only x-fluxes remain; lambda and mu keep the full discharge), then
 // mSGN_RHS (update.f90:488-551) as a second pass over the row, the update of every component and
-// the RK combination.  One kernel per stage, 256-thread workgroups; its slice order (dispatch or
-// XCD-aware) is the tuned one of the first-order stage, whose place it takes in the step.
+// the RK combination.  One kernel per stage; its workgroup size and slice order (dispatch or
+// XCD-aware) are the tuned ones of the first-order stage, whose place it takes in the step.
 // =============================================================================================
 // pTilde (update.f90:515-518) at node n: paper_constant = lambdaSGN*g/(3*SQRT(lumped)),
 // x = (h*eta)/h**2; the literal integer powers are products from the left, as flang expands them.
@@ -1651,14 +1651,15 @@ template <int SMB, bool ALPHA>
 void launch_stage_lo(Pool& P, const Node* U, const Node* Uo, Node* out, const double* d0, int stage) {
     const Mesh M = mesh_view(P);
     const LaunchCfg& L = P.tune.k[KID_STAGE_LO];
-    constexpr int BS = 256;
-    const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
-    if (P.any_src)
-        hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA, true>), g, dim3(BS), 0, P.stream, M, (const InstDev*)P.tab, U,
-                           Uo, out, d0, (const double*)P.alpha, stage, P.flags, L.remap);
-    else
-        hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA, false>), g, dim3(BS), 0, P.stream, M, (const InstDev*)P.tab, U,
-                           Uo, out, d0, (const double*)P.alpha, stage, P.flags, L.remap);
+    SWE_BS_SWITCH(L.block, {
+        const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
+        if (P.any_src)
+            hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA, true>), g, dim3(BS), 0, P.stream, M, (const InstDev*)P.tab,
+                               U, Uo, out, d0, (const double*)P.alpha, stage, P.flags, L.remap);
+        else
+            hipLaunchKernelGGL((k_stage_lo<BS, SMB, ALPHA, false>), g, dim3(BS), 0, P.stream, M, (const InstDev*)P.tab,
+                               U, Uo, out, d0, (const double*)P.alpha, stage, P.flags, L.remap);
+    })
 }
 
 template <bool ALPHA, bool WRITE_LO>
@@ -1762,27 +1763,28 @@ int launch_euler_stage(Pool& P, const Node* U, const Node* Uo, Node* out, const 
         const NodeX* UoX = ext_of(P, Uo);
         NodeX* outX = ext_of(P, out);
         if (!UX || !UoX || !outX) return fail(SWE_EINVAL, "SGN stage on a buffer without h*eta, h*w");
-        constexpr int BS = 256;
-        const dim3 g(blocks_of((int64_t)M.nact * 64, BS));
+        const LaunchCfg& L = P.tune.k[KID_STAGE_LO];
         const int smb = (md.viscosity_type == SWE_GALERKIN) ? SMB_GALERKIN
                         : (md.viscous_type == 1)             ? SMB_1
                         : (md.roundoff ? SMB_ROUNDOFF : SMB_2);
         timed_begin(P, KID_STAGE_LO);
 #define SWE_SGN_LAUNCH(S, A)                                                                                       \
-    hipLaunchKernelGGL((k_stage_sgn<BS, S, A>), g, dim3(BS), 0, P.stream, M, T, U, UX, Uo, UoX, out, outX,          \
-                       (S == SMB_GALERKIN) ? nullptr : d0, (const double*)P.alpha, stage, P.flags,                  \
-                       P.tune.k[KID_STAGE_LO].remap)
-        if (smb == SMB_GALERKIN) SWE_SGN_LAUNCH(SMB_GALERKIN, false);
-        else if (smb == SMB_1) {
-            if (alpha) SWE_SGN_LAUNCH(SMB_1, true);
-            else SWE_SGN_LAUNCH(SMB_1, false);
-        } else if (smb == SMB_ROUNDOFF) {
-            if (alpha) SWE_SGN_LAUNCH(SMB_ROUNDOFF, true);
-            else SWE_SGN_LAUNCH(SMB_ROUNDOFF, false);
-        } else {
-            if (alpha) SWE_SGN_LAUNCH(SMB_2, true);
-            else SWE_SGN_LAUNCH(SMB_2, false);
-        }
+    hipLaunchKernelGGL((k_stage_sgn<BS, S, A>), dim3(blocks_of((int64_t)M.nact * 64, BS)), dim3(BS), 0, P.stream, M, \
+                       T, U, UX, Uo, UoX, out, outX, (S == SMB_GALERKIN) ? nullptr : d0, (const double*)P.alpha,     \
+                       stage, P.flags, L.remap)
+        SWE_BS_SWITCH(L.block, {
+            if (smb == SMB_GALERKIN) SWE_SGN_LAUNCH(SMB_GALERKIN, false);
+            else if (smb == SMB_1) {
+                if (alpha) SWE_SGN_LAUNCH(SMB_1, true);
+                else SWE_SGN_LAUNCH(SMB_1, false);
+            } else if (smb == SMB_ROUNDOFF) {
+                if (alpha) SWE_SGN_LAUNCH(SMB_ROUNDOFF, true);
+                else SWE_SGN_LAUNCH(SMB_ROUNDOFF, false);
+            } else {
+                if (alpha) SWE_SGN_LAUNCH(SMB_2, true);
+                else SWE_SGN_LAUNCH(SMB_2, false);
+            }
+        })
 #undef SWE_SGN_LAUNCH
         timed_end(P, KID_STAGE_LO);
     } else if (md.viscosity_type == SWE_GALERKIN) {
