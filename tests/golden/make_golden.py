@@ -23,12 +23,15 @@ links the reference modules) and the outputs it wrote.  Cases:
   errors_bump.npz     the same norms for the steady bump flows (tests 3, 6, 7) on a [0,25]x[0,1] strip
   flow.npz            the time loop's unit-10 diagnostic (SOURCES/main.f90:94-99, ns_l1 of qx, qy, h*sqrt(h))
                       on the dam-break states (test 10) and the SGN canal's final state
+  unit10_test1.npz    the reference program (main.f90) on test 1, FCT + alpha: set-up arrays, and per step
+                      the time and the ratio it wrote to fort.10
 """
 from __future__ import annotations
 
 import json
 import os
 import shutil
+import subprocess
 import sys
 
 import numpy as np
@@ -446,6 +449,17 @@ def gen_flow():
     out["sgn_run|final"] = flow(case, z["final"])
     case.close()
     save("flow.npz", **out)
+    # the reference program itself (SOURCES/main.f90, oracle/_ref/ref_main) on test 1, FCT + alpha, to
+    # Tfinal = 1: its set-up arrays and, per step, the time and the ratio it wrote to unit 10 (fort.10)
+    case = RefCase(M.structured_mesh(81, 41, 0.0, 2.0, 0.0, 1.0, jitter=0.2, seed=3), test=1, viscosity="fct",
+                   alpha=True, cfl=0.25)
+    S = case.setup()
+    r = subprocess.run([os.path.join(REPO, "oracle", "_ref", "ref_main")], cwd=case.d, capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    rows = np.array([[float(x) for x in l.split()] for l in open(os.path.join(case.d, "fort.10")).read().splitlines()])
+    save("unit10_test1.npz", **setup_arrays(S), times=rows[:, 0], ratios=rows[:, 1])
+    case.close()
 
 
 def main():

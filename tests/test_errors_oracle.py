@@ -127,3 +127,21 @@ def test_flow_ratio_of_uniform_fields(oracle):
     assert out[1] == pytest.approx(abs(b) * area, rel=1e-13)
     assert out[2] == pytest.approx(H ** 1.5 * area, rel=1e-13)
     assert out[3] == pytest.approx(0.5 * (abs(a) + abs(b)) / (g * H ** 1.5), rel=1e-13)
+
+
+def test_oracle_unit10_series_is_the_reference_programs(oracle):
+    """The reference program itself (main.f90) on test 1, FCT + alpha, 268 steps to Tfinal = 1: the
+    oracle stepped one step at a time from the reference's set-up reaches every step's time and writes
+    every unit-10 ratio (orc_ns_l1_ratio) bit for bit as the program did (tests/golden/unit10_test1.npz)."""
+    z = golden("unit10_test1.npz")
+    g, htiny, hmax, cfl = z["scalars"]
+    o = oracle.Oracle(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+                      max_water_h=hmax, viscosity_type=oracle.FCT, alpha_limit=True, type_test=1)
+    try:
+        u, t = z["u0"], float(z["time0"])
+        for tr, ratio in zip(z["times"], z["ratios"]):
+            u, _, t, err = o.run(u, 1, t0=t)
+            assert err == 0 and t == tr
+            assert oracle.ns_l1_ratio(z["jj"], z["rr"], u, g)[3] == ratio
+    finally:
+        o.close()

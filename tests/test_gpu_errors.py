@@ -233,3 +233,26 @@ def test_device_flow_ratio_every_step_of_a_dam_break(oracle):
     print("dam break 301x61: ratio after 1 and 40 steps %.6g %.6g" % (series[0][0, 3], series[0][-1, 3]), flush=True)
     assert np.array_equal(series[0], series[1])
     assert np.all(np.diff(series[0][:, 3]) > 0)     # the dam-break front accelerates from rest
+
+
+def test_device_loop_unit10_series_equals_the_reference_program():
+    """Test 1 (FCT + alpha) device-resident from the reference program's own set-up: swe_run one step at
+    a time and swe_ns_l1_ratio after each, as a device-resident main.f90 would write unit 10.  Every
+    step's time equals the reference program's bit for bit, every ratio its fort.10 value within 1e-11
+    (tests/golden/unit10_test1.npz, 268 steps)."""
+    from swe_amd.solver import Solver
+    z = golden("unit10_test1.npz")
+    g, htiny, hmax, cfl = z["scalars"]
+    with Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
+                max_water_h=hmax, viscosity="fct", alpha_limit=True, type_test=1) as s:
+        s.set_state(z["u0"])
+        ev = s.error_evaluator(z["jj"], z["rr"])
+        t = float(z["time0"])
+        worst = 0.0
+        for tr, ratio in zip(z["times"], z["ratios"]):
+            t, _ = s.run(1, time=t)
+            assert t == tr
+            worst = max(worst, abs(ev.ns_l1_ratio()[3] - ratio) / abs(ratio))
+        ev.close()
+    print("test 1, %d steps: largest relative gap of the unit-10 ratio %.3g" % (z["times"].size, worst), flush=True)
+    assert worst <= 1e-11

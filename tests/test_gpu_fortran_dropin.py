@@ -3,8 +3,9 @@
 program linked with the reference's SOURCES/update.f90 and fct.f90.
 
 Both executables are built by `make -C oracle ref` from the reference sources (outputs in
-oracle/_ref/).  They read the same `data` file and mesh; every printed time step (time, dt) and
-the final analytic-error lines must be identical.
+oracle/_ref/).  They read the same `data` file and mesh; every printed time step (time, dt), the
+final analytic-error lines and, for tests 1 and 2, the diagnostic file the time loop writes (fort.10)
+must be identical.
 """
 import os
 import subprocess
@@ -47,3 +48,33 @@ def test_reference_driver_with_gpu_step(tmp_path, viscosity, alpha):
     assert len(steps) > 50
     assert any(l.startswith("Relative L1") for l in ref)
     assert gpu == ref
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_MAIN) and os.path.exists(DROPIN_MAIN)),
+                    reason="oracle/_ref was not built from the reference sources (make -C oracle ref)")
+@pytest.mark.parametrize("test", [1, 2])
+@pytest.mark.parametrize("viscosity,alpha", [("fct", True), ("viscous", False)])
+def test_reference_driver_unit10_diagnostic(tmp_path, test, viscosity, alpha):
+    """Tests 1 and 2 (still water over bumps, the reference's own init and sol_anal): the time loop writes
+    0.5*(q1+q2)/(g*q3) to unit 10 after every step (main.f90:94-99), from ns_l1 of the state the GPU
+    step produced.  The file fort.10 and every printed time step are identical for the two programs."""
+    import pyoracle as O
+    from swe_amd import mesh as M
+    if test == 1:
+        mesh, tfinal = M.structured_mesh(81, 41, 0.0, 2.0, 0.0, 1.0, jitter=0.2, seed=3), 0.5
+    else:
+        mesh, tfinal = M.structured_mesh(201, 9, 0.0, 25.0, 0.0, 1.0, jitter=0.2, seed=3), 1.0
+    outs = []
+    for exe in (REF_MAIN, DROPIN_MAIN):
+        d = tmp_path / os.path.basename(exe)
+        d.mkdir()
+        M.write_fem(str(d / "mesh.FEM"), mesh)
+        O.data_file(str(d / "data"), mesh_dir=".", mesh_file="mesh.FEM", test=test, tfinal=tfinal, cfl=0.25,
+                    viscosity=viscosity, alpha=alpha)
+        print("running", exe, "test", test, viscosity, alpha, flush=True)
+        printed = run(exe, str(d))
+        outs.append((printed, (d / "fort.10").read_text().splitlines()))
+    (ref, ref10), (gpu, gpu10) = outs
+    assert len(ref10) > 50 and len(ref10) == sum(1 for l in ref if l.startswith("time"))
+    assert gpu == ref and gpu10 == ref10
+
