@@ -5,7 +5,7 @@ program linked with the reference's SOURCES/update.f90 and fct.f90.
 Both executables are built by `make -C oracle ref` from the reference sources (outputs in
 oracle/_ref/).  They read the same `data` file and mesh; every printed time step (time, dt), the
 final analytic-error lines and, for tests 1 and 2, the diagnostic file the time loop writes (fort.10)
-must be identical.
+must be identical.  The cases cover the reference's own initial data of tests 1, 2, 3, 4, 6, 7 and 8.
 """
 import os
 import subprocess
@@ -78,3 +78,39 @@ def test_reference_driver_unit10_diagnostic(tmp_path, test, viscosity, alpha):
     assert len(ref10) > 50 and len(ref10) == sum(1 for l in ref if l.startswith("time"))
     assert gpu == ref and gpu10 == ref10
 
+
+
+# test -> (mesh, Tfinal, h, ux, uy Dirichlet sides): the steady flows over the bump (3, 6, 7) with the
+# analytic values of sol_anal imposed at both ends, the solitary wave around the conical island (8)
+OTHER_TESTS = {
+    3: (dict(nx=201, ny=9, x1=25.0, y1=1.0), 1.0, (2, 4), (2, 4), (1, 3)),
+    6: (dict(nx=201, ny=9, x1=25.0, y1=1.0), 1.0, (2, 4), (2, 4), (1, 3)),
+    7: (dict(nx=201, ny=9, x1=25.0, y1=1.0), 1.0, (2, 4), (2, 4), (1, 3)),
+    8: (dict(nx=61, ny=53, x1=30.0, y1=26.0), 2.0, (), (2, 4), (1, 3)),
+}
+
+
+@pytest.mark.skipif(not (os.path.exists(REF_MAIN) and os.path.exists(DROPIN_MAIN)),
+                    reason="oracle/_ref was not built from the reference sources (make -C oracle ref)")
+@pytest.mark.parametrize("test", sorted(OTHER_TESTS))
+@pytest.mark.parametrize("viscosity,alpha", [("fct", True), ("viscous", False)])
+def test_reference_driver_other_test_types(tmp_path, test, viscosity, alpha):
+    """The reference's own init, sol_anal boundary values and error lines for tests 3, 6, 7 (flows over
+    the bump: subcritical, transcritical, with a hydraulic jump) and 8 (solitary wave around an island):
+    every printed time step and error line identical for the two programs."""
+    import pyoracle as O
+    from swe_amd import mesh as M
+    m, tfinal, hs, uxs, uys = OTHER_TESTS[test]
+    mesh = M.structured_mesh(m["nx"], m["ny"], 0.0, m["x1"], 0.0, m["y1"], jitter=0.2, seed=3)
+    outs = []
+    for exe in (REF_MAIN, DROPIN_MAIN):
+        d = tmp_path / os.path.basename(exe)
+        d.mkdir()
+        M.write_fem(str(d / "mesh.FEM"), mesh)
+        O.data_file(str(d / "data"), mesh_dir=".", mesh_file="mesh.FEM", test=test, tfinal=tfinal, cfl=0.25,
+                    viscosity=viscosity, alpha=alpha, h_dir=hs, ux_dir=uxs, uy_dir=uys)
+        print("running", exe, "test", test, viscosity, alpha, flush=True)
+        outs.append(run(exe, str(d)))
+    ref, gpu = outs
+    assert sum(1 for l in ref if l.startswith("time")) > 50
+    assert gpu == ref
