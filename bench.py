@@ -595,20 +595,24 @@ def bench_errors(n=4097, reps=5):
     s.close()
     res = {"nodes": int(m.np), "elements": int(m.jj.shape[0]), "device_s": min(ts), "l1_h": rel["l1_h"],
            "l2_h": rel["l2_h"]}
+    # the CPU baseline leg of this workload: the oracle's sequential sums on one host core, timed,
+    # and used as the checker of the device sums
     try:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import pyoracle
         t1 = time.perf_counter()
         ref = pyoracle.compute_errors(m.jj, m.rr, u, 5, t, g, htiny)
-        res["oracle_s"] = time.perf_counter() - t1
-        res["max_rel_diff"] = float(np.max(np.abs(dev - ref) / np.maximum(np.abs(ref), 1e-300)))
+        cpu = {"value": time.perf_counter() - t1, "unit": "s", "cores": 1, "kind": "port",
+               "sample": "the same sums on the same state (C restatement of compute_errors, 1 thread)"}
+        cpu["max_rel_diff"] = float(np.max(np.abs(dev - ref) / np.maximum(np.abs(ref), 1e-300)))
         # the summands are >= 0, so the sequential sums are within (n-1)u/(1-(n-1)u) of the exact
         # ones, relative (half that after the square root of the L2 sums)
         nq = 3 * m.jj.shape[0]
-        res["sequential_rounding_bound"] = (nq - 1) * 2.0 ** -53 / (1 - (nq - 1) * 2.0 ** -53)
-        res["within_bound"] = bool(res["max_rel_diff"] <= res["sequential_rounding_bound"])
+        cpu["sequential_rounding_bound"] = (nq - 1) * 2.0 ** -53 / (1 - (nq - 1) * 2.0 ** -53)
+        cpu["within_bound"] = bool(cpu["max_rel_diff"] <= cpu["sequential_rounding_bound"])
+        res["cpu_baseline"] = cpu
     except (ImportError, OSError) as e:
-        res["oracle"] = "not available: %s" % e
+        res["cpu_baseline"] = "not available: %s" % e
     print("# errors: device %.4f s; %s" % (res["device_s"], {k: v for k, v in res.items() if k != "device_s"}),
           file=sys.stderr, flush=True)
     return res
