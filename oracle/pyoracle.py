@@ -19,6 +19,8 @@ REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 # the same driver with entropy_residual's O(np^2) rescale loop reduced to its one assignment
 # (oracle/hoist_rescale.awk): the same bits, timed as the CPU baseline
 REF_DRIVER_HOIST = os.path.join(HERE, "_ref", "ref_driver_hoist")
+# the reference with MODULE update's if_roundoff_fix = .FALSE. (the smb_2 branches of euler)
+REF_DRIVER_NOROUND = os.path.join(HERE, "_ref", "ref_driver_noround")
 
 GALERKIN, VISCOUS, ENTROPY_VISC, FCT = 0, 1, 2, 3
 VISC_NAMES = {GALERKIN: "galerkin", VISCOUS: "viscous", ENTROPY_VISC: "entropy_visc", FCT: "fct"}

@@ -25,6 +25,8 @@ links the reference modules) and the outputs it wrote.  Cases:
                       on the dam-break states (test 10) and the SGN canal's final state
   unit10_test1.npz    the reference program (main.f90) on test 1, FCT + alpha: set-up arrays, and per step
                       the time and the ratio it wrote to fort.10
+  noround.npz         the reference built with if_roundoff_fix = .FALSE. (its smb_2 branches): euler in five
+                      modes at the par41_modes.npz states, 150 first-order and 100 fct + alpha steps
 """
 from __future__ import annotations
 
@@ -138,8 +140,9 @@ class RefCase:
 
     def __init__(self, raw_mesh=None, fem_path=None, *, test, viscosity="viscous", alpha=False, cfl=0.25,
                  viscous_type="type2", h_sides=(), ux_sides=(), uy_sides=(), mannings=None,
-                 init=None, bc=(0, 0.0, 1.0, 0.0), lambda_sgn=None):
+                 init=None, bc=(0, 0.0, 1.0, 0.0), lambda_sgn=None, driver=None):
         self.nc = 5 if test == 13 else 3
+        self.driver = driver                  # None: the unmodified reference build
         self.d = O.tmpdir()
         if fem_path:
             shutil.copy(fem_path, os.path.join(self.d, "mesh.FEM"))
@@ -163,7 +166,7 @@ class RefCase:
 
     def setup(self):
         out = fresh(self.d, "setup")
-        O.run_ref(["setup", "data", self.init, "setup"], cwd=self.d)
+        O.run_ref(["setup", "data", self.init, "setup"], cwd=self.d, driver=self.driver)
         return O.read_setup(out)
 
     def euler(self, u, dt):
@@ -173,7 +176,7 @@ class RefCase:
         for f in os.listdir(out):
             os.remove(os.path.join(out, f))
         try:
-            O.run_ref(["euler", "data", self.init, "state.f8", repr(float(dt)), "eu"], cwd=self.d)
+            O.run_ref(["euler", "data", self.init, "state.f8", repr(float(dt)), "eu"], cwd=self.d, driver=self.driver)
         except RuntimeError as e:
             if "Min h<0, STOP" not in str(e):
                 raise
@@ -183,13 +186,13 @@ class RefCase:
 
     def dij(self, u):
         np.ascontiguousarray(u, dtype="<f8").tofile(os.path.join(self.d, "state.f8"))
-        O.run_ref(["dij", "data", self.init, "state.f8", "dij.f8"], cwd=self.d)
+        O.run_ref(["dij", "data", self.init, "state.f8", "dij.f8"], cwd=self.d, driver=self.driver)
         return np.fromfile(os.path.join(self.d, "dij.f8"))
 
     def run(self, nsteps, every=0):
         out = fresh(self.d, "run")
         args = ["run", "data", self.init, nsteps, "run"] + ([every] if every else [])
-        O.run_ref(args, cwd=self.d, timeout=7200)
+        O.run_ref(args, cwd=self.d, timeout=7200, driver=self.driver)
         npn = int(np.fromfile(os.path.join(self.d, "setup", "dims.i4"), dtype="<i4")[0]) if os.path.exists(
             os.path.join(self.d, "setup", "dims.i4")) else None
         u0 = np.fromfile(os.path.join(out, "u0.f8"))
@@ -462,6 +465,38 @@ def gen_flow():
     case.close()
 
 
+def gen_noround():
+    """The reference built with if_roundoff_fix = .FALSE. (MODULE update's compile-time switch,
+    SOURCES/update.f90:15; oracle/Makefile target ref_driver_noround): euler in the five modes whose
+    branches it changes (smb_2 and the clipped update max(h + dt*rk, 0) instead of smb_2_roundoff and
+    the fix_roundoff update, update.f90:193-215, 246-262) at the two states of par41_modes.npz, and
+    runs of 150 first-order and 100 fct + alpha steps from the initial state."""
+    z = np.load(os.path.join(HERE, "par41_modes.npz"))
+    case = RefCase(M.structured_mesh(41, 41, 0.0, 4.0, 0.0, 4.0), test=5, viscosity="viscous", cfl=0.25,
+                   driver=O.REF_DRIVER_NOROUND)
+    S = case.setup()
+    assert np.array_equal(S["jj"], z["jj"]) and np.array_equal(S["rr"], z["rr"])
+    modes = [("viscous", False, "type2"), ("viscous", True, "type2"), ("entropy_visc", True, "type2"),
+             ("fct", False, "type2"), ("fct", True, "type2")]
+    outs, status = [], []
+    for u, dt in zip(z["states"], z["dts"]):
+        for (v, a, t) in modes:
+            case.write_data(v, a, t)
+            un, _, st = case.euler(u, dt)
+            outs.append(un)
+            status.append(st)
+    runs = {}
+    for tag, (v, a), n in (("viscous", ("viscous", False), 150), ("fct_alpha", ("fct", True), 100)):
+        case.write_data(v, a, "type2")
+        r = case.run(n)
+        runs["run_%s_final" % tag] = r["final"]
+        runs["run_%s_dts" % tag] = r["dts"]
+        runs["run_%s_time" % tag] = r["time"]
+    save("noround.npz", modes=np.array(["%s|%d|%s" % m for m in modes]),
+         unext=np.stack(outs).reshape(2, len(modes), -1, 3), status=np.array(status).reshape(2, len(modes)), **runs)
+    case.close()
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/ref_driver is not built: run `make -C oracle ref` first")
@@ -469,7 +504,7 @@ def main():
     gens = [("lambda", gen_lambda), ("fct", gen_fct), ("par41", gen_par41_modes), ("par81", gen_par81_run),
             ("dambreak", gen_dambreak), ("dambreak9", gen_dambreak9), ("sgn", gen_sgn), ("bumps", gen_bumps), ("cav025", gen_cav025),
             ("cdisk", gen_cdisk_coriolis), ("friction", gen_par_friction), ("errors", gen_errors),
-            ("flow", gen_flow)]
+            ("flow", gen_flow), ("noround", gen_noround)]
     for name, fn in gens:
         if not only or name in only:
             fn()

@@ -26,12 +26,13 @@ def set_lo_kind(s, kind):
     s.schedule = {k: (b, r, kind if k == "k_fct_lo" else 0) for k, (b, r, _) in s.schedule.items()}
 
 
-def solver_from(z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, bc=0, kind=0):
+def solver_from(z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, bc=0, kind=0, roundoff_fix=True):
     from swe_amd.solver import Solver
     g, htiny, hmax, cfl = z["scalars"]
     s = Solver(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
                max_water_h=hmax, viscosity=viscosity, viscous_type=viscous_type, alpha_limit=alpha,
-               type_test=type_test, mannings=mannings, h_dir=z["h_dir"], ux_dir=z["ux_dir"], uy_dir=z["uy_dir"])
+               type_test=type_test, mannings=mannings, h_dir=z["h_dir"], ux_dir=z["ux_dir"], uy_dir=z["uy_dir"],
+               roundoff_fix=roundoff_fix)
     if bc == 2:
         amp, period, level = z["bc"]
         s.set_bc(2, amp, period, level)
@@ -296,3 +297,25 @@ def test_fct_generic_against_oracle(oracle):
         dg = rng.uniform(0.5, 2.0, n)
         print("fct_generic", n, flush=True)
         assert same_bits(fct_generic_csr(ia, ja, u, maxn, minn, f, dg), oracle.fct_generic(ia, ja, f, u, maxn, minn, dg))
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_roundoff_off_branches_against_reference(kind):
+    """The reference built with if_roundoff_fix = .FALSE. (tests/golden/noround.npz: smb_2 and the
+    clipped update): euler in five modes at two states, 150 first-order and 100 fct + alpha steps,
+    every bit, with both schedule kinds of k_fct_lo."""
+    z = golden("par41_modes.npz")
+    n = golden("noround.npz")
+    for k in range(2):
+        for m, mode in enumerate(n["modes"]):
+            visc, alpha, _ = str(mode).split("|")
+            print("roundoff off: euler state", k, "mode", mode, "kind", kind, flush=True)
+            with solver_from(z, visc, bool(int(alpha)), 5, kind=kind, roundoff_fix=False) as s:
+                assert same_bits(s.euler(z["states"][k], float(z["dts"][k])), n["unext"][k, m]), mode
+    for tag, visc, alpha, steps in (("viscous", "viscous", False, 150), ("fct_alpha", "fct", True, 100)):
+        print("roundoff off: run", tag, flush=True)
+        with solver_from(z, visc, alpha, 5, kind=kind, roundoff_fix=False) as s:
+            s.set_state(z["u0"])
+            t, dts = s.run(steps, time=float(z["time0"]))
+            assert same_bits(dts, n["run_%s_dts" % tag]) and t == float(n["run_%s_time" % tag])
+            assert same_bits(s.get_state(), n["run_%s_final" % tag]), tag

@@ -11,11 +11,11 @@ import pytest
 from conftest import golden, same_bits
 
 
-def make_oracle(O, z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2):
+def make_oracle(O, z, viscosity, alpha, type_test, mannings=0.0, viscous_type=2, roundoff_fix=True):
     g, htiny, hmax, cfl = z["scalars"]
     return O.Oracle(z["ia"], z["ja"], z["cij"], z["lumped"], z["bath"], gravity=g, htiny=htiny, cfl=cfl,
                     max_water_h=hmax, viscosity_type=viscosity, viscous_type=viscous_type,
-                    alpha_limit=alpha, type_test=type_test, mannings=mannings)
+                    alpha_limit=alpha, type_test=type_test, mannings=mannings, roundoff_fix=roundoff_fix)
 
 
 # name -> (viscosity, alpha, type_test, mannings, bc_kind)
@@ -181,3 +181,24 @@ def test_sgn_model_matches_reference(oracle):
                             max_water_h=hmax, viscosity_type=visc, alpha_limit=True, type_test=13, lambda_sgn=9.0)
         assert orc.euler(z["u0"], 1e-3)[2] == 3
 
+
+
+def test_roundoff_off_branches_match_reference(oracle):
+    """The reference built with if_roundoff_fix = .FALSE. (tests/golden/noround.npz): euler in the five
+    modes whose branches the switch changes, at both par41_modes.npz states, and runs of 150
+    first-order and 100 fct + alpha steps, all bit for bit."""
+    z = golden("par41_modes.npz")
+    n = golden("noround.npz")
+    for k in range(2):
+        for m, mode in enumerate(n["modes"]):
+            visc, alpha, _ = str(mode).split("|")
+            orc = make_oracle(oracle, z, VISC[visc], bool(int(alpha)), 5, roundoff_fix=False)
+            un, _, err = orc.euler(z["states"][k], float(z["dts"][k]))
+            assert err == 0 and n["status"][k, m] == 0
+            assert same_bits(un, n["unext"][k, m]), (k, mode)
+    for tag, visc, alpha, steps in (("viscous", "viscous", False, 150), ("fct_alpha", "fct", True, 100)):
+        orc = make_oracle(oracle, z, VISC[visc], alpha, 5, roundoff_fix=False)
+        u, dts, t, err = orc.run(z["u0"], steps, t0=float(z["time0"]))
+        assert err == 0
+        assert same_bits(dts, n["run_%s_dts" % tag]) and t == float(n["run_%s_time" % tag])
+        assert same_bits(u, n["run_%s_final" % tag]), tag
