@@ -109,21 +109,23 @@ def test_device_errors_reject_tests_without_sol_anal_and_bad_lists():
 
 
 def test_batch_instance_errors_equal_solver_alone():
-    """swe_batch_errors_create: the sums of a batch instance after a run in a mixed batch are those
-    of the same mesh run alone (same state bits, same element partition); removed instances are refused."""
+    """swe_batch_errors_create: the sums of a batch instance after a run in a mixed batch, and its
+    unit-10 diagnostic, are those of the same mesh run alone (same state bits, same element
+    partition); removed instances are refused."""
     from swe_amd import problems as PR
     from swe_amd.batch import Batch
     from swe_amd.solver import SweError
     pa, pb = PR.paraboloid(n=41), PR.paraboloid(n=61)
     pd = PR.dam_break(101, 21, seed=7, viscosity="viscous", alpha=False)    # the pool runs one scheme
     n = 30
-    alone = []
+    alone, alone_flow = [], []
     for p in (pa, pb):
         with p.solver() as s:
             s.set_state(p.u0)
             s.run(n, time=0.0)
             ev = s.error_evaluator(p.mesh.jj, p.mesh.rr)
             alone.append(ev.sums())
+            alone_flow.append(ev.ns_l1_ratio())
             ev.close()
     with Batch(200000, 1500000, 20000, 8) as b:
         ka = b.add_problem(pa, max_steps=n)
@@ -134,6 +136,7 @@ def test_batch_instance_errors_equal_solver_alone():
         eva = b.error_evaluator(ka, pa.mesh.jj, pa.mesh.rr)
         evb = b.error_evaluator(kb, pb.mesh.jj, pb.mesh.rr)
         assert np.array_equal(eva.sums(), alone[0]) and np.array_equal(evb.sums(), alone[1])
+        assert np.array_equal(eva.ns_l1_ratio(), alone_flow[0]) and np.array_equal(evb.ns_l1_ratio(), alone_flow[1])
         b.remove(ka)
         with pytest.raises(SweError):
             eva.sums(1.0)
