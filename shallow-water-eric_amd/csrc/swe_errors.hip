@@ -287,6 +287,29 @@ struct ErrArgs {
     int64_t me;
 };
 
+// The block's NS double-double sums into part[blockIdx.x * NS + q]: a butterfly in a fixed pattern
+// within each wave, then the waves in order.
+template <int NS>
+__device__ __forceinline__ void block_sums(DD (&acc)[NS], DD* __restrict__ part) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+        for (int off = 32; off > 0; off >>= 1) {
+            const DD o{__shfl_xor(acc[q].hi, off, 64), __shfl_xor(acc[q].lo, off, 64)};
+            acc[q] = dd_add(acc[q], o);
+        }
+    __shared__ DD wsum[kErrBlock / 64][NS];
+    const int wave = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NS; ++q) wsum[wave][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < NS) {
+        DD s = wsum[0][threadIdx.x];
+        for (int w = 1; w < kErrBlock / 64; ++w) s = dd_add(s, wsum[w][threadIdx.x]);
+        part[(size_t)blockIdx.x * NS + threadIdx.x] = s;
+    }
+}
+
 // Each thread sums the elements m = tid, tid + nthreads, ...; the wave, then the block, then the grid
 // are reduced in a fixed order.
 __global__ void __launch_bounds__(kErrBlock) k_errors(ErrArgs a, Anal A, DD* __restrict__ part) {
@@ -347,24 +370,7 @@ __global__ void __launch_bounds__(kErrBlock) k_errors(ErrArgs a, Anal A, DD* __r
             }
         }
     }
-    // wave: butterfly in a fixed pattern
-#pragma unroll
-    for (int q = 0; q < kSums; ++q)
-        for (int off = 32; off > 0; off >>= 1) {
-            const DD o{__shfl_xor(acc[q].hi, off, 64), __shfl_xor(acc[q].lo, off, 64)};
-            acc[q] = dd_add(acc[q], o);
-        }
-    __shared__ DD wsum[kErrBlock / 64][kSums];
-    const int wave = (int)(threadIdx.x >> 6);
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < kSums; ++q) wsum[wave][q] = acc[q];
-    __syncthreads();
-    if (threadIdx.x < kSums) {
-        DD s = wsum[0][threadIdx.x];
-        for (int w = 1; w < kErrBlock / 64; ++w) s = dd_add(s, wsum[w][threadIdx.x]);
-        part[(size_t)blockIdx.x * kSums + threadIdx.x] = s;
-    }
+    block_sums<kSums>(acc, part);
 }
 
 // One block: partial sums in block order (lane q of the block reduces sum q).
@@ -421,23 +427,7 @@ __global__ void __launch_bounds__(kErrBlock) k_flow(ErrArgs a, DD* __restrict__ 
                 acc[k] = dd_add1(acc[k], fabs(fl) * rj);
             }
     }
-#pragma unroll
-    for (int q = 0; q < kFlowSums; ++q)
-        for (int off = 32; off > 0; off >>= 1) {
-            const DD o{__shfl_xor(acc[q].hi, off, 64), __shfl_xor(acc[q].lo, off, 64)};
-            acc[q] = dd_add(acc[q], o);
-        }
-    __shared__ DD wsum[kErrBlock / 64][kFlowSums];
-    const int wave = (int)(threadIdx.x >> 6);
-    if ((threadIdx.x & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < kFlowSums; ++q) wsum[wave][q] = acc[q];
-    __syncthreads();
-    if (threadIdx.x < kFlowSums) {
-        DD s = wsum[0][threadIdx.x];
-        for (int w = 1; w < kErrBlock / 64; ++w) s = dd_add(s, wsum[w][threadIdx.x]);
-        part[(size_t)blockIdx.x * kFlowSums + threadIdx.x] = s;
-    }
+    block_sums<kFlowSums>(acc, part);
 }
 
 __global__ void __launch_bounds__(64) k_flow_final(const DD* __restrict__ part, int nblk, double* __restrict__ out) {
