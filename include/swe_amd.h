@@ -247,7 +247,10 @@ int swe_stream_bandwidth(unsigned long long bytes, int reps, double *read_gbs, d
  * ordered XCD-aware (1) or in dispatch order (0).  Kernels, in this order: k_compute_dt, k_alpha,
  * k_stage_lo, k_fct_lo, k_fct_rminus, k_fct_apply, k_ev_update.  swe_tune times every schedule
  * of each kernel the scheme uses on the handle's own mesh and state (the state and time are
- * restored afterwards) and keeps the fastest; results do not depend on the schedule. */
+ * restored afterwards) and keeps the fastest; results do not depend on the schedule.  The
+ * first-order and SGN stages (k_stage_lo) take their workgroup size and order from the same entry.
+ * SWE_TUNE_FAIL_AT=n (environment, for the test of the restore path) makes the n-th trial fail
+ * after its steps are queued; unset or 0 never. */
 #define SWE_NUM_KERNELS 7
 int swe_tune(swe_handle *h, int steps);
 int swe_get_schedule(swe_handle *h, int *blocks, int *remap);
@@ -269,7 +272,9 @@ int swe_use_graph(swe_handle *h, int on);
  * small to fill the GPU, where a launch costs more than its pass); other schemes keep the launch
  * sequence.  Results are the same bits either way.  swe_tune and swe_batch_tune time both forms
  * on the resident meshes and keep the faster.  A barrier wait is bounded: should one give up,
- * the instances that were stepping fail with SWE_EHIP.  SWE_EINVAL for an unknown form or a device
+ * the instances that were stepping fail with SWE_EHIP.  The bound is 2^21 polls, or the value of
+ * the environment variable SWE_BARRIER_SPINS where set (0 forces every barrier to give up: the test
+ * of that path).  SWE_EINVAL for an unknown form or a device
  * without cooperative launches.  No reference counterpart (a launch choice). */
 typedef enum swe_step_form { SWE_STEP_LAUNCHES = 0, SWE_STEP_PERSISTENT = 1 } swe_step_form;
 int swe_set_step_form(swe_handle *h, int form);
